@@ -1,0 +1,316 @@
+"""MI355X (gfx950) sparse-voxel-DAG build: Python mirror of the reference API.
+
+The reference exposes (Phyronnaz/DAG_Compression_Fast, src/DAGCompression.h:42-50)::
+
+    FCpuDag CreateDAG(const TStaticArray<FMortonCode, GPU>& Fragments, int32 Depth);
+    FCpuDag MergeDAGs(std::vector<FCpuDag>&& CpuDags);
+    TStaticArray<uint32, GPU> CreateFinalDAG(FCpuDag&& CpuDag);
+    void FreeAll();
+
+Here the same four operations are ``create_dag``, ``merge_dags``,
+``create_final_dag`` and ``free_all``, with the same argument meaning, running
+the hand-written HIP kernels of ``libsvdag.so`` through its C ABI.  Errors raise
+``SvdagError`` (the reference aborts).  Arrays are numpy on the host side;
+device memory is held in ``DeviceBuffer`` objects.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SvdagError, SVDAG_DEVICE, SVDAG_HOST
+
+__all__ = [
+    "Context", "DeviceBuffer", "Dag", "SvdagError", "create_dag", "create_dag_batched", "merge_dags",
+    "create_final_dag", "free_all", "default_context", "morton_encode", "gen_sphere_shells",
+    "gen_triangle_surfaces", "gen_value_noise", "gen_chunk_plan", "gen_chunk",
+]
+
+
+class DeviceBuffer:
+    """A hipMalloc'd device allocation."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _lib.hip_check(_lib.hip().hipMalloc(ctypes.byref(self.ptr), max(1, self.nbytes)), "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.dtype = arr.dtype
+        buf.count = arr.size
+        if arr.nbytes:
+            _lib.hip_check(_lib.hip().hipMemcpy(buf.ptr, arr.ctypes.data, arr.nbytes, _lib.HIP_H2D), "hipMemcpy")
+        return buf
+
+    @property
+    def address(self):
+        return self.ptr.value or 0
+
+    def to_numpy(self, dtype, count, offset_bytes=0):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _lib.hip_check(_lib.hip().hipMemcpy(out.ctypes.data, self.address + offset_bytes, out.nbytes,
+                                                _lib.HIP_D2H), "hipMemcpy")
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _lib.hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _copy_d2h(addr, dtype, count):
+    out = np.empty(count, dtype=dtype)
+    if out.nbytes:
+        _lib.hip_check(_lib.hip().hipMemcpy(out.ctypes.data, addr, out.nbytes, _lib.HIP_D2H), "hipMemcpy")
+    return out
+
+
+class Context:
+    """A build context: stream binding, scratch arena (the reference's FPool)."""
+
+    def __init__(self, stream=None):
+        self.handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().svdag_context_create(ctypes.byref(self.handle), stream))
+
+    def set_option(self, option, value):
+        _lib.check(_lib.lib().svdag_set_option(self.handle, option, int(value)))
+
+    def free_all(self):
+        _lib.check(_lib.lib().svdag_free_all(self.handle))
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            _lib.lib().svdag_context_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context()
+    return _default
+
+
+class Dag:
+    """A device- or host-resident DAG (mirror of FCpuDag) owned by Python."""
+
+    def __init__(self, raw, ctx):
+        self.raw = raw
+        self.ctx = ctx
+
+    @property
+    def num_levels(self):
+        return self.raw.num_levels
+
+    @property
+    def level_counts(self):
+        return [int(self.raw.levels[i].num_nodes) for i in range(self.raw.num_levels)]
+
+    @property
+    def num_leaves(self):
+        return int(self.raw.num_leaves)
+
+    @property
+    def pre_dedup(self):
+        return [int(self.raw.pre_dedup[i]) for i in range(self.raw.num_levels)]
+
+    @property
+    def leaf_positions(self):
+        return int(self.raw.leaf_positions)
+
+    @property
+    def hash_collisions(self):
+        return int(self.raw.hash_collisions)
+
+    @property
+    def total_words(self):
+        return 2 * self.num_leaves + sum(int(self.raw.level_words[i]) for i in range(self.raw.num_levels))
+
+    def to_numpy(self):
+        """Level arrays as numpy (the FCpuDag contents), same layout as the oracle's dict."""
+        r = self.raw
+        if r.location == SVDAG_DEVICE:
+            get = _copy_d2h
+        else:
+            def get(addr, dtype, count):
+                if count == 0:
+                    return np.zeros(0, dtype=dtype)
+                return np.ctypeslib.as_array(ctypes.cast(addr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                             shape=(count,)).copy()
+        levels = []
+        for i in range(r.num_levels):
+            lv = r.levels[i]
+            n = int(lv.num_nodes)
+            levels.append({
+                "masks": get(lv.masks, np.uint8, n),
+                "children": get(lv.children, np.uint32, 8 * n).reshape(n, 8),
+                "hashes": get(lv.hashes, np.uint64, n),
+            })
+        return {"levels": levels, "leaves": get(r.leaves, np.uint64, int(r.num_leaves)),
+                "hash_collisions": int(r.hash_collisions)}
+
+    def to_host(self):
+        """FCpuDag-like host copy (svdag_dag_to_host)."""
+        out = _lib.SvdagDag()
+        _lib.check(_lib.lib().svdag_dag_to_host(self.ctx.handle, ctypes.byref(self.raw), ctypes.byref(out)))
+        return Dag(out, self.ctx)
+
+    def to_device(self):
+        out = _lib.SvdagDag()
+        _lib.check(_lib.lib().svdag_dag_to_device(self.ctx.handle, ctypes.byref(self.raw), ctypes.byref(out)))
+        return Dag(out, self.ctx)
+
+    def free(self):
+        if self.raw is not None and self.ctx.handle is not None:
+            _lib.lib().svdag_free_dag(self.ctx.handle, ctypes.byref(self.raw))
+        self.raw = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _as_device_keys(fragments):
+    if isinstance(fragments, DeviceBuffer):
+        return fragments, int(fragments.count), fragments.dtype
+    arr = np.ascontiguousarray(fragments)
+    if arr.dtype not in (np.uint32, np.uint64):
+        arr = arr.astype(np.uint64)
+    return DeviceBuffer.from_numpy(arr), arr.size, arr.dtype
+
+
+def create_dag(fragments, depth, ctx=None):
+    """DAGCompression::CreateDAG (src/DAGCompression.cu:1003-1107).
+
+    fragments: unsorted Morton codes with duplicates, a numpy uint64/uint32 array
+    or a DeviceBuffer (with ``dtype`` and ``count``).  Returns a device-resident Dag.
+    """
+    ctx = ctx or default_context()
+    buf, n, dtype = _as_device_keys(fragments)
+    out = _lib.SvdagDag()
+    fn = _lib.lib().svdag_create_dag_u32 if dtype == np.uint32 else _lib.lib().svdag_create_dag
+    _lib.check(fn(ctx.handle, buf.ptr, n, int(depth), ctypes.byref(out)))
+    return Dag(out, ctx)
+
+
+def create_dag_batched(keys, offsets, chunk_depth, ctx=None):
+    """MergeDAGs(CreateDAG(chunk_i)) for chunks back to back in ``keys`` (chunk-local
+    codes), chunk i at keys[offsets[i]:offsets[i+1]], chunks in Morton order."""
+    ctx = ctx or default_context()
+    buf, n, dtype = _as_device_keys(keys)
+    if dtype != np.uint64:
+        raise SvdagError(1, "batched keys must be uint64")
+    offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    out = _lib.SvdagDag()
+    _lib.check(_lib.lib().svdag_create_dag_batched(
+        ctx.handle, buf.ptr, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(offs) - 1, int(chunk_depth),
+        ctypes.byref(out)))
+    return Dag(out, ctx)
+
+
+def merge_dags(dags, ctx=None):
+    """DAGCompression::MergeDAGs: 8^k DAGs of equal depth in Morton order -> one Dag."""
+    ctx = ctx or (dags[0].ctx if dags else default_context())
+    arr = (_lib.SvdagDag * len(dags))()
+    for i, d in enumerate(dags):
+        arr[i] = d.raw
+    out = _lib.SvdagDag()
+    _lib.check(_lib.lib().svdag_merge_dags(ctx.handle, arr, len(dags), ctypes.byref(out)))
+    return Dag(out, ctx)
+
+
+def create_final_dag(dag, ctx=None):
+    """DAGCompression::CreateFinalDAG -> numpy uint32 words of the flat layout."""
+    ctx = ctx or dag.ctx
+    d = dag if dag.raw.location == SVDAG_DEVICE else dag.to_device()
+    words = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    _lib.check(_lib.lib().svdag_create_final_dag(ctx.handle, ctypes.byref(d.raw), ctypes.byref(words),
+                                                 ctypes.byref(n)))
+    try:
+        return _copy_d2h(words.value or 0, np.uint32, int(n.value))
+    finally:
+        _lib.lib().svdag_free_words(ctx.handle, words)
+
+
+def free_all(ctx=None):
+    """DAGCompression::FreeAll: release the context's scratch arena."""
+    (ctx or default_context()).free_all()
+
+
+# --- synthetic inputs --------------------------------------------------------
+def morton_encode(x, y, z):
+    return int(_lib.lib().svdag_morton_encode(x, y, z))
+
+
+def _take_keys(rc, p, n):
+    _lib.check(rc)
+    try:
+        if n.value == 0:
+            return np.zeros(0, dtype=np.uint64)
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint64)), shape=(n.value,)).copy()
+    finally:
+        _lib.lib().svdag_gen_free(p)
+
+
+def gen_sphere_shells(seed, depth, count, rmin, rmax, dup_fraction, threads=8):
+    p, n = ctypes.c_void_p(), ctypes.c_uint64()
+    rc = _lib.lib().svdag_gen_sphere_shells(seed, depth, count, rmin, rmax, dup_fraction, threads, ctypes.byref(p),
+                                            ctypes.byref(n))
+    return _take_keys(rc, p, n)
+
+
+def gen_triangle_surfaces(seed, depth, count, mu, sigma, dup_fraction, threads=8):
+    p, n = ctypes.c_void_p(), ctypes.c_uint64()
+    rc = _lib.lib().svdag_gen_triangle_surfaces(seed, depth, count, mu, sigma, dup_fraction, threads,
+                                                ctypes.byref(p), ctypes.byref(n))
+    return _take_keys(rc, p, n)
+
+
+def gen_value_noise(seed, depth, spacing, sub_depth, sub_index, threads=8):
+    p, n = ctypes.c_void_p(), ctypes.c_uint64()
+    rc = _lib.lib().svdag_gen_value_noise(seed, depth, spacing, sub_depth, sub_index, threads, ctypes.byref(p),
+                                          ctypes.byref(n))
+    return _take_keys(rc, p, n)
+
+
+def gen_chunk_plan(seed, num_chunks, median=2e5, sigma=1.0, nmin=1000, nmax=4_000_000):
+    n = np.zeros(num_chunks, dtype=np.uint64)
+    u = np.zeros(num_chunks, dtype=np.uint64)
+    k = np.zeros(num_chunks, dtype=np.int32)
+    _lib.check(_lib.lib().svdag_gen_chunk_plan(seed, num_chunks, median, sigma, nmin, nmax,
+                                               n.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                               u.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                               k.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return n, u, k
+
+
+def gen_chunk(seed, chunk, depth, median=2e5, sigma=1.0, nmin=1000, nmax=4_000_000, capacity=None):
+    if capacity is None:
+        capacity = int(gen_chunk_plan(seed, chunk + 1, median, sigma, nmin, nmax)[0][chunk])
+    out = np.empty(capacity, dtype=np.uint64)
+    n = ctypes.c_uint64()
+    _lib.check(_lib.lib().svdag_gen_chunk(seed, chunk, depth, median, sigma, nmin, nmax, out.ctypes.data, capacity,
+                                          ctypes.byref(n)))
+    return out[: n.value]

@@ -1,0 +1,1076 @@
+// svdag_api.hip -- C-ABI driver of the gfx950 DAG build (include/svdag.h).
+//
+// Host orchestration of DAGCompression::CreateDAG / MergeDAGs / CreateFinalDAG
+// (src/DAGCompression.cu:1003-1107, 1447-1548, 1550-1610).  The reference
+// synchronises the host ~20 times per level (event timers, counts, frees,
+// src/DAGCompression.cu:252-268, 419, 457, 894); here every count lives in
+// device memory, kernels size themselves from it, and a build synchronises once.
+// Scratch comes from a per-context arena (the FPool counterpart,
+// src/DAGCompression.cu:32-269) sized by svdag_scratch_bytes; results live in
+// blocks of a per-context pool, owned by the returned svdag_dag until freed.
+#include "svdag_device.hpp"
+#include "svdag_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace svdag;
+
+struct svdag_context {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    char* arena = nullptr;
+    uint64_t arena_cap = 0;
+    // Result blocks (DAG storage, final words): reused across calls, returned
+    // by svdag_free_dag / svdag_free_words, released by svdag_free_all.
+    std::multimap<uint64_t, void*> free_blocks;
+    std::unordered_map<void*, uint64_t> block_size;
+    uint64_t pool_bytes = 0;
+    BuildCounters* h_counters = nullptr;  // pinned
+    int check_collisions = 0;
+    uint64_t budget = 0;
+    uint32_t load_shift = 1;
+    int host_threads = 1;
+    // Stage markers of the work in flight (bounded completion wait).
+    static constexpr int kMaxMarks = 64;
+    hipEvent_t marks[kMaxMarks] = {};
+    const char* mark_names[kMaxMarks] = {};
+    int num_marks = 0;
+    double wait_seconds = 120.0;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+    do {                                                                                                 \
+        const hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return set_error(SVDAG_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define TRY(expr)                    \
+    do {                             \
+        const int rc_ = (expr);      \
+        if (rc_ != 0) return rc_;    \
+    } while (0)
+
+// Records a named marker after the work enqueued so far.
+void mark(svdag_context* ctx, const char* name)
+{
+    if (ctx->num_marks >= svdag_context::kMaxMarks || !ctx->marks[ctx->num_marks]) return;
+    if (hipEventRecord(ctx->marks[ctx->num_marks], ctx->stream) == hipSuccess) ctx->mark_names[ctx->num_marks++] = name;
+}
+
+// Waits for the stream with a deadline instead of blocking without bound; on
+// expiry the error names the first stage whose marker has not completed.
+int finish(svdag_context* ctx, const char* what)
+{
+    const auto start = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(ctx->stream);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) {
+            ctx->num_marks = 0;
+            return set_error(SVDAG_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
+        }
+        const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        if (waited > ctx->wait_seconds) {
+            const char* stalled = "after the last marker";
+            for (int i = 0; i < ctx->num_marks; ++i)
+                if (hipEventQuery(ctx->marks[i]) != hipSuccess) {
+                    stalled = ctx->mark_names[i];
+                    break;
+                }
+            (void)hipGetLastError();
+            ctx->num_marks = 0;
+            return set_error(SVDAG_ERR_INTERNAL, "%s: device work did not complete within %.0f s (stalled in: %s)", what,
+                             ctx->wait_seconds, stalled);
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(waited < 0.01 ? 20 : 500));
+    }
+    ctx->num_marks = 0;
+    return 0;
+}
+
+constexpr uint64_t kAlign = 256;
+inline uint64_t align_up(uint64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct Carver {
+    uint64_t off = 0;
+    uint64_t take(uint64_t bytes)
+    {
+        const uint64_t o = off;
+        off += align_up(std::max<uint64_t>(bytes, 1));
+        return o;
+    }
+};
+
+inline uint64_t pow8_capped(int e, uint64_t cap)
+{
+    uint64_t v = 1;
+    for (int i = 0; i < e; ++i) {
+        if (v > cap / 8) return cap;
+        v *= 8;
+    }
+    return std::min(v, cap);
+}
+
+inline uint32_t table_lg(uint64_t count, uint32_t shift)
+{
+    return std::max(8u, ceil_log2_u64(std::max<uint64_t>(count, 1) << shift));
+}
+
+// Scratch layout of one build over n keys at `depth`.
+struct BuildLayout {
+    uint64_t total = 0;
+    uint64_t zero_begin = 0, zero_end = 0;
+    uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0, pack_status = 0, compact_status = 0;
+    uint64_t level_status[SVDAG_MAX_LEVELS][2] = {};
+    uint64_t gkeys = 0, keys_a = 0, keys_b = 0;
+    uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
+    uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
+    uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0;
+    uint64_t tkeys[2] = {}, trep[2] = {}, trank[2] = {};
+    uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
+    SortPlan key_plan{}, leaf_plan{};
+};
+
+BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched)
+{
+    BuildLayout L;
+    L.n = n;
+    L.leaf_bound = std::min<uint64_t>(n, pow8_capped(depth - 2, n));
+    L.level_bound = std::min<uint64_t>(L.leaf_bound, pow8_capped(depth - 3, n));
+    L.leaf_slots = 1ull << table_lg(L.leaf_bound, load_shift);
+    L.level_slots = (1ull << table_lg(L.level_bound, load_shift)) + kTableOverflow;
+    L.key_plan = make_sort_plan(3 * depth);
+    L.leaf_plan = make_sort_plan(64);
+    const int levels = depth - 2;
+    Carver c;
+    L.zero_begin = c.off;
+    L.counters = c.take(sizeof(BuildCounters));
+    L.key_sort_zero = c.take(4 * sort_zero_words(L.key_plan, n));
+    L.leaf_sort_zero = c.take(4 * sort_zero_words(L.leaf_plan, L.leaf_bound));
+    L.pack_status = c.take(8 * ((n + 4095) / 4096));
+    L.compact_status = c.take(8 * ((L.leaf_slots + 4095) / 4096));
+    for (int l = 0; l < levels; ++l) {
+        L.level_status[l][0] = c.take(8 * ((L.leaf_bound + 2047) / 2048));
+        L.level_status[l][1] = c.take(8 * ((L.level_slots + 2047) / 2048));
+    }
+    L.zero_end = c.off;
+    const int kb = batched ? 8 : key_bytes;
+    if (batched) L.gkeys = c.take(8 * n);
+    L.keys_a = c.take(uint64_t(kb) * n);
+    L.keys_b = c.take(uint64_t(kb) * n);
+    L.lp_code = c.take(8 * L.leaf_bound);
+    L.lp_mask = c.take(8 * L.leaf_bound);
+    L.lp_slot = c.take(4 * L.leaf_bound);
+    L.leaf_table = c.take(8 * L.leaf_slots);
+    L.leaf_rank = c.take(4 * L.leaf_slots);
+    for (int i = 0; i < 2; ++i) {
+        L.list_keys[i] = c.take(8 * L.leaf_bound);
+        L.list_slots[i] = c.take(4 * L.leaf_bound);
+    }
+    L.sorted_slots = c.take(4 * L.leaf_bound);
+    for (int i = 0; i < 2; ++i) {
+        L.codes[i] = c.take(8 * L.level_bound);
+        L.slots[i] = c.take(4 * L.level_bound);
+    }
+    L.masks = c.take(L.level_bound);
+    L.children = c.take(32 * L.level_bound);
+    for (int i = 0; i < 2; ++i) {
+        L.tkeys[i] = c.take(8 * L.level_slots);
+        L.trep[i] = c.take(4 * L.level_slots);
+        L.trank[i] = c.take(4 * L.level_slots);
+    }
+    L.total = c.off;
+    return L;
+}
+
+// Device result storage: per level hashes, children, masks at capacity
+// min(8^l, leaf bound); leaves at the leaf bound.
+struct DagStorage {
+    uint64_t total = 0;
+    uint64_t hashes[SVDAG_MAX_LEVELS] = {}, children[SVDAG_MAX_LEVELS] = {}, masks[SVDAG_MAX_LEVELS] = {};
+    uint64_t leaves = 0;
+};
+
+DagStorage plan_storage(int num_levels, const uint64_t* caps, uint64_t leaf_cap)
+{
+    DagStorage S;
+    Carver c;
+    for (int l = 0; l < num_levels; ++l) {
+        S.hashes[l] = c.take(8 * caps[l]);
+        S.children[l] = c.take(32 * caps[l]);
+        S.masks[l] = c.take(caps[l]);
+    }
+    S.leaves = c.take(8 * leaf_cap);
+    S.total = c.off;
+    return S;
+}
+
+int ensure_arena(svdag_context* ctx, uint64_t bytes)
+{
+    if (ctx->budget && bytes > ctx->budget)
+        return set_error(SVDAG_ERR_MEMORY, "build needs %llu scratch bytes, budget is %llu",
+                         (unsigned long long)bytes, (unsigned long long)ctx->budget);
+    if (bytes <= ctx->arena_cap) return 0;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->arena) HIP_TRY(hipFree(ctx->arena));
+    ctx->arena = nullptr;
+    ctx->arena_cap = 0;
+    const uint64_t cap = align_up(bytes + bytes / 8);
+    if (hipMalloc(reinterpret_cast<void**>(&ctx->arena), cap) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->arena = nullptr;
+        return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device scratch", (unsigned long long)cap);
+    }
+    ctx->arena_cap = cap;
+    return 0;
+}
+
+// Result allocation from the context's block pool: a free block of at least
+// the request (and at most twice it) is reused, otherwise a new one is made.
+int device_alloc(svdag_context* ctx, void** p, uint64_t bytes)
+{
+    *p = nullptr;
+    bytes = bytes <= (1ull << 20) ? align_up(std::max<uint64_t>(bytes, 1)) : (bytes + (1ull << 20) - 1) >> 20 << 20;
+    auto it = ctx->free_blocks.lower_bound(bytes);
+    if (it != ctx->free_blocks.end() && it->first <= 2 * bytes) {
+        *p = it->second;
+        ctx->free_blocks.erase(it);
+        return 0;
+    }
+    if (ctx->budget && ctx->arena_cap + ctx->pool_bytes + bytes > ctx->budget)
+        return set_error(SVDAG_ERR_MEMORY, "result block of %llu bytes exceeds the memory budget", (unsigned long long)bytes);
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device memory", (unsigned long long)bytes);
+    }
+    ctx->block_size[*p] = bytes;
+    ctx->pool_bytes += bytes;
+    return 0;
+}
+
+// Returns a result block to the pool (stream order keeps later reuse safe).
+// A block the pool does not know is freed directly.
+void device_release(svdag_context* ctx, void* p)
+{
+    if (!p) return;
+    const auto it = ctx->block_size.find(p);
+    if (it == ctx->block_size.end()) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(p);
+        return;
+    }
+    ctx->free_blocks.emplace(it->second, p);
+}
+
+void pool_trim(svdag_context* ctx)
+{
+    for (auto& kv : ctx->free_blocks) {
+        (void)hipFree(kv.second);
+        ctx->pool_bytes -= ctx->block_size[kv.second];
+        ctx->block_size.erase(kv.second);
+    }
+    ctx->free_blocks.clear();
+}
+
+void empty_dag(svdag_dag* out, int num_levels, int location)
+{
+    std::memset(out, 0, sizeof(*out));
+    out->num_levels = num_levels;
+    out->location = location;
+}
+
+struct BatchInfo {
+    const uint64_t* h_offsets;
+    uint64_t num_chunks;
+    int chunk_depth;
+};
+
+template <typename K>
+int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag* out, const BatchInfo* batch)
+{
+    hipStream_t st = ctx->stream;
+    const int levels = depth - 2;
+    empty_dag(out, levels, SVDAG_DEVICE);
+    out->num_keys = n;
+    if (n == 0) return 0;  // DAGCompression.cu:1010-1013
+
+    const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr);
+    const uint64_t offsets_bytes = batch ? align_up(8 * (batch->num_chunks + 1)) : 0;
+    TRY(ensure_arena(ctx, L.total + offsets_bytes));
+    char* A = ctx->arena;
+    BuildCounters* c = reinterpret_cast<BuildCounters*>(A + L.counters);
+    unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
+
+    uint64_t caps[SVDAG_MAX_LEVELS];
+    for (int l = 0; l < levels; ++l) caps[l] = pow8_capped(l, L.leaf_bound);
+    const DagStorage S = plan_storage(levels, caps, L.leaf_bound);
+    void* storage = nullptr;
+    TRY(device_alloc(ctx, &storage, S.total));
+    char* O = static_cast<char*>(storage);
+    auto fail = [&](int code) {
+        device_release(ctx, storage);
+        out->storage = nullptr;
+        return code;
+    };
+
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + L.zero_begin), (L.zero_end - L.zero_begin) / 8, 0);
+    launch_fill_u64(st, &c->n, 1, n);
+
+    // Sort keys on bits [0, 3*depth) (SortKeys, DAGCompression.cu:1024-1026).
+    int where;
+    const void* sorted;
+    if (batch) {
+        uint64_t* d_off = reinterpret_cast<uint64_t*>(A + L.total);
+        if (hipMemcpyAsync(d_off, batch->h_offsets, 8 * (batch->num_chunks + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(set_error(SVDAG_ERR_HIP, "offset upload failed"));
+        uint64_t* g = reinterpret_cast<uint64_t*>(A + L.gkeys);
+        launch_prefix_chunks<uint64_t>(st, reinterpret_cast<const uint64_t*>(d_keys), g, d_off, batch->num_chunks, n,
+                                       uint32_t(3 * batch->chunk_depth), c);
+        uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
+        uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
+        where = launch_sort<uint64_t>(st, L.key_plan, g, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
+                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), &c->tile_counters[0],
+                                      &c->key_or, err, 0);
+        sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
+    } else {
+        K* ka = reinterpret_cast<K*>(A + L.keys_a);
+        K* kb = reinterpret_cast<K*>(A + L.keys_b);
+        where = launch_sort<K>(st, L.key_plan, d_keys, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
+                               reinterpret_cast<uint32_t*>(A + L.key_sort_zero), &c->tile_counters[0], &c->key_or,
+                               err, 0);
+        sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
+    }
+    if (where < 0) return fail(set_error(SVDAG_ERR_INTERNAL, "sort plan failed"));
+    mark(ctx, "key sort");
+
+    uint64_t* lp_code = reinterpret_cast<uint64_t*>(A + L.lp_code);
+    uint64_t* lp_mask = reinterpret_cast<uint64_t*>(A + L.lp_mask);
+    uint32_t* lp_slot = reinterpret_cast<uint32_t*>(A + L.lp_slot);
+    const uint64_t code_mask = depth >= 22 ? ~0ull : ((1ull << (3 * depth)) - 1);
+    // Leaf ReduceByKey (DAGCompression.cu:1044-1068).
+    if (batch || sizeof(K) == 8)
+        launch_leaf_pack<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, lp_code, lp_mask,
+                                   reinterpret_cast<uint64_t*>(A + L.pack_status), &c->tile_counters[8], n, depth - 3,
+                                   code_mask);
+    else
+        launch_leaf_pack<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, lp_code, lp_mask,
+                                   reinterpret_cast<uint64_t*>(A + L.pack_status), &c->tile_counters[8], n, depth - 3,
+                                   code_mask);
+
+    mark(ctx, "leaf pack");
+    // SortLevel(Leaves) (DAGCompression.cu:1070-1071): dedup table, then order.
+    uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + L.leaf_table);
+    uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + L.leaf_rank);
+    uint32_t* leaf_lg = &c->table_log2[SVDAG_MAX_LEVELS];
+    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
+    launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, err);
+    uint64_t* lk0 = reinterpret_cast<uint64_t*>(A + L.list_keys[0]);
+    uint64_t* lk1 = reinterpret_cast<uint64_t*>(A + L.list_keys[1]);
+    uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + L.list_slots[0]);
+    uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + L.list_slots[1]);
+    uint32_t* sorted_slots = reinterpret_cast<uint32_t*>(A + L.sorted_slots);
+    launch_table_compact(st, leaf_table, leaf_lg, lk0, ls0, &c->unique_leaves,
+                         reinterpret_cast<uint64_t*>(A + L.compact_status), &c->tile_counters[9], L.leaf_slots, err);
+    uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
+    if (launch_sort<uint64_t>(st, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
+                              &c->unique_leaves, L.leaf_bound, reinterpret_cast<uint32_t*>(A + L.leaf_sort_zero),
+                              &c->tile_counters[10], nullptr, err, 0) < 0)
+        return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
+    launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
+    mark(ctx, "leaf dedup");
+
+    // Levels, bottom-up (DAGCompression.cu:1077-1092).
+    launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[0]), &c->pre[depth - 3], &c->table_log2[depth - 3],
+                       ctx->load_shift, kTableOverflow, L.level_slots);
+    for (int l = depth - 3; l >= 0; --l) {
+        const int step = depth - 3 - l;
+        const int cur = step & 1;
+        LevelBuildArgs a{};
+        if (l == depth - 3) {
+            a.child_codes = lp_code;
+            a.child_slots = lp_slot;
+            a.child_rank_of_slot = leaf_rank;
+            a.child_hashes = leaves_out;
+            a.child_count = &c->leaf_positions;
+        } else {
+            a.child_codes = reinterpret_cast<const uint64_t*>(A + L.codes[cur ^ 1]);
+            a.child_slots = reinterpret_cast<const uint32_t*>(A + L.slots[cur ^ 1]);
+            a.child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur ^ 1]);
+            a.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[l + 1]);
+            a.child_count = &c->pre[l + 1];
+        }
+        a.parent_codes = reinterpret_cast<uint64_t*>(A + L.codes[cur]);
+        a.parent_slots = reinterpret_cast<uint32_t*>(A + L.slots[cur]);
+        a.parent_masks = reinterpret_cast<uint8_t*>(A + L.masks);
+        a.parent_children = reinterpret_cast<uint32_t*>(A + L.children);
+        a.table_keys = reinterpret_cast<uint64_t*>(A + L.tkeys[cur]);
+        a.table_rep = reinterpret_cast<uint32_t*>(A + L.trep[cur]);
+        a.table_log2 = &c->table_log2[l];
+        a.status = reinterpret_cast<uint64_t*>(A + L.level_status[l][0]);
+        a.tile_counter = &c->tile_counters[20 + 2 * step];
+        a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
+        a.counters = c;
+        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_bound);
+        mark(ctx, "level build");
+        if (ctx->check_collisions)
+            launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
+                               &c->collisions, L.level_bound);
+        LevelExtractArgs e{};
+        e.table_keys = a.table_keys;
+        e.table_rep = a.table_rep;
+        e.table_log2 = a.table_log2;
+        e.rank_of_slot = reinterpret_cast<uint32_t*>(A + L.trank[cur]);
+        e.parent_masks = a.parent_masks;
+        e.parent_children = a.parent_children;
+        e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        e.status = reinterpret_cast<uint64_t*>(A + L.level_status[l][1]);
+        e.tile_counter = &c->tile_counters[21 + 2 * step];
+        e.q_count = &c->uniq[l];
+        e.w_count = &c->words[l];
+        e.err = err;
+        launch_level_extract(st, e, L.level_slots);
+        mark(ctx, "level extract");
+        if (l > 0)
+            launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[cur ^ 1]), &c->pre[l - 1],
+                               &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, L.level_slots);
+    }
+    {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(set_error(SVDAG_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e)));
+    }
+    if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed: %s", hipGetErrorString(hipGetLastError())));
+    mark(ctx, "counter read-back");
+    TRY(finish(ctx, "create_dag"));
+    const BuildCounters& h = *ctx->h_counters;
+    if (!batch && (h.key_or & ~code_mask))
+        return fail(set_error(SVDAG_ERR_ARGUMENT, "a key is >= 2^(3*depth) for depth %d", depth));
+    if (h.error & kErrKeyRange)
+        return fail(set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)"));
+    if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
+    if (h.error & kErrSpin) return fail(set_error(SVDAG_ERR_INTERNAL, "a device-side wait gave up"));
+    if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
+
+    out->storage = storage;
+    out->leaves = leaves_out;
+    out->num_leaves = h.unique_leaves;
+    out->leaf_positions = h.leaf_positions;
+    out->hash_collisions = h.collisions;
+    for (int l = 0; l < levels; ++l) {
+        out->levels[l].hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        out->levels[l].children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        out->levels[l].masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        out->levels[l].num_nodes = h.uniq[l];
+        out->level_words[l] = h.words[l];
+        out->pre_dedup[l] = h.pre[l];
+    }
+    return 0;
+}
+
+int check_ctx(svdag_context* ctx)
+{
+    if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return 0;
+}
+
+uint64_t dag_words(const svdag_dag* d)
+{
+    uint64_t w = 2 * d->num_leaves;
+    for (int l = 0; l < d->num_levels; ++l) w += d->level_words[l];
+    return w;
+}
+
+// MergeDAGs (src/DAGCompression.cu:1447-1548) over device-resident inputs.
+// Levels below the sub-DAG roots: every input's nodes of that level enter one
+// shared table (children remapped, hashes reused) and are extracted in hash
+// order.  Levels above: the non-empty sub-DAG roots are the children, keyed by
+// their chunk index (Morton order), and the ordinary level build runs on them.
+int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int k, svdag_dag* out)
+{
+    hipStream_t st = ctx->stream;
+    const int lc_levels = dags[0].num_levels;
+    const int levels = lc_levels + k;
+    empty_dag(out, levels, SVDAG_DEVICE);
+    std::vector<uint64_t> live;
+    for (uint64_t i = 0; i < count; ++i)
+        if (dags[i].num_leaves) live.push_back(i);
+    if (live.empty()) return 0;
+    const uint64_t nlive = live.size();
+
+    uint64_t total_leaves = 0;
+    std::vector<uint64_t> totals(lc_levels, 0);
+    for (uint64_t i : live) {
+        total_leaves += dags[i].num_leaves;
+        for (int l = 0; l < lc_levels; ++l) totals[l] += dags[i].levels[l].num_nodes;
+    }
+    uint64_t max_total = 0;
+    for (int l = 0; l < lc_levels; ++l) max_total = std::max(max_total, totals[l]);
+    max_total = std::max<uint64_t>(max_total, 1);
+    if (total_leaves >= (1ull << 30) || max_total >= (1ull << 30))
+        return set_error(SVDAG_ERR_CAPACITY, "merge input exceeds 2^30 nodes per level");
+
+    const uint32_t shift = ctx->load_shift;
+    const uint64_t leaf_slots = 1ull << table_lg(total_leaves, shift);
+    const uint64_t level_slots = (1ull << table_lg(std::max(max_total, nlive), shift)) + kTableOverflow;
+    const SortPlan leaf_plan = make_sort_plan(64);
+
+    Carver cv;
+    const uint64_t o_zero = cv.off;
+    const uint64_t o_counters = cv.take(sizeof(BuildCounters));
+    const uint64_t o_leaf_sort_zero = cv.take(4 * sort_zero_words(leaf_plan, total_leaves));
+    const uint64_t o_compact_status = cv.take(8 * ((leaf_slots + 4095) / 4096));
+    uint64_t o_status[SVDAG_MAX_LEVELS][2];
+    for (int l = 0; l < levels; ++l) {
+        o_status[l][0] = cv.take(8 * ((nlive + 2047) / 2048 + 1));
+        o_status[l][1] = cv.take(8 * ((level_slots + 2047) / 2048));
+    }
+    const uint64_t zero_end = cv.off;
+    const uint64_t o_sources = cv.take(sizeof(MergeSource) * nlive * (lc_levels + 1));
+    const uint64_t o_leaf_slot_of = cv.take(4 * total_leaves);
+    const uint64_t o_leaf_table = cv.take(8 * leaf_slots);
+    const uint64_t o_leaf_rank = cv.take(4 * leaf_slots);
+    uint64_t o_lk[2], o_ls[2];
+    for (int i = 0; i < 2; ++i) {
+        o_lk[i] = cv.take(8 * total_leaves);
+        o_ls[i] = cv.take(4 * total_leaves);
+    }
+    const uint64_t o_sorted_slots = cv.take(4 * total_leaves);
+    uint64_t o_slot_of[2];
+    for (int i = 0; i < 2; ++i) o_slot_of[i] = cv.take(4 * std::max(max_total, nlive));
+    const uint64_t o_masks = cv.take(max_total);
+    const uint64_t o_children = cv.take(32 * max_total);
+    uint64_t o_tk[2], o_trep[2], o_trank[2];
+    for (int i = 0; i < 2; ++i) {
+        o_tk[i] = cv.take(8 * level_slots);
+        o_trep[i] = cv.take(4 * level_slots);
+        o_trank[i] = cv.take(4 * level_slots);
+    }
+    uint64_t o_codes[2];
+    for (int i = 0; i < 2; ++i) o_codes[i] = cv.take(8 * nlive);
+    TRY(ensure_arena(ctx, cv.off));
+    char* A = ctx->arena;
+    BuildCounters* c = reinterpret_cast<BuildCounters*>(A + o_counters);
+    unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
+
+    uint64_t caps[SVDAG_MAX_LEVELS];
+    for (int l = 0; l < levels; ++l) caps[l] = l < k ? pow8_capped(l, nlive) : totals[l - k];
+    const DagStorage S = plan_storage(levels, caps, total_leaves);
+    void* storage = nullptr;
+    TRY(device_alloc(ctx, &storage, S.total));
+    char* O = static_cast<char*>(storage);
+    auto fail = [&](int code) {
+        device_release(ctx, storage);
+        return code;
+    };
+
+    // Source tables: block 0 = leaves, block 1 + l = level l of every live input.
+    std::vector<MergeSource> src(nlive * (lc_levels + 1));
+    for (int blk = 0; blk <= lc_levels; ++blk) {
+        uint64_t first = 0, child_first = 0;
+        for (uint64_t j = 0; j < nlive; ++j) {
+            const svdag_dag& d = dags[live[j]];
+            MergeSource& m = src[uint64_t(blk) * nlive + j];
+            if (blk == 0) {
+                m = MergeSource{nullptr, nullptr, d.leaves, first, d.num_leaves, 0};
+            } else {
+                const int l = blk - 1;
+                m = MergeSource{d.levels[l].masks, d.levels[l].children, d.levels[l].hashes, first,
+                                d.levels[l].num_nodes, child_first};
+                child_first += l + 1 < lc_levels ? d.levels[l + 1].num_nodes : d.num_leaves;
+            }
+            first += m.num;
+        }
+    }
+    MergeSource* d_src = reinterpret_cast<MergeSource*>(A + o_sources);
+    if (hipMemcpyAsync(d_src, src.data(), sizeof(MergeSource) * src.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "merge source upload failed"));
+    std::vector<uint64_t> codes(nlive);
+    for (uint64_t j = 0; j < nlive; ++j) codes[j] = live[j];
+    if (hipMemcpyAsync(A + o_codes[0], codes.data(), 8 * nlive, hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "chunk code upload failed"));
+    // The upload buffers must outlive the copies: finish them before returning.
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(set_error(SVDAG_ERR_HIP, "merge upload failed"));
+
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_zero), (zero_end - o_zero) / 8, 0);
+    launch_fill_u64(st, &c->leaf_positions, 1, total_leaves);
+    for (int l = k; l < levels; ++l) launch_fill_u64(st, &c->pre[l], 1, totals[l - k]);
+    if (k > 0) {
+        uint64_t groups = 0;
+        for (uint64_t j = 0; j < nlive; ++j)
+            if (j == 0 || (live[j] >> 3) != (live[j - 1] >> 3)) ++groups;
+        launch_fill_u64(st, &c->pre[k - 1], 1, groups);
+    }
+
+    // Leaves: shared hashed table, compact, sort, rank.
+    uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + o_leaf_table);
+    uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + o_leaf_rank);
+    uint32_t* leaf_slot_of = reinterpret_cast<uint32_t*>(A + o_leaf_slot_of);
+    uint32_t* leaf_lg = &c->table_log2[SVDAG_MAX_LEVELS];
+    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, shift, 0, leaf_slots);
+    launch_merge_insert(st, d_src, uint32_t(nlive), total_leaves, nullptr, nullptr, nullptr, nullptr, leaf_table,
+                        nullptr, leaf_lg, leaf_slot_of, c, 0);
+    uint64_t* lk0 = reinterpret_cast<uint64_t*>(A + o_lk[0]);
+    uint64_t* lk1 = reinterpret_cast<uint64_t*>(A + o_lk[1]);
+    uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + o_ls[0]);
+    uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + o_ls[1]);
+    uint32_t* sorted_slots = reinterpret_cast<uint32_t*>(A + o_sorted_slots);
+    launch_table_compact(st, leaf_table, leaf_lg, lk0, ls0, &c->unique_leaves,
+                         reinterpret_cast<uint64_t*>(A + o_compact_status), &c->tile_counters[9], leaf_slots, err);
+    uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
+    if (launch_sort<uint64_t>(st, leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots, &c->unique_leaves,
+                              total_leaves, reinterpret_cast<uint32_t*>(A + o_leaf_sort_zero), &c->tile_counters[10],
+                              nullptr, err, 0) < 0)
+        return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
+    launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, total_leaves);
+
+    const uint32_t* child_slot_of = leaf_slot_of;
+    const uint32_t* child_rank = leaf_rank;
+    uint8_t* masks = reinterpret_cast<uint8_t*>(A + o_masks);
+    uint32_t* children = reinterpret_cast<uint32_t*>(A + o_children);
+    int step = 0;
+    auto extract = [&](int l, int cur) {
+        LevelExtractArgs e{};
+        e.table_keys = reinterpret_cast<const uint64_t*>(A + o_tk[cur]);
+        e.table_rep = reinterpret_cast<const uint32_t*>(A + o_trep[cur]);
+        e.table_log2 = &c->table_log2[l];
+        e.rank_of_slot = reinterpret_cast<uint32_t*>(A + o_trank[cur]);
+        e.parent_masks = masks;
+        e.parent_children = children;
+        e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        e.status = reinterpret_cast<uint64_t*>(A + o_status[l][1]);
+        e.tile_counter = &c->tile_counters[21 + 2 * step];
+        e.q_count = &c->uniq[l];
+        e.w_count = &c->words[l];
+        e.err = err;
+        launch_level_extract(st, e, level_slots);
+    };
+    // Levels below the sub-DAG roots.
+    for (int lc = lc_levels - 1; lc >= 0; --lc, ++step) {
+        const int l = lc + k;
+        const int cur = step & 1;
+        uint64_t* tk = reinterpret_cast<uint64_t*>(A + o_tk[cur]);
+        uint32_t* slot_of = reinterpret_cast<uint32_t*>(A + o_slot_of[cur]);
+        launch_table_clear(st, tk, &c->pre[l], &c->table_log2[l], shift, kTableOverflow, level_slots);
+        launch_merge_insert(st, d_src + uint64_t(lc + 1) * nlive, uint32_t(nlive), totals[lc], child_slot_of,
+                            child_rank, masks, children, tk, reinterpret_cast<uint32_t*>(A + o_trep[cur]),
+                            &c->table_log2[l], slot_of, c, 1);
+        extract(l, cur);
+        child_slot_of = slot_of;
+        child_rank = reinterpret_cast<const uint32_t*>(A + o_trank[cur]);
+    }
+    // New top levels over the chunk roots (DAGCompression.cu:1505-1533).
+    for (int l = k - 1; l >= 0; --l, ++step) {
+        const int cur = step & 1;
+        const int first_top = l == k - 1;
+        LevelBuildArgs a{};
+        a.child_codes = reinterpret_cast<const uint64_t*>(A + o_codes[(k - 1 - l) & 1]);
+        a.child_slots = child_slot_of;
+        a.child_rank_of_slot = child_rank;
+        a.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[l + 1]);
+        a.child_count = &c->pre[l + 1];
+        a.parent_codes = reinterpret_cast<uint64_t*>(A + o_codes[(k - l) & 1]);
+        a.parent_slots = reinterpret_cast<uint32_t*>(A + o_slot_of[cur]);
+        a.parent_masks = masks;
+        a.parent_children = children;
+        a.table_keys = reinterpret_cast<uint64_t*>(A + o_tk[cur]);
+        a.table_rep = reinterpret_cast<uint32_t*>(A + o_trep[cur]);
+        a.table_log2 = &c->table_log2[l];
+        a.status = reinterpret_cast<uint64_t*>(A + o_status[l][0]);
+        a.tile_counter = &c->tile_counters[20 + 2 * step];
+        a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
+        a.counters = c;
+        (void)first_top;
+        launch_table_clear(st, a.table_keys, &c->pre[l], &c->table_log2[l], shift, kTableOverflow, level_slots);
+        launch_level_build(st, a, nlive);
+        extract(l, cur);
+        child_slot_of = a.parent_slots;
+        child_rank = reinterpret_cast<const uint32_t*>(A + o_trank[cur]);
+    }
+    {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(set_error(SVDAG_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e)));
+    }
+    if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed: %s", hipGetErrorString(hipGetLastError())));
+    mark(ctx, "counter read-back");
+    TRY(finish(ctx, "merge_dags"));
+    const BuildCounters& h = *ctx->h_counters;
+    if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
+    if (h.error & kErrSpin) return fail(set_error(SVDAG_ERR_INTERNAL, "a device-side wait gave up"));
+    if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "merged root level holds %llu nodes", (unsigned long long)h.uniq[0]));
+    out->storage = storage;
+    out->leaves = leaves_out;
+    out->num_leaves = h.unique_leaves;
+    for (int l = 0; l < levels; ++l) {
+        out->levels[l].hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        out->levels[l].children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        out->levels[l].masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        out->levels[l].num_nodes = h.uniq[l];
+        out->level_words[l] = h.words[l];
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int svdag_context_create(svdag_context** out, void* stream)
+{
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    auto* ctx = new svdag_context();
+    ctx->stream = static_cast<hipStream_t>(stream);
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        delete ctx;
+        return set_error(SVDAG_ERR_HIP, "no HIP device");
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_counters), sizeof(BuildCounters), hipHostMallocDefault) !=
+        hipSuccess) {
+        delete ctx;
+        return set_error(SVDAG_ERR_HIP, "cannot allocate pinned staging");
+    }
+    for (int i = 0; i < svdag_context::kMaxMarks; ++i)
+        if (hipEventCreateWithFlags(&ctx->marks[i], hipEventDisableTiming) != hipSuccess) ctx->marks[i] = nullptr;
+    (void)hipGetLastError();
+    *out = ctx;
+    return 0;
+}
+
+int svdag_context_destroy(svdag_context* ctx)
+{
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    for (auto& kv : ctx->block_size) (void)hipFree(kv.first);  // DAGs still alive become invalid
+    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+    for (auto& e : ctx->marks)
+        if (e) (void)hipEventDestroy(e);
+    delete ctx;
+    return 0;
+}
+
+int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
+{
+    if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "null context");
+    switch (option) {
+    case SVDAG_OPT_CHECK_COLLISIONS: ctx->check_collisions = value != 0; return 0;
+    case SVDAG_OPT_MEMORY_BUDGET: ctx->budget = uint64_t(std::max<int64_t>(0, value)); return 0;
+    case SVDAG_OPT_TABLE_LOAD_SHIFT:
+        if (value < 1 || value > 3) return set_error(SVDAG_ERR_ARGUMENT, "load shift must be 1..3");
+        ctx->load_shift = uint32_t(value);
+        return 0;
+    case SVDAG_OPT_HOST_THREADS: ctx->host_threads = int(std::max<int64_t>(1, value)); return 0;
+    case SVDAG_OPT_WAIT_SECONDS: ctx->wait_seconds = double(std::max<int64_t>(1, value)); return 0;
+    default: return set_error(SVDAG_ERR_ARGUMENT, "unknown option %d", option);
+    }
+}
+
+uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes)
+{
+    if (depth < 3 || depth > 21 || n == 0) return 0;
+    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false).total;
+}
+
+int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
+    if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
+    if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
+    return build<uint64_t>(ctx, d_keys, n, depth, out, nullptr);
+}
+
+int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    if (depth < 3 || depth > 10) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 10] for 32-bit keys", depth);
+    if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
+    if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
+    return build<uint32_t>(ctx, d_keys, n, depth, out, nullptr);
+}
+
+int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys, const uint64_t* offsets, uint64_t num_chunks,
+                             int32_t chunk_depth, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!out || !offsets) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    int extra = 0;
+    for (uint64_t c = num_chunks; c > 1; c /= 8) {
+        if (c % 8) return set_error(SVDAG_ERR_ARGUMENT, "chunk count %llu is not a power of 8", (unsigned long long)num_chunks);
+        ++extra;
+    }
+    if (num_chunks == 0) return set_error(SVDAG_ERR_ARGUMENT, "no chunks");
+    const int depth = chunk_depth + extra;
+    if (chunk_depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "bad chunk depth %d", chunk_depth);
+    for (uint64_t i = 0; i < num_chunks; ++i)
+        if (offsets[i + 1] < offsets[i]) return set_error(SVDAG_ERR_ARGUMENT, "offsets decrease at chunk %llu", (unsigned long long)i);
+    const uint64_t n = offsets[num_chunks] - offsets[0];
+    if (offsets[0] != 0) return set_error(SVDAG_ERR_ARGUMENT, "offsets[0] must be 0");
+    if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
+    if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
+    BatchInfo b{offsets, num_chunks, chunk_depth};
+    return build<uint64_t>(ctx, d_keys, n, depth, out, &b);
+}
+
+int svdag_merge_dags(svdag_context* ctx, const svdag_dag* dags, uint64_t count, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!dags || !out || count == 0) return set_error(SVDAG_ERR_ARGUMENT, "null argument or no DAG");
+    int k = 0;
+    for (uint64_t c = count; c > 1; c /= 8) {
+        if (c % 8) return set_error(SVDAG_ERR_ARGUMENT, "DAG count %llu is not a power of 8", (unsigned long long)count);
+        ++k;
+    }
+    const int lc_levels = dags[0].num_levels;
+    if (lc_levels < 1 || lc_levels + k > SVDAG_MAX_LEVELS || lc_levels + k + 2 > 21)
+        return set_error(SVDAG_ERR_ARGUMENT, "merged depth %d out of range", lc_levels + k + 2);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (dags[i].num_levels != lc_levels) return set_error(SVDAG_ERR_ARGUMENT, "DAG %llu has a different depth", (unsigned long long)i);
+        if (dags[i].num_leaves && dags[i].levels[0].num_nodes != 1)  // DAGCompression.cu:1476
+            return set_error(SVDAG_ERR_ARGUMENT, "DAG %llu has %llu root nodes", (unsigned long long)i,
+                             (unsigned long long)dags[i].levels[0].num_nodes);
+    }
+    // Device residency for the inputs (host inputs are uploaded temporarily).
+    std::vector<svdag_dag> dev(dags, dags + count);
+    std::vector<char> owned(count, 0);
+    int rc = 0;
+    for (uint64_t i = 0; i < count && rc == 0; ++i)
+        if (dags[i].location == SVDAG_HOST && dags[i].num_leaves) {
+            rc = svdag_dag_to_device(ctx, &dags[i], &dev[i]);
+            owned[i] = rc == 0;
+        }
+    if (rc == 0) rc = merge_device(ctx, dev.data(), count, k, out);
+    const std::string err = g_error;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (uint64_t i = 0; i < count; ++i)
+        if (owned[i]) svdag_free_dag(ctx, &dev[i]);
+    g_error = err;
+    return rc;
+}
+
+int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag, uint32_t** d_words, uint64_t* n_words)
+{
+    TRY(check_ctx(ctx));
+    if (!dag || !d_words || !n_words) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    *d_words = nullptr;
+    *n_words = 0;
+    if (dag->location != SVDAG_DEVICE) return set_error(SVDAG_ERR_ARGUMENT, "create_final_dag needs a device-resident DAG");
+    if (dag->num_levels < 1 || dag->num_levels > SVDAG_MAX_LEVELS) return set_error(SVDAG_ERR_ARGUMENT, "bad level count");
+    const uint64_t total_words = dag_words(dag);
+    if (total_words > 0xFFFFFFFFull)
+        return set_error(SVDAG_ERR_CAPACITY, "final DAG of %llu words does not fit uint32 offsets", (unsigned long long)total_words);
+    FinalArgs a{};
+    a.num_levels = dag->num_levels;
+    uint64_t nodes = 0;
+    for (int l = 0; l < dag->num_levels; ++l) {
+        a.levels[l].masks = dag->levels[l].masks;
+        a.levels[l].children = dag->levels[l].children;
+        a.levels[l].first_node = nodes;
+        a.levels[l].num_nodes = dag->levels[l].num_nodes;
+        nodes += dag->levels[l].num_nodes;
+    }
+    a.total_nodes = nodes;
+    a.leaves = dag->leaves;
+    a.num_leaves = dag->num_leaves;
+    a.leaves_start = total_words - 2 * dag->num_leaves;
+    Carver cv;
+    const uint64_t o_counter = cv.take(8);
+    const uint64_t o_err = cv.take(8);
+    const uint64_t o_status = cv.take(8 * ((nodes + 4095) / 4096 + 1));
+    const uint64_t zero_end = cv.off;
+    const uint64_t o_pos = cv.take(4 * std::max<uint64_t>(nodes, 1));
+    TRY(ensure_arena(ctx, cv.off));
+    void* words = nullptr;
+    TRY(device_alloc(ctx, &words, 4 * std::max<uint64_t>(total_words, 1)));
+    a.words = static_cast<uint32_t*>(words);
+    a.positions = reinterpret_cast<uint32_t*>(ctx->arena + o_pos);
+    a.status = reinterpret_cast<uint64_t*>(ctx->arena + o_status);
+    a.tile_counter = reinterpret_cast<uint32_t*>(ctx->arena + o_counter);
+    a.err = reinterpret_cast<unsigned long long*>(ctx->arena + o_err);
+    launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t*>(ctx->arena), zero_end / 8, 0);
+    launch_final_positions(ctx->stream, a);
+    launch_final_write(ctx->stream, a);
+    const hipError_t e = hipGetLastError();
+    mark(ctx, "final layout");
+    if (e != hipSuccess ||
+        hipMemcpyAsync(&ctx->h_counters->error, a.err, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+        device_release(ctx, words);
+        return set_error(SVDAG_ERR_HIP, "final layout failed: %s", hipGetErrorString(e));
+    }
+    if (const int rc = finish(ctx, "create_final_dag")) {
+        device_release(ctx, words);
+        return rc;
+    }
+    if (ctx->h_counters->error) {
+        device_release(ctx, words);
+        return set_error(SVDAG_ERR_INTERNAL, "final layout: a device-side wait gave up");
+    }
+    *d_words = static_cast<uint32_t*>(words);
+    *n_words = total_words;
+    return 0;
+}
+
+int svdag_dag_to_host(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!dag || !out) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    uint64_t total = 8 * dag->num_leaves + 8;
+    for (int l = 0; l < dag->num_levels; ++l) total += 41 * dag->levels[l].num_nodes + 32;
+    char* mem = static_cast<char*>(std::malloc(total));
+    if (!mem) return set_error(SVDAG_ERR_MEMORY, "host allocation failed");
+    *out = *dag;
+    out->location = SVDAG_HOST;
+    out->storage = mem;
+    const hipMemcpyKind kind = dag->location == SVDAG_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    char* p = mem;
+    auto copy = [&](void* src, uint64_t bytes) -> void* {
+        void* dst = p;
+        if (bytes) (void)hipMemcpyAsync(dst, src, bytes, kind, ctx->stream);
+        p += (bytes + 7) / 8 * 8;
+        return dst;
+    };
+    for (int l = 0; l < dag->num_levels; ++l) {
+        const uint64_t q = dag->levels[l].num_nodes;
+        out->levels[l].hashes = static_cast<uint64_t*>(copy(dag->levels[l].hashes, 8 * q));
+        out->levels[l].children = static_cast<uint32_t*>(copy(dag->levels[l].children, 32 * q));
+        out->levels[l].masks = static_cast<uint8_t*>(copy(dag->levels[l].masks, q));
+    }
+    out->leaves = static_cast<uint64_t*>(copy(dag->leaves, 8 * dag->num_leaves));
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        std::free(mem);
+        out->storage = nullptr;
+        return set_error(SVDAG_ERR_HIP, "copy to host failed: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int svdag_dag_to_device(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!dag || !out) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    if (dag->num_levels < 1 || dag->num_levels > SVDAG_MAX_LEVELS) return set_error(SVDAG_ERR_ARGUMENT, "bad level count");
+    if (dag->location == SVDAG_HOST) {
+        // Host DAGs come from callers: check every child index before any kernel reads through it.
+        for (int l = 0; l < dag->num_levels; ++l) {
+            const svdag_level& in = dag->levels[l];
+            const uint64_t below = l + 1 < dag->num_levels ? dag->levels[l + 1].num_nodes : dag->num_leaves;
+            for (uint64_t i = 0; i < in.num_nodes; ++i)
+                for (int s = 0; s < 8; ++s) {
+                    const uint32_t c = in.children[8 * i + s];
+                    const bool set = (in.masks[i] >> s) & 1;
+                    if (set != (c != kEmptyChild) || (set && c >= below))
+                        return set_error(SVDAG_ERR_ARGUMENT, "level %d node %llu slot %d: child %u inconsistent", l,
+                                         (unsigned long long)i, s, c);
+                }
+        }
+    }
+    uint64_t caps[SVDAG_MAX_LEVELS];
+    for (int l = 0; l < dag->num_levels; ++l) caps[l] = dag->levels[l].num_nodes;
+    const DagStorage S = plan_storage(dag->num_levels, caps, dag->num_leaves);
+    void* storage = nullptr;
+    TRY(device_alloc(ctx, &storage, S.total));
+    char* O = static_cast<char*>(storage);
+    *out = *dag;
+    out->location = SVDAG_DEVICE;
+    out->storage = storage;
+    const hipMemcpyKind kind = dag->location == SVDAG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int l = 0; l < dag->num_levels; ++l) {
+        const svdag_level& in = dag->levels[l];
+        const uint64_t q = in.num_nodes;
+        out->levels[l].hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        out->levels[l].children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        out->levels[l].masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        if (q) {
+            (void)hipMemcpyAsync(out->levels[l].hashes, in.hashes, 8 * q, kind, ctx->stream);
+            (void)hipMemcpyAsync(out->levels[l].children, in.children, 32 * q, kind, ctx->stream);
+            (void)hipMemcpyAsync(out->levels[l].masks, in.masks, q, kind, ctx->stream);
+        }
+        if (dag->location == SVDAG_HOST) {  // level words from the masks
+            uint64_t w = 0;
+            for (uint64_t i = 0; i < q; ++i) w += 1 + __builtin_popcount(in.masks[i]);
+            out->level_words[l] = w;
+        }
+    }
+    out->leaves = reinterpret_cast<uint64_t*>(O + S.leaves);
+    if (dag->num_leaves) (void)hipMemcpyAsync(out->leaves, dag->leaves, 8 * dag->num_leaves, kind, ctx->stream);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        device_release(ctx, storage);
+        out->storage = nullptr;
+        return set_error(SVDAG_ERR_HIP, "copy to device failed: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int svdag_free_dag(svdag_context* ctx, svdag_dag* dag)
+{
+    if (!dag) return 0;
+    if (dag->storage) {
+        if (dag->location == SVDAG_DEVICE) {
+            if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "device DAG needs its context to be freed");
+            (void)hipSetDevice(ctx->device);
+            device_release(ctx, dag->storage);
+        } else {
+            std::free(dag->storage);
+        }
+    }
+    std::memset(dag, 0, sizeof(*dag));
+    return 0;
+}
+
+int svdag_free_words(svdag_context* ctx, uint32_t* d_words)
+{
+    TRY(check_ctx(ctx));
+    device_release(ctx, d_words);
+    return 0;
+}
+
+int svdag_free_all(svdag_context* ctx)
+{
+    TRY(check_ctx(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->arena) HIP_TRY(hipFree(ctx->arena));
+    ctx->arena = nullptr;
+    ctx->arena_cap = 0;
+    pool_trim(ctx);
+    return 0;
+}
+
+const char* svdag_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

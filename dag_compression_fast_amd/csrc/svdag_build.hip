@@ -1,0 +1,718 @@
+// svdag_build.hip -- gfx950 kernels of the bottom-up DAG build.
+//
+// Reference path (src/DAGCompression.cu):
+//   SelectUnique + leaf ReduceByKey  :1035-1068  -> k_leaf_pack (one pass over
+//       the sorted keys: duplicates OR into the same 4^3 mask, so no separate
+//       unique step is needed)
+//   SortLevel(Leaves)               :1070-1071  -> k_table_clear / k_leaf_insert
+//       (hash table keyed by the mask) + k_table_compact + a key/value radix sort
+//       of the distinct masks + k_rank_of_slot
+//   ExtractLevel + ComputeHashes    :916-974, :827-833 -> k_level_build (parent
+//       masks, child ranks, Murmur hash, insert into the level's dedup table)
+//   SortLevel(level)                :864-914    -> k_level_extract (ordered
+//       linear-probing table: home slot = top bits of the hash, so sorting each
+//       probe cluster orders the whole table; rank = occupied slots before it)
+//   CreateFinalDAG / WriteTo        :834-861, :1550-1610 -> k_final_positions +
+//       k_final_write
+// Every kernel takes its element count from device memory and walks tiles from
+// an atomic counter, so the whole build runs without host round trips.
+#include "svdag_device.hpp"
+#include "svdag_internal.hpp"
+
+#include <algorithm>
+
+namespace svdag {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__host__ int persistent_grid(uint64_t tiles, int per_cu)
+{
+    const uint64_t cap = uint64_t(device_cu_count()) * per_cu;
+    return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
+}
+
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t count, uint64_t value)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+// Chunk-local keys -> global keys: chunk index (Morton order of the sub-volumes,
+// src/Utils/Aabb.h:83-105) above the chunk's 3*chunk_depth bits.
+template <typename K>
+__global__ void k_prefix_chunks(const K* __restrict__ in, uint64_t* __restrict__ out, const uint64_t* __restrict__ offsets,
+                                uint64_t num_chunks, uint64_t n, uint32_t chunk_bits, BuildCounters* __restrict__ c)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    bool bad = false;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint64_t lo = 0, hi = num_chunks;  // offsets[lo] <= i < offsets[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (offsets[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t k = uint64_t(in[i]);
+        bad |= (k >> chunk_bits) != 0;
+        out[i] = (lo << chunk_bits) | k;
+    }
+    if (bad) atomicOr((unsigned long long*)&c->error, (unsigned long long)kErrKeyRange);
+}
+
+// ---------------------------------------------------------------------------
+// Leaf pack: sorted keys -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block.
+// A run belongs to the thread holding its first key and may extend past the
+// thread's (and the tile's) range; duplicates OR harmlessly.
+// ---------------------------------------------------------------------------
+constexpr int kPackIpt = 16;
+constexpr int kPackTile = kThreads * kPackIpt;
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ sorted, BuildCounters* __restrict__ c,
+                                                        uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask,
+                                                        uint64_t* __restrict__ status, uint32_t* __restrict__ tile_counter,
+                                                        uint64_t* __restrict__ next_count, uint64_t code_mask)
+{
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_red[kThreads / kWave];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint64_t n = c->n;
+    const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
+    for (;;) {
+        const uint32_t tile = acquire_tile(tile_counter, &s_tile);
+        if (tile >= ntiles) break;
+        const uint64_t first = uint64_t(tile) * kPackTile + uint64_t(threadIdx.x) * kPackIpt;
+        K k[kPackIpt];
+#pragma unroll
+        for (int j = 0; j < kPackIpt; ++j) k[j] = first + j < n ? K(sorted[first + j] & code_mask) : K(0);
+        const uint64_t before = (first > 0 && first <= n) ? (uint64_t(sorted[first - 1]) & code_mask) : ~0ull;
+        uint32_t starts = 0, gcount = 0;
+#pragma unroll
+        for (int j = 0; j < kPackIpt; ++j) {
+            const uint64_t cur = uint64_t(k[j]);
+            const uint64_t prev = j ? uint64_t(k[j - 1]) : before;
+            const bool head = first + j == 0;
+            if (first + j < n && (head || (cur >> 6) != (prev >> 6))) {
+                starts |= 1u << j;
+                gcount += (head || (cur >> 9) != (prev >> 9)) ? 1u : 0u;
+            }
+        }
+        uint32_t tile_total;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, tile_total);
+        const uint64_t gsum = block_reduce_sum<kThreads>(gcount, s_red);
+        uint64_t excl;
+        if (!block_lookback(status, tile, tile_total, (unsigned long long*)&c->error, &s_excl, excl)) continue;
+        if (threadIdx.x == 0) {
+            atomicAdd((unsigned long long*)&c->leaf_positions, (unsigned long long)tile_total);
+            atomicAdd((unsigned long long*)next_count, (unsigned long long)gsum);
+        }
+        uint64_t q = excl + local;
+        bool open = false;
+        uint64_t run_code = 0, run_mask = 0;
+#pragma unroll
+        for (int j = 0; j < kPackIpt; ++j) {
+            if ((starts >> j) & 1u) {
+                if (open) {
+                    lp_code[q] = run_code;
+                    lp_mask[q] = run_mask;
+                    ++q;
+                }
+                open = true;
+                run_code = uint64_t(k[j]) >> 6;
+                run_mask = 0;
+            }
+            if (open && first + j < n) run_mask |= 1ull << (uint64_t(k[j]) & 63);
+        }
+        if (open) {
+            for (uint64_t i = first + kPackIpt; i < n; ++i) {
+                const uint64_t kk = uint64_t(sorted[i]) & code_mask;
+                if ((kk >> 6) != run_code) break;
+                run_mask |= 1ull << (kk & 63);
+            }
+            lp_code[q] = run_code;
+            lp_mask[q] = run_mask;
+        }
+    }
+}
+
+// Clears slots [0, 2^lg + overflow) with lg = max(8, ceil(log2(count << load_shift))).
+__global__ void k_table_clear(uint64_t* __restrict__ keys, const uint64_t* __restrict__ count_ptr,
+                              uint32_t* __restrict__ log2_out, uint32_t load_shift, uint32_t overflow)
+{
+    const uint64_t cnt = *count_ptr;
+    const uint32_t need = ceil_log2_u64((cnt ? cnt : 1ull) << load_shift);
+    const uint32_t lg = need > 8u ? need : 8u;
+    const uint64_t total = (1ull << lg) + overflow;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *log2_out = lg;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) keys[i] = 0;
+}
+
+// Leaf dedup table: key = mask (never 0), home = fmix64(mask), wrap-around probing.
+// Inserts key h (never 0) into a wrap-around table of 2^lg slots; a probe never
+// exceeds the table size, and a full table reports kErrTableOverflow.
+__device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uint32_t lg, uint64_t h,
+                                                  unsigned long long* err)
+{
+    const uint64_t wrap = (1ull << lg) - 1;
+    uint64_t slot = fmix64(h) >> (64 - lg);
+    for (uint64_t probe = 0; probe <= wrap; ++probe, slot = (slot + 1) & wrap) {
+        const uint64_t cur = atomic_load_relaxed(&tk[slot]);
+        if (cur == h) return uint32_t(slot);
+        if (cur == 0) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
+            if (prev == 0 || prev == h) return uint32_t(slot);
+        }
+    }
+    atomicOr(err, (unsigned long long)kErrTableOverflow);
+    return 0;
+}
+
+__global__ void k_leaf_insert(const uint64_t* __restrict__ lp_mask, const uint64_t* __restrict__ count_ptr,
+                              uint64_t* __restrict__ tk, const uint32_t* __restrict__ log2_ptr,
+                              uint32_t* __restrict__ lp_slot, unsigned long long* __restrict__ err)
+{
+    const uint64_t n = *count_ptr;
+    const uint32_t lg = *log2_ptr;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
+        lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], err);
+}
+
+// Occupied slots of a table, compacted in slot order.
+constexpr int kCompactIpt = 16;
+constexpr int kCompactTile = kThreads * kCompactIpt;
+
+__global__ __launch_bounds__(kThreads) void k_table_compact(const uint64_t* __restrict__ tk,
+                                                            const uint32_t* __restrict__ log2_ptr,
+                                                            uint64_t* __restrict__ list_keys,
+                                                            uint32_t* __restrict__ list_slots,
+                                                            uint64_t* __restrict__ count_out,
+                                                            uint64_t* __restrict__ status,
+                                                            uint32_t* __restrict__ tile_counter,
+                                                            unsigned long long* __restrict__ err)
+{
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint64_t slots = 1ull << *log2_ptr;
+    const uint64_t ntiles = (slots + kCompactTile - 1) / kCompactTile;
+    for (;;) {
+        const uint32_t tile = acquire_tile(tile_counter, &s_tile);
+        if (tile >= ntiles) break;
+        const uint64_t first = uint64_t(tile) * kCompactTile + uint64_t(threadIdx.x) * kCompactIpt;
+        uint64_t key[kCompactIpt];
+        uint32_t occ = 0;
+#pragma unroll
+        for (int j = 0; j < kCompactIpt; ++j) {
+            key[j] = first + j < slots ? tk[first + j] : 0;
+            occ |= (key[j] != 0 ? 1u : 0u) << j;
+        }
+        uint32_t tile_total;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, tile_total);
+        uint64_t excl;
+        if (!block_lookback(status, tile, tile_total, err, &s_excl, excl)) continue;
+        if (threadIdx.x == 0) atomicAdd((unsigned long long*)count_out, (unsigned long long)tile_total);
+        uint64_t r = excl + local;
+#pragma unroll
+        for (int j = 0; j < kCompactIpt; ++j)
+            if ((occ >> j) & 1u) {
+                list_keys[r] = key[j];
+                list_slots[r] = uint32_t(first + j);
+                ++r;
+            }
+    }
+}
+
+__global__ void k_rank_of_slot(const uint32_t* __restrict__ sorted_slots, const uint64_t* __restrict__ count_ptr,
+                               uint32_t* __restrict__ rank_of_slot)
+{
+    const uint64_t n = *count_ptr;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < n; r += stride)
+        rank_of_slot[sorted_slots[r]] = uint32_t(r);
+}
+
+// Ordered-table insert: home = top lg bits of the hash, linear probing forward
+// into an overflow tail (no wrap), so every key sits at or after its home slot.
+__device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, uint32_t* __restrict__ rep, uint32_t lg,
+                                                   uint64_t limit, uint64_t h, uint32_t p, BuildCounters* c)
+{
+    uint64_t slot = h >> (64 - lg);
+    for (; slot < limit; ++slot) {
+        const uint64_t cur = atomic_load_relaxed(&tk[slot]);
+        if (cur == h) return uint32_t(slot);
+        if (cur == 0) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
+            if (prev == 0) {
+                rep[slot] = p;
+                return uint32_t(slot);
+            }
+            if (prev == h) return uint32_t(slot);
+        }
+    }
+    atomicOr((unsigned long long*)&c->error, (unsigned long long)kErrTableOverflow);
+    return uint32_t(limit - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Level build: children (Morton-sorted codes of the level below, their table
+// slots) -> parents.  A parent belongs to the thread holding its first child and
+// has at most 8 children, so a run extends at most 7 keys past the thread.
+// ---------------------------------------------------------------------------
+constexpr int kBuildIpt = 8;
+constexpr int kBuildTile = kThreads * kBuildIpt;
+
+struct ParentAcc {
+    uint64_t code;
+    uint32_t mask;
+    uint32_t child[8];
+    uint64_t hash[8];
+};
+
+__device__ __forceinline__ void acc_reset(ParentAcc& a, uint64_t code)
+{
+    a.code = code;
+    a.mask = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        a.child[s] = kEmptyChild;
+        a.hash[s] = 0;
+    }
+}
+
+__device__ __forceinline__ void acc_add(ParentAcc& a, uint32_t slot, uint32_t rank, uint64_t hash)
+{
+    a.mask |= 1u << slot;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        a.child[s] = slot == uint32_t(s) ? rank : a.child[s];
+        a.hash[s] = slot == uint32_t(s) ? hash : a.hash[s];
+    }
+}
+
+__device__ __forceinline__ void acc_emit(const ParentAcc& a, uint64_t p, const LevelBuildArgs& g, uint32_t lg,
+                                         uint64_t limit)
+{
+    const uint64_t h = hash_children(a.hash);
+    g.parent_codes[p] = a.code;
+    g.parent_masks[p] = uint8_t(a.mask);
+    uint4* dst = reinterpret_cast<uint4*>(g.parent_children + 8 * p);
+    dst[0] = make_uint4(a.child[0], a.child[1], a.child[2], a.child[3]);
+    dst[1] = make_uint4(a.child[4], a.child[5], a.child[6], a.child[7]);
+    g.parent_slots[p] = ordered_insert(g.table_keys, g.table_rep, lg, limit, h, uint32_t(p), g.counters);
+}
+
+__global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
+{
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_red[kThreads / kWave];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint64_t n = *g.child_count;
+    const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
+    const uint32_t lg = *g.table_log2;
+    const uint64_t limit = (1ull << lg) + kTableOverflow;
+    for (;;) {
+        const uint32_t tile = acquire_tile(g.tile_counter, &s_tile);
+        if (tile >= ntiles) break;
+        const uint64_t first = uint64_t(tile) * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt;
+        uint64_t code[kBuildIpt];
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) code[j] = first + j < n ? g.child_codes[first + j] : 0;
+        const uint64_t before = (first > 0 && first <= n) ? g.child_codes[first - 1] : ~0ull;
+        uint32_t starts = 0, gcount = 0;
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) {
+            const uint64_t prev = j ? code[j - 1] : before;
+            const bool head = first + j == 0;
+            if (first + j < n && (head || (code[j] >> 3) != (prev >> 3))) {
+                starts |= 1u << j;
+                gcount += (head || (code[j] >> 6) != (prev >> 6)) ? 1u : 0u;
+            }
+        }
+        uint32_t tile_total;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, tile_total);
+        const uint64_t gsum = block_reduce_sum<kThreads>(gcount, s_red);
+        uint64_t excl;
+        if (!block_lookback(g.status, tile, tile_total, (unsigned long long*)&g.counters->error, &s_excl, excl)) continue;
+        if (threadIdx.x == 0 && g.next_count) atomicAdd((unsigned long long*)g.next_count, (unsigned long long)gsum);
+
+        uint64_t p = excl + local;
+        bool open = false;
+        ParentAcc acc;
+        acc_reset(acc, 0);
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) {
+            if ((starts >> j) & 1u) {
+                if (open) {
+                    acc_emit(acc, p, g, lg, limit);
+                    ++p;
+                }
+                open = true;
+                acc_reset(acc, code[j] >> 3);
+            }
+            if (open && first + j < n) {
+                const uint32_t rank = g.child_rank_of_slot[g.child_slots[first + j]];
+                acc_add(acc, uint32_t(code[j] & 7), rank, g.child_hashes[rank]);
+            }
+        }
+        if (open) {
+            for (uint64_t i = first + kBuildIpt; i < n; ++i) {
+                const uint64_t cc = g.child_codes[i];
+                if ((cc >> 3) != acc.code) break;
+                const uint32_t rank = g.child_rank_of_slot[g.child_slots[i]];
+                acc_add(acc, uint32_t(cc & 7), rank, g.child_hashes[rank]);
+            }
+            acc_emit(acc, p, g, lg, limit);
+        }
+    }
+}
+
+// Exact-content check of every parent against its table representative.
+__global__ void k_level_check(const uint32_t* __restrict__ parent_slots, const uint32_t* __restrict__ table_rep,
+                              const uint8_t* __restrict__ masks, const uint32_t* __restrict__ children,
+                              const uint64_t* __restrict__ count_ptr, uint64_t* __restrict__ collisions)
+{
+    const uint64_t n = *count_ptr;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint32_t bad = 0;
+    for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const uint32_t r = table_rep[parent_slots[p]];
+        bool diff = masks[p] != masks[r];
+        for (int s = 0; s < 8; ++s) diff |= children[8 * p + s] != children[8 * uint64_t(r) + s];
+        bad += diff ? 1u : 0u;
+    }
+    if (bad) atomicAdd((unsigned long long*)collisions, (unsigned long long)bad);
+}
+
+// ---------------------------------------------------------------------------
+// Level extract: ordered table -> the level's arrays in ascending hash order.
+// For occupied slot k in probe cluster [js, je]:
+//   rank(k) = (#occupied slots before k) - (k - js) + #{keys in cluster < key(k)}
+// Slots are read through an LDS window with a halo; clusters reaching past it
+// fall back to global loads.
+// ---------------------------------------------------------------------------
+constexpr int kExtractIpt = 8;
+constexpr int kExtractTile = kThreads * kExtractIpt;
+constexpr int kHalo = 256;
+
+__global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
+{
+    __shared__ uint64_t s_win[kExtractTile + 2 * kHalo];
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_red[kThreads / kWave];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint32_t lg = *g.table_log2;
+    const uint64_t total = (1ull << lg) + kTableOverflow;
+    const uint64_t ntiles = (total + kExtractTile - 1) / kExtractTile;
+    for (;;) {
+        const uint32_t tile = acquire_tile(g.tile_counter, &s_tile);
+        if (tile >= ntiles) break;
+        const int64_t tile_base = int64_t(tile) * kExtractTile;
+        const int64_t win_lo = tile_base - kHalo;
+        const int64_t win_hi = tile_base + kExtractTile + kHalo;
+        for (int i = threadIdx.x; i < kExtractTile + 2 * kHalo; i += kThreads) {
+            const int64_t s = win_lo + i;
+            s_win[i] = (s >= 0 && s < int64_t(total)) ? g.table_keys[s] : 0;
+        }
+        __syncthreads();
+        auto key_at = [&](int64_t s) -> uint64_t {
+            if (s < 0 || s >= int64_t(total)) return 0;
+            if (s >= win_lo && s < win_hi) return s_win[s - win_lo];
+            return g.table_keys[s];
+        };
+        const int64_t first = tile_base + int64_t(threadIdx.x) * kExtractIpt;
+        uint32_t occ = 0;
+#pragma unroll
+        for (int j = 0; j < kExtractIpt; ++j) occ |= (s_win[first + j - win_lo] != 0 ? 1u : 0u) << j;
+        uint32_t tile_total;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, tile_total);
+        uint64_t excl;
+        const bool ok = block_lookback(g.status, tile, tile_total, g.err, &s_excl, excl);
+        uint64_t before = excl + local;  // occupied slots before the current one
+        uint64_t wsum = 0;
+#pragma unroll
+        for (int j = 0; j < kExtractIpt; ++j) {
+            if (ok && ((occ >> j) & 1u)) {
+                const int64_t k = first + j;
+                const uint64_t key = s_win[k - win_lo];
+                int64_t js = k;
+                while (key_at(js - 1) != 0) --js;
+                int64_t je = k;
+                while (key_at(je + 1) != 0) ++je;
+                uint64_t lt = 0;
+                for (int64_t x = js; x <= je; ++x) lt += key_at(x) < key ? 1 : 0;
+                const uint64_t rank = before - uint64_t(k - js) + lt;
+                g.rank_of_slot[k] = uint32_t(rank);
+                g.out_hashes[rank] = key;
+                const uint32_t rep = g.table_rep[k];
+                const uint8_t m = g.parent_masks[rep];
+                g.out_masks[rank] = m;
+                const uint4* src = reinterpret_cast<const uint4*>(g.parent_children + 8 * uint64_t(rep));
+                uint4* dst = reinterpret_cast<uint4*>(g.out_children + 8 * rank);
+                dst[0] = src[0];
+                dst[1] = src[1];
+                wsum += 1 + __popc(uint32_t(m));
+                ++before;
+            }
+        }
+        const uint64_t wtot = block_reduce_sum<kThreads>(wsum, s_red);
+        if (threadIdx.x == 0) {
+            atomicAdd((unsigned long long*)g.q_count, (unsigned long long)tile_total);
+            atomicAdd((unsigned long long*)g.w_count, (unsigned long long)wtot);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Final layout.  Node g of the level concatenation (levels 0.., in order) starts
+// at word positions[g] = exclusive prefix of (1 + popc(mask)); leaves follow.
+// ---------------------------------------------------------------------------
+constexpr int kFinalIpt = 16;
+constexpr int kFinalTile = kThreads * kFinalIpt;
+
+__device__ __forceinline__ int level_of(const FinalArgs& a, uint64_t g)
+{
+    int l = 0;
+    while (l + 1 < a.num_levels && g >= a.levels[l + 1].first_node) ++l;
+    return l;
+}
+
+__global__ __launch_bounds__(kThreads) void k_final_positions(FinalArgs a)
+{
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+    const uint64_t ntiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
+    for (;;) {
+        const uint32_t tile = acquire_tile(a.tile_counter, &s_tile);
+        if (tile >= ntiles) break;
+        const uint64_t first = uint64_t(tile) * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt;
+        uint32_t sizes[kFinalIpt];
+        uint32_t sum = 0;
+        int l = first < a.total_nodes ? level_of(a, first) : 0;
+#pragma unroll
+        for (int j = 0; j < kFinalIpt; ++j) {
+            const uint64_t gi = first + j;
+            sizes[j] = 0;
+            if (gi < a.total_nodes) {
+                while (gi >= a.levels[l].first_node + a.levels[l].num_nodes) ++l;
+                sizes[j] = 1 + __popc(uint32_t(a.levels[l].masks[gi - a.levels[l].first_node]));
+            }
+            sum += sizes[j];
+        }
+        uint32_t tile_total;
+        const uint32_t local = block_exclusive_scan<kThreads>(sum, s_scan, tile_total);
+        uint64_t excl;
+        if (!block_lookback(a.status, tile, tile_total, a.err, &s_excl, excl)) continue;
+        uint64_t pos = excl + local;
+#pragma unroll
+        for (int j = 0; j < kFinalIpt; ++j) {
+            if (first + j < a.total_nodes) a.positions[first + j] = uint32_t(pos);
+            pos += sizes[j];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_final_write(FinalArgs a)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t gi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < a.total_nodes; gi += stride) {
+        const int l = level_of(a, gi);
+        const FinalLevel& L = a.levels[l];
+        const uint64_t i = gi - L.first_node;
+        const uint32_t mask = L.masks[i];
+        const uint32_t pos = a.positions[gi];
+        a.words[pos] = mask;
+        const bool last = l + 1 == a.num_levels;
+        const uint4* src = reinterpret_cast<const uint4*>(L.children + 8 * i);
+        const uint4 c0 = src[0], c1 = src[1];
+        const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        uint32_t off = 1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if ((mask >> s) & 1u) {
+                const uint32_t c = ch[s];
+                const uint32_t target = last ? uint32_t(a.leaves_start + 2ull * c)
+                                             : a.positions[a.levels[l + 1].first_node + c];
+                a.words[pos + off] = target;
+                ++off;
+            }
+        }
+    }
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < a.num_leaves; r += stride) {
+        const uint64_t leaf = a.leaves[r];
+        a.words[a.leaves_start + 2 * r] = uint32_t(leaf);
+        a.words[a.leaves_start + 2 * r + 1] = uint32_t(leaf >> 32);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge insert (MergeDAGs, src/DAGCompression.cu:1109-1320): the nodes of one
+// level of many sub-DAGs, concatenated, enter one shared dedup table.  A node's
+// hash depends only on its content, so it is reused; its children are remapped
+// from sub-DAG-local indices to the merged level below through that level's
+// slot and rank arrays.  Leaves (ordered == 0) go into the hashed leaf table.
+// ---------------------------------------------------------------------------
+__global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsrc, uint64_t total,
+                               const uint32_t* __restrict__ child_slot_of,
+                               const uint32_t* __restrict__ child_rank_of_slot, uint8_t* __restrict__ masks_out,
+                               uint32_t* __restrict__ children_out, uint64_t* __restrict__ tk,
+                               uint32_t* __restrict__ trep, const uint32_t* __restrict__ log2_ptr,
+                               uint32_t* __restrict__ slot_of, BuildCounters* __restrict__ c, int ordered)
+{
+    const uint32_t lg = *log2_ptr;
+    const uint64_t limit = (1ull << lg) + kTableOverflow;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t gi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < total; gi += stride) {
+        uint32_t lo = 0, hi = nsrc;  // src[lo].first <= gi < src[hi].first
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (src[mid].first <= gi) lo = mid;
+            else hi = mid;
+        }
+        const MergeSource& s = src[lo];
+        const uint64_t i = gi - s.first;
+        const uint64_t key = s.hashes[i];
+        if (!ordered) {
+            slot_of[gi] = hashed_insert(tk, lg, key, (unsigned long long*)&c->error);
+            continue;
+        }
+        const uint4* cs = reinterpret_cast<const uint4*>(s.children + 8 * i);
+        const uint4 a = cs[0], b = cs[1];
+        uint32_t ch[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (ch[k] != kEmptyChild) ch[k] = child_rank_of_slot[child_slot_of[s.child_first + ch[k]]];
+        masks_out[gi] = s.masks[i];
+        uint4* dst = reinterpret_cast<uint4*>(children_out + 8 * gi);
+        dst[0] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
+        dst[1] = make_uint4(ch[4], ch[5], ch[6], ch[7]);
+        slot_of[gi] = ordered_insert(tk, trep, lg, limit, key, uint32_t(gi), c);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
+{
+    if (count == 0) return;
+    const int grid = persistent_grid((count + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(kThreads), 0, s, p, count, value);
+}
+
+template <typename K>
+void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint64_t* d_offsets, uint64_t num_chunks,
+                          uint64_t n, uint32_t chunk_bits, BuildCounters* c)
+{
+    if (n == 0) return;
+    const int grid = persistent_grid((n + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_prefix_chunks<K>, dim3(grid), dim3(kThreads), 0, s, in, out, d_offsets, num_chunks, n,
+                       chunk_bits, c);
+}
+template void launch_prefix_chunks<uint64_t>(hipStream_t, const uint64_t*, uint64_t*, const uint64_t*, uint64_t,
+                                             uint64_t, uint32_t, BuildCounters*);
+
+template <typename K>
+void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
+                      uint64_t* status, uint32_t* tile_counter, uint64_t n_bound, int leaf_level,
+                      uint64_t code_mask)
+{
+    const int grid = persistent_grid((n_bound + kPackTile - 1) / kPackTile, 4);
+    hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, c, lp_code, lp_mask, status,
+                       tile_counter, &c->pre[leaf_level], code_mask);
+}
+template void launch_leaf_pack<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint64_t*, uint64_t*,
+                                         uint64_t*, uint32_t*, uint64_t, int, uint64_t);
+template void launch_leaf_pack<uint32_t>(hipStream_t, const uint32_t*, BuildCounters*, uint64_t*, uint64_t*,
+                                         uint64_t*, uint32_t*, uint64_t, int, uint64_t);
+
+void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
+                        uint32_t load_shift, uint32_t overflow, uint64_t slot_bound)
+{
+    const int grid = persistent_grid((slot_bound + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_table_clear, dim3(grid), dim3(kThreads), 0, s, keys, count_ptr, log2_out, load_shift,
+                       overflow);
+}
+
+void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err)
+{
+    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid), dim3(kThreads), 0, s, lp_mask, count_ptr, table_keys, log2_ptr,
+                       lp_slot, err);
+}
+
+void launch_table_compact(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t* list_keys,
+                          uint32_t* list_slots, uint64_t* count_out, uint64_t* status, uint32_t* tile_counter,
+                          uint64_t slot_bound, unsigned long long* err)
+{
+    const int grid = persistent_grid((slot_bound + kCompactTile - 1) / kCompactTile, 4);
+    hipLaunchKernelGGL(k_table_compact, dim3(grid), dim3(kThreads), 0, s, table_keys, log2_ptr, list_keys, list_slots,
+                       count_out, status, tile_counter, err);
+}
+
+void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
+                         uint32_t* rank_of_slot, uint64_t n_bound)
+{
+    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_rank_of_slot, dim3(grid), dim3(kThreads), 0, s, sorted_slots, count_ptr, rank_of_slot);
+}
+
+void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound)
+{
+    const int grid = persistent_grid((child_bound + kBuildTile - 1) / kBuildTile, 4);
+    hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
+}
+
+void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound)
+{
+    const int grid = persistent_grid((slot_bound + kExtractTile - 1) / kExtractTile, 4);
+    hipLaunchKernelGGL(k_level_extract, dim3(grid), dim3(kThreads), 0, s, a);
+}
+
+void launch_level_check(hipStream_t s, const uint32_t* parent_slots, const uint32_t* table_rep,
+                        const uint8_t* parent_masks, const uint32_t* parent_children, const uint64_t* count_ptr,
+                        uint64_t* collisions, uint64_t n_bound)
+{
+    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_level_check, dim3(grid), dim3(kThreads), 0, s, parent_slots, table_rep, parent_masks,
+                       parent_children, count_ptr, collisions);
+}
+
+void launch_final_positions(hipStream_t s, const FinalArgs& a)
+{
+    if (a.total_nodes == 0) return;
+    const int grid = persistent_grid((a.total_nodes + kFinalTile - 1) / kFinalTile, 4);
+    hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
+}
+
+void launch_final_write(hipStream_t s, const FinalArgs& a)
+{
+    const uint64_t work = std::max(a.total_nodes, a.num_leaves);
+    if (work == 0) return;
+    const int grid = persistent_grid((work + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_final_write, dim3(grid), dim3(kThreads), 0, s, a);
+}
+
+void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,
+                         const uint32_t* child_slot_of, const uint32_t* child_rank_of_slot, uint8_t* masks_out,
+                         uint32_t* children_out, uint64_t* table_keys, uint32_t* table_rep, const uint32_t* table_log2,
+                         uint32_t* slot_of, BuildCounters* counters, int ordered)
+{
+    if (total == 0) return;
+    const int grid = persistent_grid((total + kThreads - 1) / kThreads, 8);
+    hipLaunchKernelGGL(k_merge_insert, dim3(grid), dim3(kThreads), 0, s, d_sources, num_sources, total, child_slot_of,
+                       child_rank_of_slot, masks_out, children_out, table_keys, table_rep, table_log2, slot_of,
+                       counters, ordered);
+}
+
+}  // namespace svdag

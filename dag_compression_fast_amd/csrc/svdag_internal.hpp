@@ -1,0 +1,184 @@
+// svdag_internal.hpp -- host-side launchers and device-memory layouts shared by
+// the kernels (svdag_sort.hip, svdag_build.hip) and the C-ABI driver
+// (svdag_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "svdag.h"
+
+namespace svdag {
+
+// ---- radix sort -----------------------------------------------------------
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 8;
+// 256 threads x 16 keys: a tile's LDS (keys, values, per-wave digit counters)
+// stays under 64 KB, the per-workgroup limit of every CDNA generation.
+constexpr int kSortThreads = 256;
+constexpr int kSortIpt = 16;
+constexpr int kSortTile = kSortThreads * kSortIpt;  // keys per tile
+
+struct SortPlan {
+    int passes;
+    uint32_t shift[kMaxPasses];
+    uint32_t bits[kMaxPasses];
+};
+
+// Digits of width <= 8 covering bits [0, key_bits), spread evenly.
+SortPlan make_sort_plan(int key_bits);
+
+inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+// Device scratch of one sort: histograms (passes*256 u32) + look-back status
+// (passes * tiles * 256 u32).  Both must be zero before launch_sort.
+inline uint64_t sort_zero_words(const SortPlan& p, uint64_t n_bound)
+{
+    return uint64_t(p.passes) * kRadix * (1 + sort_tiles(n_bound));
+}
+
+// LSD onesweep sort of keys (and optionally u32 values) on bits [0, key_bits).
+// n is read from device memory (*n_ptr, < 2^30); n_bound is a host upper bound
+// sizing the grid.  Passes ping-pong between (keys_a, keys_b); the first pass
+// reads keys_in.  Returns which buffer holds the result (0 = a, 1 = b); if
+// final_keys / final_vals are non-null the last pass writes there instead.
+// `zero` points to sort_zero_words(...) zeroed u32 words; `tile_counters` to
+// `passes` zeroed u32 counters.
+template <typename K>
+int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,
+                K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
+                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* tile_counters,
+                unsigned long long* key_or, unsigned long long* err, int grid_limit);
+
+// ---- build ----------------------------------------------------------------
+constexpr uint32_t kTableOverflow = 4096;  // slots past 2^log2 for ordered tables
+
+// Device counters of one build; zeroed at the start.
+struct BuildCounters {
+    uint64_t n;
+    unsigned long long key_or;
+    uint64_t leaf_positions;                  // L0
+    uint64_t unique_leaves;                   // L
+    uint64_t pre[SVDAG_MAX_LEVELS];           // M_l
+    uint64_t uniq[SVDAG_MAX_LEVELS];          // Q_l
+    uint64_t words[SVDAG_MAX_LEVELS];         // W_l
+    uint64_t collisions;
+    uint64_t error;                           // kErr* bits
+    uint32_t table_log2[SVDAG_MAX_LEVELS + 1];  // [l] level tables, [MAX] leaf table
+    uint32_t tile_counters[160];
+};
+
+constexpr uint64_t kErrTableOverflow = 1;
+constexpr uint64_t kErrKeyRange = 2;
+// kErrSpin = 4 (svdag_device.hpp): a bounded wait gave up.
+
+int device_cu_count();
+
+void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value);
+
+template <typename K>
+void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint64_t* d_offsets, uint64_t num_chunks,
+                          uint64_t n, uint32_t chunk_bits, BuildCounters* c);
+
+template <typename K>
+void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
+                      uint64_t* status, uint32_t* tile_counter, uint64_t n_bound, int leaf_level,
+                      uint64_t code_mask);
+
+// Clears an ordered (load_shift, overflow) or hashed table sized from *count_ptr
+// and records its log2 size.
+void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
+                        uint32_t load_shift, uint32_t overflow, uint64_t slot_bound);
+
+void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err);
+
+void launch_table_compact(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t* list_keys,
+                          uint32_t* list_slots, uint64_t* count_out, uint64_t* status, uint32_t* tile_counter,
+                          uint64_t slot_bound, unsigned long long* err);
+
+void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
+                         uint32_t* rank_of_slot, uint64_t n_bound);
+
+struct LevelBuildArgs {
+    const uint64_t* child_codes;
+    const uint32_t* child_slots;
+    const uint32_t* child_rank_of_slot;
+    const uint64_t* child_hashes;
+    const uint64_t* child_count;
+    uint64_t* parent_codes;
+    uint32_t* parent_slots;
+    uint8_t* parent_masks;
+    uint32_t* parent_children;
+    uint64_t* table_keys;
+    uint32_t* table_rep;
+    const uint32_t* table_log2;
+    uint64_t* status;
+    uint32_t* tile_counter;
+    uint64_t* next_count;     // M of the level above (atomic adds), or null
+    BuildCounters* counters;
+};
+void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound);
+
+struct LevelExtractArgs {
+    const uint64_t* table_keys;
+    const uint32_t* table_rep;
+    const uint32_t* table_log2;
+    uint32_t* rank_of_slot;
+    const uint8_t* parent_masks;
+    const uint32_t* parent_children;
+    uint64_t* out_hashes;
+    uint8_t* out_masks;
+    uint32_t* out_children;
+    uint64_t* status;
+    uint32_t* tile_counter;
+    uint64_t* q_count;
+    uint64_t* w_count;
+    unsigned long long* err;
+};
+void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound);
+
+void launch_level_check(hipStream_t s, const uint32_t* parent_slots, const uint32_t* table_rep,
+                        const uint8_t* parent_masks, const uint32_t* parent_children, const uint64_t* count_ptr,
+                        uint64_t* collisions, uint64_t n_bound);
+
+// ---- final layout ---------------------------------------------------------
+struct FinalLevel {
+    const uint8_t* masks;
+    const uint32_t* children;
+    uint64_t first_node;   // index of the level's first node in the concatenation
+    uint64_t num_nodes;
+};
+struct FinalArgs {
+    FinalLevel levels[SVDAG_MAX_LEVELS];
+    int num_levels;
+    uint64_t total_nodes;
+    const uint64_t* leaves;
+    uint64_t num_leaves;
+    uint64_t leaves_start;   // word index of the first leaf
+    uint32_t* positions;     // [total_nodes] absolute word offset of each node
+    uint32_t* words;
+    uint64_t* status;
+    uint32_t* tile_counter;
+    unsigned long long* err;
+};
+void launch_final_positions(hipStream_t s, const FinalArgs& a);
+void launch_final_write(hipStream_t s, const FinalArgs& a);
+
+// ---- merge ----------------------------------------------------------------
+// Remaps and inserts the nodes of many sub-DAG levels into one shared table.
+struct MergeSource {
+    const uint8_t* masks;
+    const uint32_t* children;
+    const uint64_t* hashes;
+    uint64_t first;          // index of the source's first node in the concatenation
+    uint64_t num;
+    uint64_t child_first;    // index of the source's first child in the child concatenation
+};
+void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,
+                         const uint32_t* child_slot_of, const uint32_t* child_rank_of_slot, uint8_t* masks_out,
+                         uint32_t* children_out, uint64_t* table_keys, uint32_t* table_rep, const uint32_t* table_log2,
+                         uint32_t* slot_of, BuildCounters* counters, int ordered);
+
+}  // namespace svdag

@@ -1,0 +1,304 @@
+// svdag_sort.hip -- LSD onesweep radix sort for gfx950 (wave64).
+//
+// Replaces the CUB DeviceRadixSort calls of the reference (SortKeys,
+// src/DAGCompression.cu:336-357; SortPairs, :359-394), which run three kernels
+// per digit pass.  Here one histogram kernel reads the keys once for all digit
+// positions, then each pass is a single kernel:
+//   * a workgroup takes tiles in increasing order from an atomic counter;
+//   * each wave ranks its keys with 8 ballots per key (lanes holding the same
+//     digit form a match mask; rank = popcount below the lane + a per-wave LDS
+//     counter), which keeps the sort stable;
+//   * per-digit tile counts are published and prefixed by decoupled look-back;
+//   * keys are staged in LDS in digit order, then written out in runs.
+#include "svdag_device.hpp"
+#include "svdag_internal.hpp"
+
+namespace svdag {
+
+SortPlan make_sort_plan(int key_bits)
+{
+    SortPlan p{};
+    if (key_bits <= 0) return p;
+    p.passes = (key_bits + kRadixBits - 1) / kRadixBits;
+    const int base = key_bits / p.passes, extra = key_bits % p.passes;
+    uint32_t shift = 0;
+    for (int i = 0; i < p.passes; ++i) {
+        p.bits[i] = uint32_t(base + (i < extra ? 1 : 0));
+        p.shift[i] = shift;
+        shift += p.bits[i];
+    }
+    return p;
+}
+
+namespace {
+
+constexpr uint32_t kDigitAgg = 1u << 30;
+constexpr uint32_t kDigitPrefix = 2u << 30;
+constexpr uint32_t kDigitValue = (1u << 30) - 1;
+
+struct HistPlan {
+    int passes;
+    uint32_t shift[kMaxPasses];
+    uint32_t mask[kMaxPasses];
+};
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, const uint64_t* __restrict__ n_ptr,
+                                                   HistPlan plan, uint32_t* __restrict__ hist,
+                                                   unsigned long long* __restrict__ key_or)
+{
+    __shared__ uint32_t s_hist[kMaxPasses][kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += 256) (&s_hist[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t n = *n_ptr;
+    K acc = 0;
+    const uint64_t stride = uint64_t(gridDim.x) * 256;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+        const K k = keys[i];
+        acc |= k;
+        for (int p = 0; p < plan.passes; ++p) atomicAdd(&s_hist[p][uint32_t(k >> plan.shift[p]) & plan.mask[p]], 1u);
+    }
+    __syncthreads();
+    for (int p = 0; p < plan.passes; ++p)
+        for (int d = threadIdx.x; d < kRadix; d += 256) {
+            const uint32_t v = s_hist[p][d];
+            if (v) atomicAdd(&hist[p * kRadix + d], v);
+        }
+    if (key_or) {
+        uint64_t a = uint64_t(acc);
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) a |= __shfl_xor(a, d, kWave);
+        if (lane_id() == 0 && a) atomicOr(key_or, (unsigned long long)a);
+    }
+}
+
+// Sequential per-digit look-back run by thread `d` of the tile.  The wait is
+// bounded (kSpinLimit polls); on giving up it sets kErrSpin and returns false.
+__device__ __forceinline__ bool digit_lookback(uint32_t* __restrict__ status, uint32_t tile, uint32_t d,
+                                               uint32_t total, unsigned long long* err, uint32_t& excl)
+{
+    excl = 0;
+    if (tile == 0) {
+        atomic_store_relaxed(&status[d], kDigitPrefix | total);
+        return true;
+    }
+    atomic_store_relaxed(&status[uint64_t(tile) * kRadix + d], kDigitAgg | total);
+    int64_t t = int64_t(tile) - 1;
+    uint32_t spins = 0;
+    for (;;) {
+        const uint32_t s = atomic_load_relaxed(&status[uint64_t(t) * kRadix + d]);
+        const uint32_t f = s >> 30;
+        if (f == 0) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, (unsigned long long)kErrSpin);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += s & kDigitValue;
+        if (f == 2) break;
+        --t;
+    }
+    atomic_store_relaxed(&status[uint64_t(tile) * kRadix + d], kDigitPrefix | (excl + total));
+    return true;
+}
+
+template <typename K, bool PAIRS>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+                                                           const uint32_t* __restrict__ vals_in,
+                                                           uint32_t* __restrict__ vals_out,
+                                                           const uint64_t* __restrict__ n_ptr, uint32_t shift,
+                                                           uint32_t digit_mask, const uint32_t* __restrict__ hist,
+                                                           uint32_t* __restrict__ status,
+                                                           uint32_t* __restrict__ tile_counter,
+                                                           unsigned long long* __restrict__ err)
+{
+    constexpr int W = kSortThreads / kWave;
+    __shared__ K s_keys[kSortTile];
+    __shared__ uint32_t s_vals[PAIRS ? kSortTile : 1];
+    __shared__ uint32_t s_wcnt[W][kRadix];
+    __shared__ uint32_t s_gstart[kRadix];
+    __shared__ uint32_t s_lstart[kRadix];
+    __shared__ uint32_t s_base[kRadix];
+    __shared__ uint32_t s_scan[W];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_fail;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w = tid / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t n = uint32_t(*n_ptr);
+    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
+
+    {
+        uint32_t total;
+        const uint32_t g = block_exclusive_scan<kSortThreads>(tid < kRadix ? hist[tid] : 0u, s_scan, total);
+        if (tid < kRadix) s_gstart[tid] = g;
+    }
+
+    for (;;) {
+        if (tid == 0) {
+            s_tile = atomicAdd(tile_counter, 1u);
+            s_fail = 0;
+        }
+        for (int i = tid; i < W * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const uint32_t tile_base = tile * kSortTile;
+        const uint32_t tile_n = min(uint32_t(kSortTile), n - tile_base);
+
+        K key[kSortIpt];
+        uint32_t val[PAIRS ? kSortIpt : 1];
+        uint32_t dig[kSortIpt];
+        uint32_t rnk[kSortIpt];
+        const uint32_t wave_base = w * kWave * kSortIpt;
+#pragma unroll
+        for (int i = 0; i < kSortIpt; ++i) {
+            const uint32_t idx = wave_base + i * kWave + lane;
+            const bool valid = idx < tile_n;
+            key[i] = valid ? keys_in[tile_base + idx] : K(0);
+            if constexpr (PAIRS) val[i] = valid ? vals_in[tile_base + idx] : 0u;
+            dig[i] = uint32_t(key[i] >> shift) & digit_mask;
+        }
+        uint32_t* wc = s_wcnt[w];
+#pragma unroll
+        for (int i = 0; i < kSortIpt; ++i) {
+            const uint32_t idx = wave_base + i * kWave + lane;
+            const bool valid = idx < tile_n;
+            const uint32_t d = dig[i];
+            uint64_t match = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t vote = __ballot(bit);
+                match &= bit ? vote : ~vote;
+            }
+            const uint32_t below = __popcll(match & lanemask_lt());
+            uint32_t c = 0;
+            if (valid) c = wc[d];
+            // Every lane of the group reads the counter before its highest lane
+            // (the leader) writes it back.
+            __builtin_amdgcn_wave_barrier();
+            rnk[i] = c + below;
+            if (valid && (match >> lane) == 1ull) wc[d] = c + uint32_t(__popcll(match));
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+
+        uint32_t total = 0;
+        if (tid < kRadix) {
+#pragma unroll
+            for (int i = 0; i < W; ++i) {
+                const uint32_t c = s_wcnt[i][tid];
+                s_wcnt[i][tid] = total;
+                total += c;
+            }
+        }
+        uint32_t excl = 0;
+        if (tid < kRadix && !digit_lookback(status, tile, tid, total, err, excl)) s_fail = 1;
+        uint32_t unused;
+        const uint32_t lstart = block_exclusive_scan<kSortThreads>(tid < kRadix ? total : 0u, s_scan, unused);
+        if (tid < kRadix) {
+            s_lstart[tid] = lstart;
+            s_base[tid] = s_gstart[tid] + excl - lstart;
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int i = 0; i < kSortIpt; ++i) {
+            const uint32_t idx = wave_base + i * kWave + lane;
+            if (idx < tile_n) {
+                const uint32_t d = dig[i];
+                const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + rnk[i];
+                s_keys[pos] = key[i];
+                if constexpr (PAIRS) s_vals[pos] = val[i];
+            }
+        }
+        __syncthreads();
+        const bool ok = s_fail == 0;
+        for (uint32_t j = tid; ok && j < tile_n; j += kSortThreads) {
+            const K k = s_keys[j];
+            const uint32_t d = uint32_t(k >> shift) & digit_mask;
+            const uint32_t o = s_base[d] + j;
+            keys_out[o] = k;
+            if constexpr (PAIRS) vals_out[o] = s_vals[j];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int device_cu_count()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+template <typename K>
+int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in, K* keys_a,
+                K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
+                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* tile_counters,
+                unsigned long long* key_or, unsigned long long* err, int grid_limit)
+{
+    if (plan.passes == 0 || n_bound == 0) return -1;
+    uint32_t* hist = zero;
+    uint32_t* status = zero + plan.passes * kRadix;
+    const uint64_t tiles = sort_tiles(n_bound);
+    HistPlan hp{};
+    hp.passes = plan.passes;
+    for (int p = 0; p < plan.passes; ++p) {
+        hp.shift[p] = plan.shift[p];
+        hp.mask[p] = (1u << plan.bits[p]) - 1u;
+    }
+    const int cus = device_cu_count();
+    const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 8));
+    hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist, key_or);
+
+    const bool pairs = vals_in != nullptr;
+    const K* src = keys_in;
+    const uint32_t* vsrc = vals_in;
+    int where = -1;
+    uint64_t grid = std::min<uint64_t>(tiles, uint64_t(grid_limit > 0 ? grid_limit : cus * 2));
+    for (int p = 0; p < plan.passes; ++p) {
+        const bool last = p + 1 == plan.passes;
+        K* dst = (p % 2 == 0) ? keys_a : keys_b;
+        uint32_t* vdst = (p % 2 == 0) ? vals_a : vals_b;
+        where = p % 2;
+        if (last && final_keys) {
+            dst = final_keys;
+            where = 2;
+        }
+        if (last && final_vals) vdst = final_vals;
+        uint32_t* st = status + uint64_t(p) * tiles * kRadix;
+        if (pairs)
+            hipLaunchKernelGGL((k_onesweep<K, true>), dim3(uint32_t(grid)), dim3(kSortThreads), 0, stream, src, dst,
+                               vsrc, vdst, n_ptr, plan.shift[p], (1u << plan.bits[p]) - 1u, hist + p * kRadix, st,
+                               tile_counters + p, err);
+        else
+            hipLaunchKernelGGL((k_onesweep<K, false>), dim3(uint32_t(grid)), dim3(kSortThreads), 0, stream, src, dst,
+                               nullptr, nullptr, n_ptr, plan.shift[p], (1u << plan.bits[p]) - 1u, hist + p * kRadix,
+                               st, tile_counters + p, err);
+        src = dst;
+        vsrc = vdst;
+    }
+    return where;
+}
+
+template int launch_sort<uint64_t>(hipStream_t, const SortPlan&, const uint64_t*, const uint32_t*, uint64_t*,
+                                   uint64_t*, uint32_t*, uint32_t*, uint64_t*, uint32_t*, const uint64_t*, uint64_t,
+                                   uint32_t*, uint32_t*, unsigned long long*, unsigned long long*, int);
+template int launch_sort<uint32_t>(hipStream_t, const SortPlan&, const uint32_t*, const uint32_t*, uint32_t*,
+                                   uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint64_t*, uint64_t,
+                                   uint32_t*, uint32_t*, unsigned long long*, unsigned long long*, int);
+
+}  // namespace svdag

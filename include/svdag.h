@@ -1,0 +1,178 @@
+/*
+ * svdag.h -- C ABI of the MI355X (gfx950) sparse-voxel-DAG build.
+ *
+ * Drop-in boundary for the reference's DAG build API
+ * (Phyronnaz/DAG_Compression_Fast, src/DAGCompression.h:42-50):
+ *
+ *   FCpuDag CreateDAG(const TStaticArray<FMortonCode, GPU>& Fragments, int32 Depth);
+ *   FCpuDag MergeDAGs(std::vector<FCpuDag>&& CpuDags);
+ *   TStaticArray<uint32, GPU> CreateFinalDAG(FCpuDag&& CpuDag);
+ *   void FreeAll();
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and a
+ * non-zero svdag_status otherwise, with a message in svdag_last_error(); none
+ * aborts (the reference prints and aborts, src/Core.h:111-128, 196-217).
+ *
+ * Output contract (bit-exact with the reference):
+ *   levels[0] is the root level (one node for non-empty input), levels[depth-3]
+ *   the level above the leaves (src/DAGCompression.cu:1008, 1077, 1093);
+ *   nodes of a level are in ascending order of their 64-bit hash, leaves in
+ *   ascending mask order (src/DAGCompression.cu:864-914);
+ *   children[8*i + slot] indexes the next level (or the leaves), 0xFFFFFFFF
+ *   marks an empty slot (src/DAGCompression.cu:952-957);
+ *   hash = low word of MurmurHash128xN over the 8 child hashes, 0 -> 1
+ *   (src/DAGCompression.cu:758-774, src/Utils.h:119-154).
+ */
+#ifndef SVDAG_H
+#define SVDAG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVDAG_MAX_LEVELS 24
+
+typedef enum svdag_status {
+    SVDAG_OK = 0,
+    SVDAG_ERR_ARGUMENT = 1,   /* bad depth, null pointer, count, key >= 2^(3*depth) */
+    SVDAG_ERR_HIP = 2,        /* a HIP runtime call failed                          */
+    SVDAG_ERR_MEMORY = 3,     /* device allocation failed or exceeds the budget     */
+    SVDAG_ERR_INTERNAL = 4,   /* a device-side invariant failed (e.g. table overflow)*/
+    SVDAG_ERR_CAPACITY = 5    /* result does not fit the uint32 index space         */
+} svdag_status;
+
+typedef enum svdag_location {
+    SVDAG_HOST = 0,           /* arrays in host memory (the reference's FCpuDag)    */
+    SVDAG_DEVICE = 1          /* arrays in device memory (HBM)                      */
+} svdag_location;
+
+/* One level: mirror of FCpuLevel (src/DAGCompression.h:12-24). */
+typedef struct svdag_level {
+    uint8_t*  masks;          /* [num_nodes]                                        */
+    uint32_t* children;       /* [num_nodes * 8]                                    */
+    uint64_t* hashes;         /* [num_nodes], strictly ascending                    */
+    uint64_t  num_nodes;
+} svdag_level;
+
+/* Mirror of FCpuDag (src/DAGCompression.h:26-40). */
+typedef struct svdag_dag {
+    int32_t     num_levels;               /* depth - 2                              */
+    int32_t     location;                 /* svdag_location                         */
+    svdag_level levels[SVDAG_MAX_LEVELS];
+    uint64_t*   leaves;                   /* [num_leaves] 4x4x4 occupancy masks     */
+    uint64_t    num_leaves;
+    uint64_t    level_words[SVDAG_MAX_LEVELS]; /* sum of 1 + popc(mask) per level   */
+    uint64_t    hash_collisions;          /* counted only when collision checks are on */
+    /* Build statistics of create_dag (zero otherwise): */
+    uint64_t    num_keys;                 /* N, input keys including duplicates     */
+    uint64_t    leaf_positions;           /* L0, non-empty 4^3 blocks before dedup  */
+    uint64_t    pre_dedup[SVDAG_MAX_LEVELS]; /* M_l, nodes of level l before dedup  */
+    void*       storage;                  /* library-owned backing allocation       */
+} svdag_dag;
+
+typedef struct svdag_context svdag_context;
+
+/* Options for svdag_set_option. */
+typedef enum svdag_option {
+    SVDAG_OPT_CHECK_COLLISIONS = 1,   /* 1: compare full node content on equal hash   */
+    SVDAG_OPT_MEMORY_BUDGET = 2,      /* bytes of scratch the context may hold, 0 = no limit */
+    SVDAG_OPT_TABLE_LOAD_SHIFT = 3,   /* dedup table slots >= 2^shift * nodes (1..3)  */
+    SVDAG_OPT_HOST_THREADS = 4,       /* host threads for host-side work (merge)      */
+    SVDAG_OPT_WAIT_SECONDS = 5        /* completion deadline of a call (default 120 s); past it
+                                         the call fails and names the stage that stalled */
+} svdag_option;
+
+/* --- context ------------------------------------------------------------ */
+
+/* A context owns a HIP stream binding, a device scratch arena and pinned
+ * staging.  One context per host thread (the reference's global pool,
+ * src/DAGCompression.cu:996-1001, is process-global and not thread-safe).
+ * `stream` is a hipStream_t (NULL: the default stream). */
+int svdag_context_create(svdag_context** ctx, void* stream);
+int svdag_context_destroy(svdag_context* ctx);
+int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value);
+
+/* Device scratch bytes create_dag needs for n keys at this depth (admission
+ * input for serving). */
+uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes);
+
+/* --- the build ------------------------------------------------------------ */
+
+/* DAGCompression::CreateDAG (src/DAGCompression.cu:1003-1107).
+ * d_keys: device pointer to n unsorted Morton codes with duplicates, each
+ * < 2^(3*depth); 3 <= depth <= 21; n < 2^30.  The input is not modified
+ * (the reference clobbers it, src/DAGCompression.cu:1028-1033).
+ * On return `out` is a device-resident DAG owned by the caller
+ * (svdag_free_dag); the call synchronises its stream once at the end. */
+int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n,
+                     int32_t depth, svdag_dag* out);
+
+/* Same with 32-bit keys (depth <= 10): the 512^3 u32 variant. */
+int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n,
+                         int32_t depth, svdag_dag* out);
+
+/* DAGCompression::MergeDAGs (src/DAGCompression.cu:1109-1320, 1447-1548).
+ * `count` DAGs of equal depth, a power of 8, in Morton order of their
+ * sub-volumes (src/Utils/Aabb.h:83-105); host- or device-resident.  Inputs are
+ * not consumed.  Output: device-resident DAG with log8(count) more levels. */
+int svdag_merge_dags(svdag_context* ctx, const svdag_dag* dags, uint64_t count,
+                     svdag_dag* out);
+
+/* Batched build: MergeDAGs(CreateDAG(chunk_i)) for `num_chunks` (a power of
+ * 8) chunks in one device pass sharing one dedup table per level.
+ * d_keys holds the chunks back to back, chunk i at [offsets[i], offsets[i+1])
+ * (offsets: host array of num_chunks + 1), keys chunk-local (< 2^(3*chunk_depth)).
+ * Output equals svdag_merge_dags of the per-chunk svdag_create_dag results. */
+int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys,
+                             const uint64_t* offsets, uint64_t num_chunks,
+                             int32_t chunk_depth, svdag_dag* out);
+
+/* DAGCompression::CreateFinalDAG (src/DAGCompression.cu:1550-1610): the flat
+ * uint32 layout.  *d_words is a device allocation (svdag_free_words). */
+int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag,
+                           uint32_t** d_words, uint64_t* n_words);
+
+/* Copies between host (FCpuDag-like) and device residency.  `out` is a new
+ * DAG owned by the caller. */
+int svdag_dag_to_host(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out);
+int svdag_dag_to_device(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out);
+
+int svdag_free_dag(svdag_context* ctx, svdag_dag* dag);
+int svdag_free_words(svdag_context* ctx, uint32_t* d_words);
+
+/* DAGCompression::FreeAll (src/DAGCompression.cu:1612-1615): release the
+ * context's scratch arena. */
+int svdag_free_all(svdag_context* ctx);
+
+/* Message of the last failing call on this host thread. */
+const char* svdag_last_error(void);
+
+/* --- synthetic inputs (host) --------------------------------------------- */
+
+/* Morton code of (x, y, z) (src/DAG_Compression_Fast.cpp:223-236). */
+uint64_t svdag_morton_encode(uint32_t x, uint32_t y, uint32_t z);
+
+/* Seeded generators; *keys is malloc'd (svdag_gen_free). */
+int svdag_gen_sphere_shells(uint64_t seed, int32_t depth, int32_t count, double rmin,
+                            double rmax, double dup_fraction, int32_t threads,
+                            uint64_t** keys, uint64_t* n);
+int svdag_gen_triangle_surfaces(uint64_t seed, int32_t depth, int32_t count, double mu,
+                                double sigma, double dup_fraction, int32_t threads,
+                                uint64_t** keys, uint64_t* n);
+int svdag_gen_value_noise(uint64_t seed, int32_t depth, int32_t spacing, uint32_t sub_depth,
+                          uint64_t sub_index, int32_t threads, uint64_t** keys, uint64_t* n);
+int svdag_gen_chunk_plan(uint64_t seed, uint64_t num_chunks, double median, double sigma,
+                         uint64_t nmin, uint64_t nmax, uint64_t* n_out,
+                         uint64_t* unique_out, int32_t* kind_out);
+int svdag_gen_chunk(uint64_t seed, uint64_t chunk, int32_t depth, double median, double sigma,
+                    uint64_t nmin, uint64_t nmax, uint64_t* keys, uint64_t capacity,
+                    uint64_t* n);
+void svdag_gen_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVDAG_H */

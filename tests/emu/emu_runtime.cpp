@@ -1,0 +1,231 @@
+// CPU kernel emulator runtime (tests only): fiber scheduler for the mock
+// <hip/hip_runtime.h> and host-side stand-ins for the HIP runtime calls.
+#include "hip/hip_runtime.h"
+
+#include <ucontext.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <thread>
+#include <vector>
+
+namespace emu {
+namespace {
+
+struct Fiber {
+    ucontext_t ctx;
+    dim3 tid;
+    bool done = false;
+};
+
+struct Wave {
+    unsigned count = 0, gen = 0, size = 64;
+    unsigned long long slot[64];
+};
+
+constexpr size_t kStack = 512 * 1024;
+
+// Per OS thread (= per running block):
+thread_local ucontext_t g_main;
+thread_local std::vector<Fiber> g_fibers;
+thread_local std::vector<char*> g_stacks;
+thread_local std::vector<Wave> g_waves;
+thread_local int g_cur = -1;
+thread_local dim3 g_block;
+thread_local unsigned g_bar_count = 0, g_bar_gen = 0;
+thread_local std::chrono::steady_clock::time_point g_block_start;
+// Per launch:
+dim3 g_block_dim, g_grid_dim;
+std::function<void()>* g_body = nullptr;
+int g_launch_threads = 8;
+
+void yield()
+{
+    const int me = g_cur;
+    swapcontext(&g_fibers[me].ctx, &g_main);
+}
+
+// Every lane, the last arriver included, yields once the wave has met, so the
+// lanes resume in lane order like one lock-stepped instruction stream.
+void wave_barrier(Wave& w)
+{
+    const unsigned gen = w.gen;
+    if (++w.count == w.size) {
+        w.count = 0;
+        ++w.gen;
+        yield();
+    } else {
+        while (w.gen == gen) yield();
+    }
+}
+
+unsigned flat_tid() { return g_fibers[g_cur].tid.x + g_block_dim.x * (g_fibers[g_cur].tid.y + g_block_dim.y * g_fibers[g_cur].tid.z); }
+
+Wave& my_wave() { return g_waves[flat_tid() / 64]; }
+
+void fiber_entry()
+{
+    (*g_body)();
+    g_fibers[g_cur].done = true;
+}
+
+void run_block(unsigned nthreads)
+{
+    if (g_stacks.size() < nthreads)
+        while (g_stacks.size() < nthreads) g_stacks.push_back(static_cast<char*>(std::malloc(kStack)));
+    g_fibers.assign(nthreads, Fiber{});
+    g_waves.assign((nthreads + 63) / 64, Wave{});
+    for (size_t w = 0; w < g_waves.size(); ++w) g_waves[w].size = std::min<unsigned>(64, nthreads - unsigned(w) * 64);
+    g_bar_count = 0;
+    for (unsigned i = 0; i < nthreads; ++i) {
+        Fiber& f = g_fibers[i];
+        f.tid = dim3(i % g_block_dim.x, (i / g_block_dim.x) % g_block_dim.y, i / (g_block_dim.x * g_block_dim.y));
+        getcontext(&f.ctx);
+        f.ctx.uc_stack.ss_sp = g_stacks[i];
+        f.ctx.uc_stack.ss_size = kStack;
+        f.ctx.uc_link = &g_main;
+        makecontext(&f.ctx, fiber_entry, 0);
+    }
+    unsigned alive = nthreads;
+    while (alive) {
+        for (unsigned i = 0; i < nthreads; ++i) {
+            if (g_fibers[i].done) continue;
+            g_cur = int(i);
+            swapcontext(&g_main, &g_fibers[i].ctx);
+            if (g_fibers[i].done) --alive;
+        }
+    }
+    g_cur = -1;
+}
+
+}  // namespace
+
+const dim3& thread_idx() { return g_fibers[g_cur].tid; }
+const dim3& block_idx() { return g_block; }
+const dim3& block_dim() { return g_block_dim; }
+const dim3& grid_dim() { return g_grid_dim; }
+unsigned lane() { return flat_tid() % 64; }
+
+void sync_block()
+{
+    const unsigned gen = g_bar_gen;
+    if (++g_bar_count == g_fibers.size()) {
+        g_bar_count = 0;
+        ++g_bar_gen;
+    } else {
+        while (g_bar_gen == gen) yield();
+    }
+}
+
+void yield_sleep()
+{
+    if (std::chrono::steady_clock::now() - g_block_start > std::chrono::seconds(60)) {
+        std::fprintf(stderr, "emu: a spin-wait did not resolve in 60 s (block %u thread %u)\n", g_block.x, flat_tid());
+        std::abort();
+    }
+    yield();
+}
+
+void wave_sync() { wave_barrier(my_wave()); }
+
+unsigned long long ballot(bool p)
+{
+    Wave& w = my_wave();
+    w.slot[lane()] = p ? 1 : 0;
+    wave_barrier(w);
+    unsigned long long r = 0;
+    for (unsigned i = 0; i < w.size; ++i)
+        if (w.slot[i]) r |= 1ull << i;
+    wave_barrier(w);
+    return r;
+}
+
+unsigned long long exchange(unsigned long long v, int src)
+{
+    Wave& w = my_wave();
+    w.slot[lane()] = v;
+    wave_barrier(w);
+    const unsigned long long r = (src >= 0 && unsigned(src) < w.size) ? w.slot[src] : v;
+    wave_barrier(w);
+    return r;
+}
+
+void launch(dim3 grid, dim3 block, std::function<void()> body)
+{
+    if (g_cur >= 0) {
+        std::fprintf(stderr, "emu: launch from inside a kernel\n");
+        std::abort();
+    }
+    g_grid_dim = grid;
+    g_block_dim = block;
+    g_body = &body;
+    const unsigned nthreads = block.x * block.y * block.z;
+    const unsigned nblocks = grid.x * grid.y * grid.z;
+    std::atomic<unsigned> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const unsigned b = next.fetch_add(1);
+            if (b >= nblocks) break;
+            g_block = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
+            g_block_start = std::chrono::steady_clock::now();
+            run_block(nthreads);
+        }
+    };
+    const unsigned workers = std::min<unsigned>(nblocks, unsigned(g_launch_threads));
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < workers; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    g_body = nullptr;
+}
+
+}  // namespace emu
+
+extern "C" {
+
+static void* emu_alloc(size_t n)
+{
+    const size_t bytes = (n + 255) / 256 * 256;
+    void* p = std::aligned_alloc(256, bytes ? bytes : 256);
+    if (p) std::memset(p, 0xCD, bytes ? bytes : 256);  // device memory is never assumed zero
+    return p;
+}
+
+hipError_t hipMalloc(void** p, size_t n) { *p = emu_alloc(n); return *p ? hipSuccess : hipErrorOutOfMemory; }
+hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
+hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
+hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
+hipError_t hipHostFree(void* p) { return hipFree(p); }
+hipError_t hipMemcpy(void* d, const void* s, size_t n, int) { if (n) std::memmove(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, int k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipGetLastError() { return hipSuccess; }
+const char* hipGetErrorString(hipError_t) { return "emulator error"; }
+hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int)
+{
+    p->multiProcessorCount = 2;
+    std::strcpy(p->name, "cpu-emulator");
+    return hipSuccess;
+}
+hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* p, int) { *p = nullptr; return hipSuccess; }
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, int, void*) { return hipSuccess; }
+hipError_t hipMemPoolTrimTo(hipMemPool_t, size_t) { return hipSuccess; }
+hipError_t hipEventCreate(void** e) { *e = std::malloc(8); return hipSuccess; }
+hipError_t hipEventCreateWithFlags(void** e, unsigned) { *e = std::malloc(8); return hipSuccess; }
+hipError_t hipEventRecord(void*, void*) { return hipSuccess; }
+hipError_t hipEventQuery(void*) { return hipSuccess; }
+hipError_t hipStreamQuery(void*) { return hipSuccess; }
+hipError_t hipEventSynchronize(void*) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, void*, void*) { *ms = 0; return hipSuccess; }
+hipError_t hipEventDestroy(void* e) { std::free(e); return hipSuccess; }
+hipError_t hipStreamCreate(void** s) { *s = nullptr; return hipSuccess; }
+hipError_t hipStreamDestroy(void*) { return hipSuccess; }
+}
