@@ -21,6 +21,7 @@ OPT_CHECK_COLLISIONS = 1
 OPT_MEMORY_BUDGET = 2
 OPT_TABLE_LOAD_SHIFT = 3
 OPT_HOST_THREADS = 4
+OPT_WAIT_SECONDS = 5
 
 STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "HIP", 3: "MEMORY", 4: "INTERNAL", 5: "CAPACITY"}
 

@@ -141,12 +141,14 @@ inline uint32_t table_lg(uint64_t count, uint32_t shift)
     return std::max(8u, ceil_log2_u64(std::max<uint64_t>(count, 1) << shift));
 }
 
-// Scratch layout of one build over n keys at `depth`.
+// Scratch layout of one build over n keys at `depth`.  Only [zero_begin,
+// zero_end) (counters and sort histograms) must be zero at the start; every
+// other buffer is fully written before it is read.
 struct BuildLayout {
     uint64_t total = 0;
     uint64_t zero_begin = 0, zero_end = 0;
-    uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0, pack_status = 0, compact_status = 0;
-    uint64_t level_status[SVDAG_MAX_LEVELS][2] = {};
+    uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0;
+    uint64_t sort_counts = 0, tile_counts = 0, tile_offsets = 0;
     uint64_t gkeys = 0, keys_a = 0, keys_b = 0;
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
@@ -166,19 +168,17 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.level_slots = (1ull << table_lg(L.level_bound, load_shift)) + kTableOverflow;
     L.key_plan = make_sort_plan(3 * depth);
     L.leaf_plan = make_sort_plan(64);
-    const int levels = depth - 2;
+    const uint64_t stage_tiles = std::max({tiles(n, kPackTileItems), tiles(L.leaf_bound, kBuildTileItems),
+                                           tiles(L.level_slots, kExtractTileItems), uint64_t(1)});
     Carver c;
     L.zero_begin = c.off;
     L.counters = c.take(sizeof(BuildCounters));
-    L.key_sort_zero = c.take(4 * sort_zero_words(L.key_plan, n));
-    L.leaf_sort_zero = c.take(4 * sort_zero_words(L.leaf_plan, L.leaf_bound));
-    L.pack_status = c.take(8 * ((n + 4095) / 4096));
-    L.compact_status = c.take(8 * ((L.leaf_slots + 4095) / 4096));
-    for (int l = 0; l < levels; ++l) {
-        L.level_status[l][0] = c.take(8 * ((L.leaf_bound + 2047) / 2048));
-        L.level_status[l][1] = c.take(8 * ((L.level_slots + 2047) / 2048));
-    }
+    L.key_sort_zero = c.take(4 * sort_zero_words(L.key_plan));
+    L.leaf_sort_zero = c.take(4 * sort_zero_words(L.leaf_plan));
     L.zero_end = c.off;
+    L.sort_counts = c.take(4 * sort_count_words(n));
+    L.tile_counts = c.take(4 * stage_tiles);
+    L.tile_offsets = c.take(8 * stage_tiles);
     const int kb = batched ? 8 : key_bytes;
     if (batched) L.gkeys = c.take(8 * n);
     L.keys_a = c.take(uint64_t(kb) * n);
@@ -326,6 +326,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     char* A = ctx->arena;
     BuildCounters* c = reinterpret_cast<BuildCounters*>(A + L.counters);
     unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
+    uint32_t* sort_counts = reinterpret_cast<uint32_t*>(A + L.sort_counts);
+    uint32_t* tile_counts = reinterpret_cast<uint32_t*>(A + L.tile_counts);
+    uint64_t* tile_offsets = reinterpret_cast<uint64_t*>(A + L.tile_offsets);
 
     uint64_t caps[SVDAG_MAX_LEVELS];
     for (int l = 0; l < levels; ++l) caps[l] = pow8_capped(l, L.leaf_bound);
@@ -355,15 +358,13 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
         uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
         where = launch_sort<uint64_t>(st, L.key_plan, g, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
-                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), &c->tile_counters[0],
-                                      &c->key_or, err, 0);
+                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or);
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     } else {
         K* ka = reinterpret_cast<K*>(A + L.keys_a);
         K* kb = reinterpret_cast<K*>(A + L.keys_b);
         where = launch_sort<K>(st, L.key_plan, d_keys, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
-                               reinterpret_cast<uint32_t*>(A + L.key_sort_zero), &c->tile_counters[0], &c->key_or,
-                               err, 0);
+                               reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or);
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     }
     if (where < 0) return fail(set_error(SVDAG_ERR_INTERNAL, "sort plan failed"));
@@ -375,32 +376,29 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     const uint64_t code_mask = depth >= 22 ? ~0ull : ((1ull << (3 * depth)) - 1);
     // Leaf ReduceByKey (DAGCompression.cu:1044-1068).
     if (batch || sizeof(K) == 8)
-        launch_leaf_pack<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, lp_code, lp_mask,
-                                   reinterpret_cast<uint64_t*>(A + L.pack_status), &c->tile_counters[8], n, depth - 3,
-                                   code_mask);
+        launch_leaf_pack<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, lp_code, lp_mask, tile_counts,
+                                   tile_offsets, n, depth - 3, code_mask);
     else
-        launch_leaf_pack<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, lp_code, lp_mask,
-                                   reinterpret_cast<uint64_t*>(A + L.pack_status), &c->tile_counters[8], n, depth - 3,
-                                   code_mask);
+        launch_leaf_pack<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, lp_code, lp_mask, tile_counts,
+                                   tile_offsets, n, depth - 3, code_mask);
 
     mark(ctx, "leaf pack");
     // SortLevel(Leaves) (DAGCompression.cu:1070-1071): dedup table, then order.
     uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + L.leaf_table);
     uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + L.leaf_rank);
     uint32_t* leaf_lg = &c->table_log2[SVDAG_MAX_LEVELS];
-    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
-    launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, err);
     uint64_t* lk0 = reinterpret_cast<uint64_t*>(A + L.list_keys[0]);
     uint64_t* lk1 = reinterpret_cast<uint64_t*>(A + L.list_keys[1]);
     uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + L.list_slots[0]);
     uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + L.list_slots[1]);
     uint32_t* sorted_slots = reinterpret_cast<uint32_t*>(A + L.sorted_slots);
-    launch_table_compact(st, leaf_table, leaf_lg, lk0, ls0, &c->unique_leaves,
-                         reinterpret_cast<uint64_t*>(A + L.compact_status), &c->tile_counters[9], L.leaf_slots, err);
+    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
+    launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, lk0, ls0,
+                       &c->unique_leaves, err);
     uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
     if (launch_sort<uint64_t>(st, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
                               &c->unique_leaves, L.leaf_bound, reinterpret_cast<uint32_t*>(A + L.leaf_sort_zero),
-                              &c->tile_counters[10], nullptr, err, 0) < 0)
+                              sort_counts, nullptr) < 0)
         return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
     launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
     mark(ctx, "leaf dedup");
@@ -432,8 +430,8 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         a.table_keys = reinterpret_cast<uint64_t*>(A + L.tkeys[cur]);
         a.table_rep = reinterpret_cast<uint32_t*>(A + L.trep[cur]);
         a.table_log2 = &c->table_log2[l];
-        a.status = reinterpret_cast<uint64_t*>(A + L.level_status[l][0]);
-        a.tile_counter = &c->tile_counters[20 + 2 * step];
+        a.tile_counts = tile_counts;
+        a.offsets = tile_offsets;
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
         launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_bound);
@@ -451,11 +449,11 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
         e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
         e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
-        e.status = reinterpret_cast<uint64_t*>(A + L.level_status[l][1]);
-        e.tile_counter = &c->tile_counters[21 + 2 * step];
+        e.tile_counts = tile_counts;
+        e.offsets = tile_offsets;
+        e.slots_scratch = &c->table_slots;
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
-        e.err = err;
         launch_level_extract(st, e, L.level_slots);
         mark(ctx, "level extract");
         if (l > 0)
@@ -476,7 +474,6 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     if (h.error & kErrKeyRange)
         return fail(set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)"));
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
-    if (h.error & kErrSpin) return fail(set_error(SVDAG_ERR_INTERNAL, "a device-side wait gave up"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
 
     out->storage = storage;
@@ -543,17 +540,16 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     const uint64_t level_slots = (1ull << table_lg(std::max(max_total, nlive), shift)) + kTableOverflow;
     const SortPlan leaf_plan = make_sort_plan(64);
 
+    const uint64_t stage_tiles =
+        std::max({tiles(level_slots, kExtractTileItems), tiles(nlive, kBuildTileItems), uint64_t(1)});
     Carver cv;
     const uint64_t o_zero = cv.off;
     const uint64_t o_counters = cv.take(sizeof(BuildCounters));
-    const uint64_t o_leaf_sort_zero = cv.take(4 * sort_zero_words(leaf_plan, total_leaves));
-    const uint64_t o_compact_status = cv.take(8 * ((leaf_slots + 4095) / 4096));
-    uint64_t o_status[SVDAG_MAX_LEVELS][2];
-    for (int l = 0; l < levels; ++l) {
-        o_status[l][0] = cv.take(8 * ((nlive + 2047) / 2048 + 1));
-        o_status[l][1] = cv.take(8 * ((level_slots + 2047) / 2048));
-    }
+    const uint64_t o_leaf_sort_zero = cv.take(4 * sort_zero_words(leaf_plan));
     const uint64_t zero_end = cv.off;
+    const uint64_t o_sort_counts = cv.take(4 * sort_count_words(total_leaves));
+    const uint64_t o_tile_counts = cv.take(4 * stage_tiles);
+    const uint64_t o_tile_offsets = cv.take(8 * stage_tiles);
     const uint64_t o_sources = cv.take(sizeof(MergeSource) * nlive * (lc_levels + 1));
     const uint64_t o_leaf_slot_of = cv.take(4 * total_leaves);
     const uint64_t o_leaf_table = cv.take(8 * leaf_slots);
@@ -579,7 +575,8 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     TRY(ensure_arena(ctx, cv.off));
     char* A = ctx->arena;
     BuildCounters* c = reinterpret_cast<BuildCounters*>(A + o_counters);
-    unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
+    uint32_t* tile_counts = reinterpret_cast<uint32_t*>(A + o_tile_counts);
+    uint64_t* tile_offsets = reinterpret_cast<uint64_t*>(A + o_tile_offsets);
 
     uint64_t caps[SVDAG_MAX_LEVELS];
     for (int l = 0; l < levels; ++l) caps[l] = l < k ? pow8_capped(l, nlive) : totals[l - k];
@@ -630,25 +627,24 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
         launch_fill_u64(st, &c->pre[k - 1], 1, groups);
     }
 
-    // Leaves: shared hashed table, compact, sort, rank.
+    // Leaves: shared hashed table (first inserters append to the unique list),
+    // sort, rank.
     uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + o_leaf_table);
     uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + o_leaf_rank);
     uint32_t* leaf_slot_of = reinterpret_cast<uint32_t*>(A + o_leaf_slot_of);
     uint32_t* leaf_lg = &c->table_log2[SVDAG_MAX_LEVELS];
-    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, shift, 0, leaf_slots);
-    launch_merge_insert(st, d_src, uint32_t(nlive), total_leaves, nullptr, nullptr, nullptr, nullptr, leaf_table,
-                        nullptr, leaf_lg, leaf_slot_of, c, 0);
     uint64_t* lk0 = reinterpret_cast<uint64_t*>(A + o_lk[0]);
     uint64_t* lk1 = reinterpret_cast<uint64_t*>(A + o_lk[1]);
     uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + o_ls[0]);
     uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + o_ls[1]);
     uint32_t* sorted_slots = reinterpret_cast<uint32_t*>(A + o_sorted_slots);
-    launch_table_compact(st, leaf_table, leaf_lg, lk0, ls0, &c->unique_leaves,
-                         reinterpret_cast<uint64_t*>(A + o_compact_status), &c->tile_counters[9], leaf_slots, err);
+    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, shift, 0, leaf_slots);
+    launch_merge_insert(st, d_src, uint32_t(nlive), total_leaves, nullptr, nullptr, nullptr, nullptr, leaf_table,
+                        nullptr, leaf_lg, leaf_slot_of, c, 0, lk0, ls0);
     uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
     if (launch_sort<uint64_t>(st, leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots, &c->unique_leaves,
-                              total_leaves, reinterpret_cast<uint32_t*>(A + o_leaf_sort_zero), &c->tile_counters[10],
-                              nullptr, err, 0) < 0)
+                              total_leaves, reinterpret_cast<uint32_t*>(A + o_leaf_sort_zero),
+                              reinterpret_cast<uint32_t*>(A + o_sort_counts), nullptr) < 0)
         return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
     launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, total_leaves);
 
@@ -656,7 +652,6 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     const uint32_t* child_rank = leaf_rank;
     uint8_t* masks = reinterpret_cast<uint8_t*>(A + o_masks);
     uint32_t* children = reinterpret_cast<uint32_t*>(A + o_children);
-    int step = 0;
     auto extract = [&](int l, int cur) {
         LevelExtractArgs e{};
         e.table_keys = reinterpret_cast<const uint64_t*>(A + o_tk[cur]);
@@ -668,14 +663,15 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
         e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
         e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
         e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
-        e.status = reinterpret_cast<uint64_t*>(A + o_status[l][1]);
-        e.tile_counter = &c->tile_counters[21 + 2 * step];
+        e.tile_counts = tile_counts;
+        e.offsets = tile_offsets;
+        e.slots_scratch = &c->table_slots;
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
-        e.err = err;
         launch_level_extract(st, e, level_slots);
     };
     // Levels below the sub-DAG roots.
+    int step = 0;
     for (int lc = lc_levels - 1; lc >= 0; --lc, ++step) {
         const int l = lc + k;
         const int cur = step & 1;
@@ -684,7 +680,7 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
         launch_table_clear(st, tk, &c->pre[l], &c->table_log2[l], shift, kTableOverflow, level_slots);
         launch_merge_insert(st, d_src + uint64_t(lc + 1) * nlive, uint32_t(nlive), totals[lc], child_slot_of,
                             child_rank, masks, children, tk, reinterpret_cast<uint32_t*>(A + o_trep[cur]),
-                            &c->table_log2[l], slot_of, c, 1);
+                            &c->table_log2[l], slot_of, c, 1, nullptr, nullptr);
         extract(l, cur);
         child_slot_of = slot_of;
         child_rank = reinterpret_cast<const uint32_t*>(A + o_trank[cur]);
@@ -692,7 +688,6 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     // New top levels over the chunk roots (DAGCompression.cu:1505-1533).
     for (int l = k - 1; l >= 0; --l, ++step) {
         const int cur = step & 1;
-        const int first_top = l == k - 1;
         LevelBuildArgs a{};
         a.child_codes = reinterpret_cast<const uint64_t*>(A + o_codes[(k - 1 - l) & 1]);
         a.child_slots = child_slot_of;
@@ -706,11 +701,10 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
         a.table_keys = reinterpret_cast<uint64_t*>(A + o_tk[cur]);
         a.table_rep = reinterpret_cast<uint32_t*>(A + o_trep[cur]);
         a.table_log2 = &c->table_log2[l];
-        a.status = reinterpret_cast<uint64_t*>(A + o_status[l][0]);
-        a.tile_counter = &c->tile_counters[20 + 2 * step];
+        a.tile_counts = tile_counts;
+        a.offsets = tile_offsets;
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
-        (void)first_top;
         launch_table_clear(st, a.table_keys, &c->pre[l], &c->table_log2[l], shift, kTableOverflow, level_slots);
         launch_level_build(st, a, nlive);
         extract(l, cur);
@@ -727,7 +721,6 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     TRY(finish(ctx, "merge_dags"));
     const BuildCounters& h = *ctx->h_counters;
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
-    if (h.error & kErrSpin) return fail(set_error(SVDAG_ERR_INTERNAL, "a device-side wait gave up"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "merged root level holds %llu nodes", (unsigned long long)h.uniq[0]));
     out->storage = storage;
     out->leaves = leaves_out;
@@ -907,37 +900,31 @@ int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag, uint32_t** 
     a.leaves = dag->leaves;
     a.num_leaves = dag->num_leaves;
     a.leaves_start = total_words - 2 * dag->num_leaves;
+    const uint64_t ntiles = std::max<uint64_t>(1, tiles(nodes, kFinalTileItems));
     Carver cv;
-    const uint64_t o_counter = cv.take(8);
-    const uint64_t o_err = cv.take(8);
-    const uint64_t o_status = cv.take(8 * ((nodes + 4095) / 4096 + 1));
-    const uint64_t zero_end = cv.off;
+    const uint64_t o_total = cv.take(8);
+    const uint64_t o_counts = cv.take(4 * ntiles);
+    const uint64_t o_offsets = cv.take(8 * ntiles);
     const uint64_t o_pos = cv.take(4 * std::max<uint64_t>(nodes, 1));
     TRY(ensure_arena(ctx, cv.off));
     void* words = nullptr;
     TRY(device_alloc(ctx, &words, 4 * std::max<uint64_t>(total_words, 1)));
     a.words = static_cast<uint32_t*>(words);
     a.positions = reinterpret_cast<uint32_t*>(ctx->arena + o_pos);
-    a.status = reinterpret_cast<uint64_t*>(ctx->arena + o_status);
-    a.tile_counter = reinterpret_cast<uint32_t*>(ctx->arena + o_counter);
-    a.err = reinterpret_cast<unsigned long long*>(ctx->arena + o_err);
-    launch_fill_u64(ctx->stream, reinterpret_cast<uint64_t*>(ctx->arena), zero_end / 8, 0);
-    launch_final_positions(ctx->stream, a);
-    launch_final_write(ctx->stream, a);
+    a.counts = reinterpret_cast<uint32_t*>(ctx->arena + o_counts);
+    a.offsets = reinterpret_cast<uint64_t*>(ctx->arena + o_offsets);
+    uint64_t* d_total = reinterpret_cast<uint64_t*>(ctx->arena + o_total);
+    launch_fill_u64(ctx->stream, d_total, 1, nodes);
+    launch_final(ctx->stream, a, d_total);
     const hipError_t e = hipGetLastError();
     mark(ctx, "final layout");
-    if (e != hipSuccess ||
-        hipMemcpyAsync(&ctx->h_counters->error, a.err, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+    if (e != hipSuccess) {
         device_release(ctx, words);
         return set_error(SVDAG_ERR_HIP, "final layout failed: %s", hipGetErrorString(e));
     }
     if (const int rc = finish(ctx, "create_final_dag")) {
         device_release(ctx, words);
         return rc;
-    }
-    if (ctx->h_counters->error) {
-        device_release(ctx, words);
-        return set_error(SVDAG_ERR_INTERNAL, "final layout: a device-side wait gave up");
     }
     *d_words = static_cast<uint32_t*>(words);
     *n_words = total_words;

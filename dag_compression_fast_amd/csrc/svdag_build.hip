@@ -1,21 +1,27 @@
 // svdag_build.hip -- gfx950 kernels of the bottom-up DAG build.
 //
 // Reference path (src/DAGCompression.cu):
-//   SelectUnique + leaf ReduceByKey  :1035-1068  -> k_leaf_pack (one pass over
-//       the sorted keys: duplicates OR into the same 4^3 mask, so no separate
-//       unique step is needed)
+//   SelectUnique + leaf ReduceByKey  :1035-1068  -> k_leaf_count / k_leaf_pack
+//       (one pass over the sorted keys: duplicates OR into the same 4^3 mask,
+//       so no separate unique step is needed)
 //   SortLevel(Leaves)               :1070-1071  -> k_table_clear / k_leaf_insert
-//       (hash table keyed by the mask) + k_table_compact + a key/value radix sort
-//       of the distinct masks + k_rank_of_slot
-//   ExtractLevel + ComputeHashes    :916-974, :827-833 -> k_level_build (parent
-//       masks, child ranks, Murmur hash, insert into the level's dedup table)
-//   SortLevel(level)                :864-914    -> k_level_extract (ordered
-//       linear-probing table: home slot = top bits of the hash, so sorting each
-//       probe cluster orders the whole table; rank = occupied slots before it)
-//   CreateFinalDAG / WriteTo        :834-861, :1550-1610 -> k_final_positions +
-//       k_final_write
-// Every kernel takes its element count from device memory and walks tiles from
-// an atomic counter, so the whole build runs without host round trips.
+//       (hash table keyed by the mask; the first inserter of a mask appends it
+//       to the unique list) + a key/value radix sort of that list +
+//       k_rank_of_slot
+//   ExtractLevel + ComputeHashes    :916-974, :827-833 -> k_level_count /
+//       k_level_build (parent masks, child ranks, Murmur hash, insert into the
+//       level's dedup table)
+//   SortLevel(level)                :864-914    -> k_extract_count /
+//       k_level_extract (ordered linear-probing table: home slot = top bits of
+//       the hash, so sorting each probe cluster orders the whole table; rank =
+//       occupied slots before it)
+//   CreateFinalDAG / WriteTo        :834-861, :1550-1610 -> k_final_count /
+//       k_final_positions / k_final_write
+// Every compaction is reduce-then-scan: a count kernel writes one count per
+// tile, k_scan_counts (one workgroup) turns them into offsets, and the main
+// kernel handles tile blockIdx.x + k * gridDim.x.  No workgroup waits on
+// another; element counts come from device memory, so the build runs without
+// host round trips.
 #include "svdag_device.hpp"
 #include "svdag_internal.hpp"
 
@@ -26,8 +32,9 @@ namespace svdag {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kScanThreads = 1024;
 
-__host__ int persistent_grid(uint64_t tiles, int per_cu)
+__host__ int grid_for(uint64_t tiles, int per_cu)
 {
     const uint64_t cap = uint64_t(device_cu_count()) * per_cu;
     return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
@@ -37,6 +44,27 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t count, uint64_t va
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = value;
+}
+
+// Exclusive scan of counts[0, ntiles) into offsets, ntiles = ceil(*items / per_tile);
+// the grand total goes to *total_out when given.  One workgroup.
+__global__ __launch_bounds__(kScanThreads) void k_scan_counts(const uint32_t* __restrict__ counts,
+                                                              uint64_t* __restrict__ offsets,
+                                                              const uint64_t* __restrict__ items, uint64_t per_tile,
+                                                              uint64_t* __restrict__ total_out)
+{
+    __shared__ uint64_t s_scan[kScanThreads / kWave];
+    const uint64_t ntiles = (*items + per_tile - 1) / per_tile;
+    uint64_t running = 0;
+    for (uint64_t chunk = 0; chunk < ntiles; chunk += kScanThreads) {
+        const uint64_t i = chunk + threadIdx.x;
+        const uint64_t v = i < ntiles ? counts[i] : 0;
+        uint64_t total;
+        const uint64_t ex = block_exclusive_scan64<kScanThreads>(v, s_scan, total);
+        if (i < ntiles) offsets[i] = running + ex;
+        running += total;
+    }
+    if (total_out && threadIdx.x == 0) *total_out = running;
 }
 
 // Chunk-local keys -> global keys: chunk index (Morton order of the sub-volumes,
@@ -69,47 +97,65 @@ __global__ void k_prefix_chunks(const K* __restrict__ in, uint64_t* __restrict__
 constexpr int kPackIpt = 16;
 constexpr int kPackTile = kThreads * kPackIpt;
 
+// Loads the thread's keys and marks run starts (code >> 6 changes) and
+// grand-parent starts (code >> 9 changes).
 template <typename K>
-__global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ sorted, BuildCounters* __restrict__ c,
-                                                        uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask,
-                                                        uint64_t* __restrict__ status, uint32_t* __restrict__ tile_counter,
-                                                        uint64_t* __restrict__ next_count, uint64_t code_mask)
+__device__ __forceinline__ void leaf_runs(const K* __restrict__ sorted, uint64_t n, uint64_t first, uint64_t code_mask,
+                                          uint64_t (&k)[kPackIpt], uint32_t& starts, uint32_t& gcount)
+{
+#pragma unroll
+    for (int j = 0; j < kPackIpt; ++j) k[j] = first + j < n ? (uint64_t(sorted[first + j]) & code_mask) : 0;
+    const uint64_t before = (first > 0 && first <= n) ? (uint64_t(sorted[first - 1]) & code_mask) : ~0ull;
+    starts = 0;
+    gcount = 0;
+#pragma unroll
+    for (int j = 0; j < kPackIpt; ++j) {
+        const uint64_t prev = j ? k[j - 1] : before;
+        const bool head = first + j == 0;
+        if (first + j < n && (head || (k[j] >> 6) != (prev >> 6))) {
+            starts |= 1u << j;
+            gcount += (head || (k[j] >> 9) != (prev >> 9)) ? 1u : 0u;
+        }
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_leaf_count(const K* __restrict__ sorted, const uint64_t* __restrict__ n_ptr,
+                                                         uint64_t code_mask, uint32_t* __restrict__ counts,
+                                                         uint64_t* __restrict__ next_count)
+{
+    __shared__ uint64_t s_red[kThreads / kWave];
+    const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
+    uint64_t gtot = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t k[kPackIpt];
+        uint32_t starts, gcount;
+        leaf_runs(sorted, n, tile * kPackTile + uint64_t(threadIdx.x) * kPackIpt, code_mask, k, starts, gcount);
+        const uint64_t cnt = block_reduce_sum<kThreads>(__popc(starts), s_red);
+        gtot += gcount;
+        if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
+    }
+    gtot = block_reduce_sum<kThreads>(gtot, s_red);
+    if (threadIdx.x == 0 && gtot) atomicAdd((unsigned long long*)next_count, (unsigned long long)gtot);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ sorted, const uint64_t* __restrict__ n_ptr,
+                                                        uint64_t code_mask, const uint64_t* __restrict__ offsets,
+                                                        uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
-    __shared__ uint64_t s_red[kThreads / kWave];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_tile;
-    const uint64_t n = c->n;
+    const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
-    for (;;) {
-        const uint32_t tile = acquire_tile(tile_counter, &s_tile);
-        if (tile >= ntiles) break;
-        const uint64_t first = uint64_t(tile) * kPackTile + uint64_t(threadIdx.x) * kPackIpt;
-        K k[kPackIpt];
-#pragma unroll
-        for (int j = 0; j < kPackIpt; ++j) k[j] = first + j < n ? K(sorted[first + j] & code_mask) : K(0);
-        const uint64_t before = (first > 0 && first <= n) ? (uint64_t(sorted[first - 1]) & code_mask) : ~0ull;
-        uint32_t starts = 0, gcount = 0;
-#pragma unroll
-        for (int j = 0; j < kPackIpt; ++j) {
-            const uint64_t cur = uint64_t(k[j]);
-            const uint64_t prev = j ? uint64_t(k[j - 1]) : before;
-            const bool head = first + j == 0;
-            if (first + j < n && (head || (cur >> 6) != (prev >> 6))) {
-                starts |= 1u << j;
-                gcount += (head || (cur >> 9) != (prev >> 9)) ? 1u : 0u;
-            }
-        }
-        uint32_t tile_total;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, tile_total);
-        const uint64_t gsum = block_reduce_sum<kThreads>(gcount, s_red);
-        uint64_t excl;
-        if (!block_lookback(status, tile, tile_total, (unsigned long long*)&c->error, &s_excl, excl)) continue;
-        if (threadIdx.x == 0) {
-            atomicAdd((unsigned long long*)&c->leaf_positions, (unsigned long long)tile_total);
-            atomicAdd((unsigned long long*)next_count, (unsigned long long)gsum);
-        }
-        uint64_t q = excl + local;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t first = tile * kPackTile + uint64_t(threadIdx.x) * kPackIpt;
+        uint64_t k[kPackIpt];
+        uint32_t starts, gcount;
+        leaf_runs(sorted, n, first, code_mask, k, starts, gcount);
+        uint32_t unused;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, unused);
+        uint64_t q = offsets[tile] + local;
         bool open = false;
         uint64_t run_code = 0, run_mask = 0;
 #pragma unroll
@@ -121,10 +167,10 @@ __global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ so
                     ++q;
                 }
                 open = true;
-                run_code = uint64_t(k[j]) >> 6;
+                run_code = k[j] >> 6;
                 run_mask = 0;
             }
-            if (open && first + j < n) run_mask |= 1ull << (uint64_t(k[j]) & 63);
+            if (open && first + j < n) run_mask |= 1ull << (k[j] & 63);
         }
         if (open) {
             for (uint64_t i = first + kPackIpt; i < n; ++i) {
@@ -151,11 +197,12 @@ __global__ void k_table_clear(uint64_t* __restrict__ keys, const uint64_t* __res
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) keys[i] = 0;
 }
 
-// Leaf dedup table: key = mask (never 0), home = fmix64(mask), wrap-around probing.
-// Inserts key h (never 0) into a wrap-around table of 2^lg slots; a probe never
-// exceeds the table size, and a full table reports kErrTableOverflow.
+// Inserts key h (never 0) into a wrap-around table of 2^lg slots; the first
+// inserter of a key appends (key, slot) to the unique list.  A probe never
+// exceeds the table size; a full table reports kErrTableOverflow.
 __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uint32_t lg, uint64_t h,
-                                                  unsigned long long* err)
+                                                  uint64_t* __restrict__ list_keys, uint32_t* __restrict__ list_slots,
+                                                  uint64_t* __restrict__ list_count, unsigned long long* err)
 {
     const uint64_t wrap = (1ull << lg) - 1;
     uint64_t slot = fmix64(h) >> (64 - lg);
@@ -164,7 +211,13 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
         if (cur == h) return uint32_t(slot);
         if (cur == 0) {
             const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
-            if (prev == 0 || prev == h) return uint32_t(slot);
+            if (prev == 0) {
+                const uint64_t idx = atomicAdd((unsigned long long*)list_count, 1ull);
+                list_keys[idx] = h;
+                list_slots[idx] = uint32_t(slot);
+                return uint32_t(slot);
+            }
+            if (prev == h) return uint32_t(slot);
         }
     }
     atomicOr(err, (unsigned long long)kErrTableOverflow);
@@ -173,58 +226,15 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
 
 __global__ void k_leaf_insert(const uint64_t* __restrict__ lp_mask, const uint64_t* __restrict__ count_ptr,
                               uint64_t* __restrict__ tk, const uint32_t* __restrict__ log2_ptr,
-                              uint32_t* __restrict__ lp_slot, unsigned long long* __restrict__ err)
+                              uint32_t* __restrict__ lp_slot, uint64_t* __restrict__ list_keys,
+                              uint32_t* __restrict__ list_slots, uint64_t* __restrict__ list_count,
+                              unsigned long long* __restrict__ err)
 {
     const uint64_t n = *count_ptr;
     const uint32_t lg = *log2_ptr;
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
-        lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], err);
-}
-
-// Occupied slots of a table, compacted in slot order.
-constexpr int kCompactIpt = 16;
-constexpr int kCompactTile = kThreads * kCompactIpt;
-
-__global__ __launch_bounds__(kThreads) void k_table_compact(const uint64_t* __restrict__ tk,
-                                                            const uint32_t* __restrict__ log2_ptr,
-                                                            uint64_t* __restrict__ list_keys,
-                                                            uint32_t* __restrict__ list_slots,
-                                                            uint64_t* __restrict__ count_out,
-                                                            uint64_t* __restrict__ status,
-                                                            uint32_t* __restrict__ tile_counter,
-                                                            unsigned long long* __restrict__ err)
-{
-    __shared__ uint32_t s_scan[kThreads / kWave];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_tile;
-    const uint64_t slots = 1ull << *log2_ptr;
-    const uint64_t ntiles = (slots + kCompactTile - 1) / kCompactTile;
-    for (;;) {
-        const uint32_t tile = acquire_tile(tile_counter, &s_tile);
-        if (tile >= ntiles) break;
-        const uint64_t first = uint64_t(tile) * kCompactTile + uint64_t(threadIdx.x) * kCompactIpt;
-        uint64_t key[kCompactIpt];
-        uint32_t occ = 0;
-#pragma unroll
-        for (int j = 0; j < kCompactIpt; ++j) {
-            key[j] = first + j < slots ? tk[first + j] : 0;
-            occ |= (key[j] != 0 ? 1u : 0u) << j;
-        }
-        uint32_t tile_total;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, tile_total);
-        uint64_t excl;
-        if (!block_lookback(status, tile, tile_total, err, &s_excl, excl)) continue;
-        if (threadIdx.x == 0) atomicAdd((unsigned long long*)count_out, (unsigned long long)tile_total);
-        uint64_t r = excl + local;
-#pragma unroll
-        for (int j = 0; j < kCompactIpt; ++j)
-            if ((occ >> j) & 1u) {
-                list_keys[r] = key[j];
-                list_slots[r] = uint32_t(first + j);
-                ++r;
-            }
-    }
+        lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], list_keys, list_slots, list_count, err);
 }
 
 __global__ void k_rank_of_slot(const uint32_t* __restrict__ sorted_slots, const uint64_t* __restrict__ count_ptr,
@@ -265,6 +275,48 @@ __device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, ui
 // ---------------------------------------------------------------------------
 constexpr int kBuildIpt = 8;
 constexpr int kBuildTile = kThreads * kBuildIpt;
+
+__device__ __forceinline__ void parent_runs(const uint64_t* __restrict__ codes, uint64_t n, uint64_t first,
+                                            uint64_t (&code)[kBuildIpt], uint32_t& starts, uint32_t& gcount)
+{
+#pragma unroll
+    for (int j = 0; j < kBuildIpt; ++j) code[j] = first + j < n ? codes[first + j] : 0;
+    const uint64_t before = (first > 0 && first <= n) ? codes[first - 1] : ~0ull;
+    starts = 0;
+    gcount = 0;
+#pragma unroll
+    for (int j = 0; j < kBuildIpt; ++j) {
+        const uint64_t prev = j ? code[j - 1] : before;
+        const bool head = first + j == 0;
+        if (first + j < n && (head || (code[j] >> 3) != (prev >> 3))) {
+            starts |= 1u << j;
+            gcount += (head || (code[j] >> 6) != (prev >> 6)) ? 1u : 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_level_count(const uint64_t* __restrict__ codes,
+                                                          const uint64_t* __restrict__ n_ptr,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint64_t* __restrict__ next_count)
+{
+    __shared__ uint64_t s_red[kThreads / kWave];
+    const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
+    uint64_t gtot = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t code[kBuildIpt];
+        uint32_t starts, gcount;
+        parent_runs(codes, n, tile * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt, code, starts, gcount);
+        const uint64_t cnt = block_reduce_sum<kThreads>(__popc(starts), s_red);
+        gtot += gcount;
+        if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
+    }
+    if (next_count) {
+        gtot = block_reduce_sum<kThreads>(gtot, s_red);
+        if (threadIdx.x == 0 && gtot) atomicAdd((unsigned long long*)next_count, (unsigned long long)gtot);
+    }
+}
 
 struct ParentAcc {
     uint64_t code;
@@ -309,39 +361,18 @@ __device__ __forceinline__ void acc_emit(const ParentAcc& a, uint64_t p, const L
 __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
-    __shared__ uint64_t s_red[kThreads / kWave];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_tile;
     const uint64_t n = *g.child_count;
     const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
     const uint32_t lg = *g.table_log2;
     const uint64_t limit = (1ull << lg) + kTableOverflow;
-    for (;;) {
-        const uint32_t tile = acquire_tile(g.tile_counter, &s_tile);
-        if (tile >= ntiles) break;
-        const uint64_t first = uint64_t(tile) * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t first = tile * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt;
         uint64_t code[kBuildIpt];
-#pragma unroll
-        for (int j = 0; j < kBuildIpt; ++j) code[j] = first + j < n ? g.child_codes[first + j] : 0;
-        const uint64_t before = (first > 0 && first <= n) ? g.child_codes[first - 1] : ~0ull;
-        uint32_t starts = 0, gcount = 0;
-#pragma unroll
-        for (int j = 0; j < kBuildIpt; ++j) {
-            const uint64_t prev = j ? code[j - 1] : before;
-            const bool head = first + j == 0;
-            if (first + j < n && (head || (code[j] >> 3) != (prev >> 3))) {
-                starts |= 1u << j;
-                gcount += (head || (code[j] >> 6) != (prev >> 6)) ? 1u : 0u;
-            }
-        }
-        uint32_t tile_total;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, tile_total);
-        const uint64_t gsum = block_reduce_sum<kThreads>(gcount, s_red);
-        uint64_t excl;
-        if (!block_lookback(g.status, tile, tile_total, (unsigned long long*)&g.counters->error, &s_excl, excl)) continue;
-        if (threadIdx.x == 0 && g.next_count) atomicAdd((unsigned long long*)g.next_count, (unsigned long long)gsum);
-
-        uint64_t p = excl + local;
+        uint32_t starts, gcount;
+        parent_runs(g.child_codes, n, first, code, starts, gcount);
+        uint32_t unused;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, unused);
+        uint64_t p = g.offsets[tile] + local;
         bool open = false;
         ParentAcc acc;
         acc_reset(acc, 0);
@@ -400,19 +431,35 @@ constexpr int kExtractIpt = 8;
 constexpr int kExtractTile = kThreads * kExtractIpt;
 constexpr int kHalo = 256;
 
+__global__ __launch_bounds__(kThreads) void k_extract_count(const uint64_t* __restrict__ table_keys,
+                                                            const uint32_t* __restrict__ table_log2,
+                                                            uint32_t* __restrict__ counts,
+                                                            uint64_t* __restrict__ slots_out)
+{
+    __shared__ uint64_t s_red[kThreads / kWave];
+    const uint64_t total = (1ull << *table_log2) + kTableOverflow;
+    const uint64_t ntiles = (total + kExtractTile - 1) / kExtractTile;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *slots_out = total;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t first = tile * kExtractTile;
+        uint32_t occ = 0;
+        for (uint32_t j = threadIdx.x; j < uint32_t(kExtractTile); j += kThreads)
+            occ += (first + j < total && table_keys[first + j] != 0) ? 1u : 0u;
+        const uint64_t cnt = block_reduce_sum<kThreads>(occ, s_red);
+        if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
 {
     __shared__ uint64_t s_win[kExtractTile + 2 * kHalo];
     __shared__ uint32_t s_scan[kThreads / kWave];
     __shared__ uint64_t s_red[kThreads / kWave];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_tile;
     const uint32_t lg = *g.table_log2;
     const uint64_t total = (1ull << lg) + kTableOverflow;
     const uint64_t ntiles = (total + kExtractTile - 1) / kExtractTile;
-    for (;;) {
-        const uint32_t tile = acquire_tile(g.tile_counter, &s_tile);
-        if (tile >= ntiles) break;
+    uint64_t wsum = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int64_t tile_base = int64_t(tile) * kExtractTile;
         const int64_t win_lo = tile_base - kHalo;
         const int64_t win_hi = tile_base + kExtractTile + kHalo;
@@ -430,15 +477,12 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
         uint32_t occ = 0;
 #pragma unroll
         for (int j = 0; j < kExtractIpt; ++j) occ |= (s_win[first + j - win_lo] != 0 ? 1u : 0u) << j;
-        uint32_t tile_total;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, tile_total);
-        uint64_t excl;
-        const bool ok = block_lookback(g.status, tile, tile_total, g.err, &s_excl, excl);
-        uint64_t before = excl + local;  // occupied slots before the current one
-        uint64_t wsum = 0;
+        uint32_t unused;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, unused);
+        uint64_t before = g.offsets[tile] + local;  // occupied slots before the current one
 #pragma unroll
         for (int j = 0; j < kExtractIpt; ++j) {
-            if (ok && ((occ >> j) & 1u)) {
+            if ((occ >> j) & 1u) {
                 const int64_t k = first + j;
                 const uint64_t key = s_win[k - win_lo];
                 int64_t js = k;
@@ -461,12 +505,10 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
                 ++before;
             }
         }
-        const uint64_t wtot = block_reduce_sum<kThreads>(wsum, s_red);
-        if (threadIdx.x == 0) {
-            atomicAdd((unsigned long long*)g.q_count, (unsigned long long)tile_total);
-            atomicAdd((unsigned long long*)g.w_count, (unsigned long long)wtot);
-        }
+        __syncthreads();  // the window is rewritten by the next tile
     }
+    const uint64_t wtot = block_reduce_sum<kThreads>(wsum, s_red);
+    if (threadIdx.x == 0 && wtot) atomicAdd((unsigned long long*)g.w_count, (unsigned long long)wtot);
 }
 
 // ---------------------------------------------------------------------------
@@ -475,6 +517,9 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
 // ---------------------------------------------------------------------------
 constexpr int kFinalIpt = 16;
 constexpr int kFinalTile = kThreads * kFinalIpt;
+static_assert(kPackTile == kPackTileItems && kBuildTile == kBuildTileItems && kExtractTile == kExtractTileItems &&
+                  kFinalTile == kFinalTileItems,
+              "tile sizes shared with the scratch planner");
 
 __device__ __forceinline__ int level_of(const FinalArgs& a, uint64_t g)
 {
@@ -483,34 +528,47 @@ __device__ __forceinline__ int level_of(const FinalArgs& a, uint64_t g)
     return l;
 }
 
+// Word counts (1 + popc(mask)) of the thread's nodes.
+__device__ __forceinline__ uint32_t node_words(const FinalArgs& a, uint64_t first, uint32_t (&sizes)[kFinalIpt])
+{
+    uint32_t sum = 0;
+    int l = first < a.total_nodes ? level_of(a, first) : 0;
+#pragma unroll
+    for (int j = 0; j < kFinalIpt; ++j) {
+        const uint64_t gi = first + j;
+        sizes[j] = 0;
+        if (gi < a.total_nodes) {
+            while (gi >= a.levels[l].first_node + a.levels[l].num_nodes) ++l;
+            sizes[j] = 1 + __popc(uint32_t(a.levels[l].masks[gi - a.levels[l].first_node]));
+        }
+        sum += sizes[j];
+    }
+    return sum;
+}
+
+__global__ __launch_bounds__(kThreads) void k_final_count(FinalArgs a)
+{
+    __shared__ uint64_t s_red[kThreads / kWave];
+    const uint64_t ntiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t sizes[kFinalIpt];
+        const uint32_t sum = node_words(a, tile * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt, sizes);
+        const uint64_t tot = block_reduce_sum<kThreads>(sum, s_red);
+        if (threadIdx.x == 0) a.counts[tile] = uint32_t(tot);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_final_positions(FinalArgs a)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_tile;
     const uint64_t ntiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
-    for (;;) {
-        const uint32_t tile = acquire_tile(a.tile_counter, &s_tile);
-        if (tile >= ntiles) break;
-        const uint64_t first = uint64_t(tile) * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t first = tile * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt;
         uint32_t sizes[kFinalIpt];
-        uint32_t sum = 0;
-        int l = first < a.total_nodes ? level_of(a, first) : 0;
-#pragma unroll
-        for (int j = 0; j < kFinalIpt; ++j) {
-            const uint64_t gi = first + j;
-            sizes[j] = 0;
-            if (gi < a.total_nodes) {
-                while (gi >= a.levels[l].first_node + a.levels[l].num_nodes) ++l;
-                sizes[j] = 1 + __popc(uint32_t(a.levels[l].masks[gi - a.levels[l].first_node]));
-            }
-            sum += sizes[j];
-        }
-        uint32_t tile_total;
-        const uint32_t local = block_exclusive_scan<kThreads>(sum, s_scan, tile_total);
-        uint64_t excl;
-        if (!block_lookback(a.status, tile, tile_total, a.err, &s_excl, excl)) continue;
-        uint64_t pos = excl + local;
+        const uint32_t sum = node_words(a, first, sizes);
+        uint32_t unused;
+        const uint32_t local = block_exclusive_scan<kThreads>(sum, s_scan, unused);
+        uint64_t pos = a.offsets[tile] + local;
 #pragma unroll
         for (int j = 0; j < kFinalIpt; ++j) {
             if (first + j < a.total_nodes) a.positions[first + j] = uint32_t(pos);
@@ -564,7 +622,8 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
                                const uint32_t* __restrict__ child_rank_of_slot, uint8_t* __restrict__ masks_out,
                                uint32_t* __restrict__ children_out, uint64_t* __restrict__ tk,
                                uint32_t* __restrict__ trep, const uint32_t* __restrict__ log2_ptr,
-                               uint32_t* __restrict__ slot_of, BuildCounters* __restrict__ c, int ordered)
+                               uint32_t* __restrict__ slot_of, BuildCounters* __restrict__ c, int ordered,
+                               uint64_t* __restrict__ list_keys, uint32_t* __restrict__ list_slots)
 {
     const uint32_t lg = *log2_ptr;
     const uint64_t limit = (1ull << lg) + kTableOverflow;
@@ -580,7 +639,8 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
         const uint64_t i = gi - s.first;
         const uint64_t key = s.hashes[i];
         if (!ordered) {
-            slot_of[gi] = hashed_insert(tk, lg, key, (unsigned long long*)&c->error);
+            slot_of[gi] = hashed_insert(tk, lg, key, list_keys, list_slots, &c->unique_leaves,
+                                        (unsigned long long*)&c->error);
             continue;
         }
         const uint4* cs = reinterpret_cast<const uint4*>(s.children + 8 * i);
@@ -605,8 +665,14 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
 void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
 {
     if (count == 0) return;
-    const int grid = persistent_grid((count + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(kThreads), 0, s, p, count, value);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, p, count,
+                       value);
+}
+
+void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
+                        uint64_t per_tile, uint64_t* total_out)
+{
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, s, counts, offsets, items, per_tile, total_out);
 }
 
 template <typename K>
@@ -614,68 +680,66 @@ void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint6
                           uint64_t n, uint32_t chunk_bits, BuildCounters* c)
 {
     if (n == 0) return;
-    const int grid = persistent_grid((n + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_prefix_chunks<K>, dim3(grid), dim3(kThreads), 0, s, in, out, d_offsets, num_chunks, n,
-                       chunk_bits, c);
+    hipLaunchKernelGGL(k_prefix_chunks<K>, dim3(grid_for((n + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, in,
+                       out, d_offsets, num_chunks, n, chunk_bits, c);
 }
 template void launch_prefix_chunks<uint64_t>(hipStream_t, const uint64_t*, uint64_t*, const uint64_t*, uint64_t,
                                              uint64_t, uint32_t, BuildCounters*);
 
 template <typename K>
 void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
-                      uint64_t* status, uint32_t* tile_counter, uint64_t n_bound, int leaf_level,
+                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
                       uint64_t code_mask)
 {
-    const int grid = persistent_grid((n_bound + kPackTile - 1) / kPackTile, 4);
-    hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, c, lp_code, lp_mask, status,
-                       tile_counter, &c->pre[leaf_level], code_mask);
+    const int grid = grid_for((n_bound + kPackTile - 1) / kPackTile, 8);
+    hipLaunchKernelGGL(k_leaf_count<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_counts,
+                       &c->pre[leaf_level]);
+    launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kPackTile, &c->leaf_positions);
+    hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_offsets,
+                       lp_code, lp_mask);
 }
 template void launch_leaf_pack<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint64_t*, uint64_t*,
-                                         uint64_t*, uint32_t*, uint64_t, int, uint64_t);
+                                         uint32_t*, uint64_t*, uint64_t, int, uint64_t);
 template void launch_leaf_pack<uint32_t>(hipStream_t, const uint32_t*, BuildCounters*, uint64_t*, uint64_t*,
-                                         uint64_t*, uint32_t*, uint64_t, int, uint64_t);
+                                         uint32_t*, uint64_t*, uint64_t, int, uint64_t);
 
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound)
 {
-    const int grid = persistent_grid((slot_bound + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_table_clear, dim3(grid), dim3(kThreads), 0, s, keys, count_ptr, log2_out, load_shift,
-                       overflow);
+    hipLaunchKernelGGL(k_table_clear, dim3(grid_for((slot_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       keys, count_ptr, log2_out, load_shift, overflow);
 }
 
 void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
-                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err)
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
+                        uint32_t* list_slots, uint64_t* list_count, unsigned long long* err)
 {
-    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_leaf_insert, dim3(grid), dim3(kThreads), 0, s, lp_mask, count_ptr, table_keys, log2_ptr,
-                       lp_slot, err);
-}
-
-void launch_table_compact(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t* list_keys,
-                          uint32_t* list_slots, uint64_t* count_out, uint64_t* status, uint32_t* tile_counter,
-                          uint64_t slot_bound, unsigned long long* err)
-{
-    const int grid = persistent_grid((slot_bound + kCompactTile - 1) / kCompactTile, 4);
-    hipLaunchKernelGGL(k_table_compact, dim3(grid), dim3(kThreads), 0, s, table_keys, log2_ptr, list_keys, list_slots,
-                       count_out, status, tile_counter, err);
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, list_keys, list_slots, list_count, err);
 }
 
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
                          uint32_t* rank_of_slot, uint64_t n_bound)
 {
-    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_rank_of_slot, dim3(grid), dim3(kThreads), 0, s, sorted_slots, count_ptr, rank_of_slot);
+    hipLaunchKernelGGL(k_rank_of_slot, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       sorted_slots, count_ptr, rank_of_slot);
 }
 
 void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound)
 {
-    const int grid = persistent_grid((child_bound + kBuildTile - 1) / kBuildTile, 4);
+    const int grid = grid_for((child_bound + kBuildTile - 1) / kBuildTile, 8);
+    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(kThreads), 0, s, a.child_codes, a.child_count, a.tile_counts,
+                       a.next_count);
+    launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.child_count, kBuildTile, nullptr);
     hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
 }
 
 void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound)
 {
-    const int grid = persistent_grid((slot_bound + kExtractTile - 1) / kExtractTile, 4);
+    const int grid = grid_for((slot_bound + kExtractTile - 1) / kExtractTile, 8);
+    hipLaunchKernelGGL(k_extract_count, dim3(grid), dim3(kThreads), 0, s, a.table_keys, a.table_log2, a.tile_counts,
+                       a.slots_scratch);
+    launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.slots_scratch, kExtractTile, a.q_count);
     hipLaunchKernelGGL(k_level_extract, dim3(grid), dim3(kThreads), 0, s, a);
 }
 
@@ -683,36 +747,33 @@ void launch_level_check(hipStream_t s, const uint32_t* parent_slots, const uint3
                         const uint8_t* parent_masks, const uint32_t* parent_children, const uint64_t* count_ptr,
                         uint64_t* collisions, uint64_t n_bound)
 {
-    const int grid = persistent_grid((n_bound + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_level_check, dim3(grid), dim3(kThreads), 0, s, parent_slots, table_rep, parent_masks,
-                       parent_children, count_ptr, collisions);
+    hipLaunchKernelGGL(k_level_check, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       parent_slots, table_rep, parent_masks, parent_children, count_ptr, collisions);
 }
 
-void launch_final_positions(hipStream_t s, const FinalArgs& a)
+void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev)
 {
-    if (a.total_nodes == 0) return;
-    const int grid = persistent_grid((a.total_nodes + kFinalTile - 1) / kFinalTile, 4);
-    hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
-}
-
-void launch_final_write(hipStream_t s, const FinalArgs& a)
-{
+    if (a.total_nodes == 0 && a.num_leaves == 0) return;
+    if (a.total_nodes) {
+        const int grid = grid_for((a.total_nodes + kFinalTile - 1) / kFinalTile, 8);
+        hipLaunchKernelGGL(k_final_count, dim3(grid), dim3(kThreads), 0, s, a);
+        launch_scan_counts(s, a.counts, a.offsets, total_nodes_dev, kFinalTile, nullptr);
+        hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
+    }
     const uint64_t work = std::max(a.total_nodes, a.num_leaves);
-    if (work == 0) return;
-    const int grid = persistent_grid((work + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_final_write, dim3(grid), dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL(k_final_write, dim3(grid_for((work + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, a);
 }
 
 void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,
                          const uint32_t* child_slot_of, const uint32_t* child_rank_of_slot, uint8_t* masks_out,
                          uint32_t* children_out, uint64_t* table_keys, uint32_t* table_rep, const uint32_t* table_log2,
-                         uint32_t* slot_of, BuildCounters* counters, int ordered)
+                         uint32_t* slot_of, BuildCounters* counters, int ordered, uint64_t* list_keys,
+                         uint32_t* list_slots)
 {
     if (total == 0) return;
-    const int grid = persistent_grid((total + kThreads - 1) / kThreads, 8);
-    hipLaunchKernelGGL(k_merge_insert, dim3(grid), dim3(kThreads), 0, s, d_sources, num_sources, total, child_slot_of,
-                       child_rank_of_slot, masks_out, children_out, table_keys, table_rep, table_log2, slot_of,
-                       counters, ordered);
+    hipLaunchKernelGGL(k_merge_insert, dim3(grid_for((total + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       d_sources, num_sources, total, child_slot_of, child_rank_of_slot, masks_out, children_out,
+                       table_keys, table_rep, table_log2, slot_of, counters, ordered, list_keys, list_slots);
 }
 
 }  // namespace svdag

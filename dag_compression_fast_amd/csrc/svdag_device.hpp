@@ -1,7 +1,6 @@
 // svdag_device.hpp -- device helpers shared by the gfx950 kernels.
 //
-// Wave64 scans and reductions, decoupled look-back over 64-bit status words,
-// agent-scope atomics, and the node hash (MurmurHash128xN of the reference,
+// Wave64 scans and reductions, agent-scope atomics, and the node hash (MurmurHash128xN of the reference,
 // src/Utils.h:119-154, as used by HashChildrenHashes, src/DAGCompression.cu:758-763).
 #pragma once
 
@@ -12,11 +11,6 @@ namespace svdag {
 
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr int kWave = 64;
-
-// Status words of a decoupled look-back: [63:62] flag, [61:0] value.
-constexpr uint64_t kFlagAggregate = 1ull << 62;
-constexpr uint64_t kFlagPrefix = 2ull << 62;
-constexpr uint64_t kValueMask = (1ull << 62) - 1;
 
 __device__ __forceinline__ uint64_t atomic_load_relaxed(const uint64_t* p)
 {
@@ -144,82 +138,31 @@ __device__ __forceinline__ uint64_t block_reduce_sum(uint64_t v, uint64_t* scrat
     return tot;
 }
 
-// ---------------------------------------------------------------------------
-// Decoupled look-back (single value per tile), executed by one full wave.
-// status[tile] receives aggregate / inclusive prefix; returns the exclusive
-// prefix of `tile` in every lane.  Tiles are handed out in increasing order by
-// an atomic counter, so every tile waited on is owned by a running or finished
-// workgroup.  The wait is bounded all the same: after kSpinLimit polls the
-// look-back gives up, sets kErrSpin in *err and reports failure, and the
-// caller skips every write that depends on the prefix.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr uint64_t kErrSpin = 4;
-
-__device__ __forceinline__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggregate,
-                                                  unsigned long long* err, bool& ok)
+// 64-bit exclusive block scan (two barriers; every thread must call it).
+template <int THREADS>
+__device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t* scratch, uint64_t& total)
 {
+    constexpr int W = THREADS / kWave;
+    const int w = threadIdx.x / kWave;
     const uint32_t lane = lane_id();
-    ok = true;
-    if (tile == 0) {
-        if (lane == 0) atomic_store_relaxed(&status[0], kFlagPrefix | aggregate);
-        return 0;
+    uint64_t inc = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint64_t t = __shfl_up(inc, d, kWave);
+        if (lane >= uint32_t(d)) inc += t;
     }
-    if (lane == 0) atomic_store_relaxed(&status[tile], kFlagAggregate | aggregate);
-    uint64_t exclusive = 0;
-    int64_t head = int64_t(tile) - 1;  // lane l inspects tile head - l
-    uint32_t spins = 0;
-    for (;;) {
-        const int64_t t = head - int64_t(lane);
-        const uint64_t s = t >= 0 ? atomic_load_relaxed(&status[t]) : kFlagPrefix;
-        const uint32_t flag = uint32_t(s >> 62);
-        const uint64_t prefix_lanes = __ballot(flag == 2);
-        const uint64_t waiting_lanes = __ballot(flag == 0);
-        // Lanes up to and including the nearest prefix (all lanes if none).
-        const uint64_t upto = prefix_lanes ? ((prefix_lanes & (~prefix_lanes + 1)) << 1) - 1 : ~0ull;
-        if (waiting_lanes & upto) {
-            if (++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(err, (unsigned long long)kErrSpin);
-                ok = false;
-                return 0;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        const uint64_t v = ((upto >> lane) & 1) ? (s & kValueMask) : 0;
-        exclusive += wave_reduce_sum(v);
-        if (prefix_lanes) break;
-        head -= kWave;
-    }
-    if (lane == 0) atomic_store_relaxed(&status[tile], kFlagPrefix | (exclusive + aggregate));
-    return exclusive;
-}
-
-// Block-wide helper: wave 0 runs the look-back; every thread must call it.
-// Returns false (in every thread) when the look-back gave up.
-__device__ __forceinline__ bool block_lookback(uint64_t* status, uint32_t tile, uint64_t aggregate,
-                                               unsigned long long* err, uint64_t* lds_slot, uint64_t& exclusive)
-{
-    if (threadIdx.x < kWave) {
-        bool ok;
-        const uint64_t e = lookback_wave(status, tile, aggregate, err, ok);
-        if (threadIdx.x == 0) *lds_slot = ok ? e : ~0ull;
+    if (lane == kWave - 1) scratch[w] = inc;
+    __syncthreads();
+    uint64_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const uint64_t x = scratch[i];
+        base += i < w ? x : 0;
+        tot += x;
     }
     __syncthreads();
-    const uint64_t e = *lds_slot;
-    __syncthreads();
-    exclusive = e;
-    return e != ~0ull;
-}
-
-// Tile id from a per-launch atomic counter, broadcast through LDS.
-__device__ __forceinline__ uint32_t acquire_tile(uint32_t* counter, uint32_t* lds_slot)
-{
-    if (threadIdx.x == 0) *lds_slot = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t t = *lds_slot;
-    __syncthreads();
-    return t;
+    total = tot;
+    return base + inc - v;
 }
 
 __host__ __device__ __forceinline__ uint32_t ceil_log2_u64(uint64_t v)

@@ -31,25 +31,23 @@ SortPlan make_sort_plan(int key_bits);
 
 inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
-// Device scratch of one sort: histograms (passes*256 u32) + look-back status
-// (passes * tiles * 256 u32).  Both must be zero before launch_sort.
-inline uint64_t sort_zero_words(const SortPlan& p, uint64_t n_bound)
-{
-    return uint64_t(p.passes) * kRadix * (1 + sort_tiles(n_bound));
-}
+// Device scratch of one sort: the all-pass digit histogram (passes * 256 u32,
+// zero before launch_sort) and the per-tile digit counts (sort_count_words u32,
+// any contents).
+inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) * kRadix; }
+inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound); }
 
-// LSD onesweep sort of keys (and optionally u32 values) on bits [0, key_bits).
+// LSD radix sort of keys (and optionally u32 values) on the plan's bits.
 // n is read from device memory (*n_ptr, < 2^30); n_bound is a host upper bound
-// sizing the grid.  Passes ping-pong between (keys_a, keys_b); the first pass
-// reads keys_in.  Returns which buffer holds the result (0 = a, 1 = b); if
-// final_keys / final_vals are non-null the last pass writes there instead.
-// `zero` points to sort_zero_words(...) zeroed u32 words; `tile_counters` to
-// `passes` zeroed u32 counters.
+// sizing the grid and the count buffer.  Passes ping-pong between (keys_a,
+// keys_b); the first pass reads keys_in.  Returns which buffer holds the result
+// (0 = a, 1 = b, 2 = final); if final_keys / final_vals are non-null the last
+// pass writes there instead.  key_or (optional) receives the OR of all keys.
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,
                 K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
-                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* tile_counters,
-                unsigned long long* key_or, unsigned long long* err, int grid_limit);
+                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
+                unsigned long long* key_or);
 
 // ---- build ----------------------------------------------------------------
 constexpr uint32_t kTableOverflow = 4096;  // slots past 2^log2 for ordered tables
@@ -66,16 +64,28 @@ struct BuildCounters {
     uint64_t collisions;
     uint64_t error;                           // kErr* bits
     uint32_t table_log2[SVDAG_MAX_LEVELS + 1];  // [l] level tables, [MAX] leaf table
-    uint32_t tile_counters[160];
+    uint64_t table_slots;                     // slots of the table being extracted
 };
 
 constexpr uint64_t kErrTableOverflow = 1;
 constexpr uint64_t kErrKeyRange = 2;
-// kErrSpin = 4 (svdag_device.hpp): a bounded wait gave up.
+
+// Tile sizes of the compaction stages; a stage over at most n items needs
+// tiles(n, kXxxTile) u32 counts and u64 offsets of scratch.
+constexpr uint64_t kPackTileItems = 4096;     // sorted keys per leaf-pack tile
+constexpr uint64_t kBuildTileItems = 2048;    // child codes per level-build tile
+constexpr uint64_t kExtractTileItems = 2048;  // table slots per extract tile
+constexpr uint64_t kFinalTileItems = 4096;    // nodes per final-layout tile
+inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
 
 int device_cu_count();
 
 void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value);
+
+// One-workgroup exclusive scan of tile counts into offsets; the number of tiles
+// is ceil(*items / per_tile); the total goes to *total_out when non-null.
+void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
+                        uint64_t per_tile, uint64_t* total_out);
 
 template <typename K>
 void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint64_t* d_offsets, uint64_t num_chunks,
@@ -83,7 +93,7 @@ void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint6
 
 template <typename K>
 void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
-                      uint64_t* status, uint32_t* tile_counter, uint64_t n_bound, int leaf_level,
+                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
                       uint64_t code_mask);
 
 // Clears an ordered (load_shift, overflow) or hashed table sized from *count_ptr
@@ -91,12 +101,11 @@ void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound);
 
+// Inserts every leaf mask into the hashed table; the first inserter of a mask
+// appends (mask, slot) to the unique list at atomicAdd(list_count).
 void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
-                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err);
-
-void launch_table_compact(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t* list_keys,
-                          uint32_t* list_slots, uint64_t* count_out, uint64_t* status, uint32_t* tile_counter,
-                          uint64_t slot_bound, unsigned long long* err);
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
+                        uint32_t* list_slots, uint64_t* list_count, unsigned long long* err);
 
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
                          uint32_t* rank_of_slot, uint64_t n_bound);
@@ -114,8 +123,8 @@ struct LevelBuildArgs {
     uint64_t* table_keys;
     uint32_t* table_rep;
     const uint32_t* table_log2;
-    uint64_t* status;
-    uint32_t* tile_counter;
+    uint32_t* tile_counts;    // tiles(child_bound, kBuildTileItems)
+    const uint64_t* offsets;  // same count, written by the scan
     uint64_t* next_count;     // M of the level above (atomic adds), or null
     BuildCounters* counters;
 };
@@ -131,11 +140,11 @@ struct LevelExtractArgs {
     uint64_t* out_hashes;
     uint8_t* out_masks;
     uint32_t* out_children;
-    uint64_t* status;
-    uint32_t* tile_counter;
-    uint64_t* q_count;
-    uint64_t* w_count;
-    unsigned long long* err;
+    uint32_t* tile_counts;    // tiles(slot_bound, kExtractTileItems)
+    const uint64_t* offsets;
+    uint64_t* slots_scratch;  // receives the table's slot count
+    uint64_t* q_count;        // receives the level's unique node count
+    uint64_t* w_count;        // receives the level's word count (atomic adds)
 };
 void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound);
 
@@ -159,12 +168,11 @@ struct FinalArgs {
     uint64_t leaves_start;   // word index of the first leaf
     uint32_t* positions;     // [total_nodes] absolute word offset of each node
     uint32_t* words;
-    uint64_t* status;
-    uint32_t* tile_counter;
-    unsigned long long* err;
+    uint32_t* counts;        // tiles(total_nodes, kFinalTileItems)
+    uint64_t* offsets;
 };
-void launch_final_positions(hipStream_t s, const FinalArgs& a);
-void launch_final_write(hipStream_t s, const FinalArgs& a);
+// total_nodes_dev: device copy of a.total_nodes (the scan reads it there).
+void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev);
 
 // ---- merge ----------------------------------------------------------------
 // Remaps and inserts the nodes of many sub-DAG levels into one shared table.
@@ -179,6 +187,7 @@ struct MergeSource {
 void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,
                          const uint32_t* child_slot_of, const uint32_t* child_rank_of_slot, uint8_t* masks_out,
                          uint32_t* children_out, uint64_t* table_keys, uint32_t* table_rep, const uint32_t* table_log2,
-                         uint32_t* slot_of, BuildCounters* counters, int ordered);
+                         uint32_t* slot_of, BuildCounters* counters, int ordered, uint64_t* list_keys,
+                         uint32_t* list_slots);
 
 }  // namespace svdag

@@ -1,17 +1,21 @@
-// svdag_sort.hip -- LSD onesweep radix sort for gfx950 (wave64).
+// svdag_sort.hip -- LSD radix sort for gfx950 (wave64), reduce-then-scan.
 //
 // Replaces the CUB DeviceRadixSort calls of the reference (SortKeys,
-// src/DAGCompression.cu:336-357; SortPairs, :359-394), which run three kernels
-// per digit pass.  Here one histogram kernel reads the keys once for all digit
-// positions, then each pass is a single kernel:
-//   * a workgroup takes tiles in increasing order from an atomic counter;
-//   * each wave ranks its keys with 8 ballots per key (lanes holding the same
-//     digit form a match mask; rank = popcount below the lane + a per-wave LDS
-//     counter), which keeps the sort stable;
-//   * per-digit tile counts are published and prefixed by decoupled look-back;
-//   * keys are staged in LDS in digit order, then written out in runs.
+// src/DAGCompression.cu:336-357; SortPairs, :359-394).  No workgroup ever waits
+// on another one: every pass is
+//   k_sort_count    tile t (a fixed range of 4096 keys) -> 256 digit counts
+//   k_sort_scan     one workgroup per digit scans that digit's row of tile
+//                   counts and adds the digit's global start (from the
+//                   all-pass histogram, read once up front)
+//   k_sort_scatter  tile t ranks its keys stably (8 ballots per key give the
+//                   lanes sharing a digit; rank = popcount below the lane plus a
+//                   per-wave LDS counter), stages them in LDS in digit order and
+//                   writes runs to offset[digit][t].
+// Tiles are assigned statically (blockIdx.x + k * gridDim.x).
 #include "svdag_device.hpp"
 #include "svdag_internal.hpp"
+
+#include <algorithm>
 
 namespace svdag {
 
@@ -32,16 +36,14 @@ SortPlan make_sort_plan(int key_bits)
 
 namespace {
 
-constexpr uint32_t kDigitAgg = 1u << 30;
-constexpr uint32_t kDigitPrefix = 2u << 30;
-constexpr uint32_t kDigitValue = (1u << 30) - 1;
-
 struct HistPlan {
     int passes;
     uint32_t shift[kMaxPasses];
     uint32_t mask[kMaxPasses];
 };
 
+// Digit totals of every pass in one read of the keys; OR of the keys for the
+// range check.
 template <typename K>
 __global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, const uint64_t* __restrict__ n_ptr,
                                                    HistPlan plan, uint32_t* __restrict__ hist,
@@ -72,82 +74,86 @@ __global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, c
     }
 }
 
-// Sequential per-digit look-back run by thread `d` of the tile.  The wait is
-// bounded (kSpinLimit polls); on giving up it sets kErrSpin and returns false.
-__device__ __forceinline__ bool digit_lookback(uint32_t* __restrict__ status, uint32_t tile, uint32_t d,
-                                               uint32_t total, unsigned long long* err, uint32_t& excl)
+// Per-tile digit counts, digit-major: counts[d * stride + tile].
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict__ keys,
+                                                             const uint64_t* __restrict__ n_ptr, uint32_t shift,
+                                                             uint32_t digit_mask, uint32_t* __restrict__ counts,
+                                                             uint64_t stride)
 {
-    excl = 0;
-    if (tile == 0) {
-        atomic_store_relaxed(&status[d], kDigitPrefix | total);
-        return true;
-    }
-    atomic_store_relaxed(&status[uint64_t(tile) * kRadix + d], kDigitAgg | total);
-    int64_t t = int64_t(tile) - 1;
-    uint32_t spins = 0;
-    for (;;) {
-        const uint32_t s = atomic_load_relaxed(&status[uint64_t(t) * kRadix + d]);
-        const uint32_t f = s >> 30;
-        if (f == 0) {
-            if (++spins > kSpinLimit) {
-                atomicOr(err, (unsigned long long)kErrSpin);
-                return false;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
+    __shared__ uint32_t s_cnt[kRadix];
+    const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (int d = threadIdx.x; d < kRadix; d += kSortThreads) s_cnt[d] = 0;
+        __syncthreads();
+        const uint64_t base = tile * kSortTile;
+#pragma unroll 4
+        for (int i = 0; i < kSortIpt; ++i) {
+            const uint64_t idx = base + uint64_t(i) * kSortThreads + threadIdx.x;
+            if (idx < n) atomicAdd(&s_cnt[uint32_t(keys[idx] >> shift) & digit_mask], 1u);
         }
-        excl += s & kDigitValue;
-        if (f == 2) break;
-        --t;
+        __syncthreads();
+        for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
+        __syncthreads();
     }
-    atomic_store_relaxed(&status[uint64_t(tile) * kRadix + d], kDigitPrefix | (excl + total));
-    return true;
+}
+
+// Block d: exclusive scan of row d of the tile counts, offset by the digit's
+// global start (sum of the pass histogram below d).  In place.
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ hist,
+                                                            const uint64_t* __restrict__ n_ptr, uint64_t stride)
+{
+    __shared__ uint64_t s_red[kScanThreads / kWave];
+    __shared__ uint32_t s_scan[kScanThreads / kWave];
+    const uint32_t d = blockIdx.x;
+    const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    // Global start of digit d.
+    uint64_t below = 0;
+    for (uint32_t i = threadIdx.x; i < d; i += kScanThreads) below += hist[i];
+    uint64_t running = block_reduce_sum<kScanThreads>(below, s_red);
+    uint32_t* row = counts + uint64_t(d) * stride;
+    for (uint64_t chunk = 0; chunk < ntiles; chunk += kScanThreads) {
+        const uint64_t i = chunk + threadIdx.x;
+        const uint32_t v = i < ntiles ? row[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, s_scan, total);
+        if (i < ntiles) row[i] = uint32_t(running + ex);
+        running += total;
+    }
 }
 
 template <typename K, bool PAIRS>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
-                                                           const uint32_t* __restrict__ vals_in,
-                                                           uint32_t* __restrict__ vals_out,
-                                                           const uint64_t* __restrict__ n_ptr, uint32_t shift,
-                                                           uint32_t digit_mask, const uint32_t* __restrict__ hist,
-                                                           uint32_t* __restrict__ status,
-                                                           uint32_t* __restrict__ tile_counter,
-                                                           unsigned long long* __restrict__ err)
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+                                                               const uint32_t* __restrict__ vals_in,
+                                                               uint32_t* __restrict__ vals_out,
+                                                               const uint64_t* __restrict__ n_ptr, uint32_t shift,
+                                                               uint32_t digit_mask,
+                                                               const uint32_t* __restrict__ offsets, uint64_t stride)
 {
     constexpr int W = kSortThreads / kWave;
     __shared__ K s_keys[kSortTile];
     __shared__ uint32_t s_vals[PAIRS ? kSortTile : 1];
     __shared__ uint32_t s_wcnt[W][kRadix];
-    __shared__ uint32_t s_gstart[kRadix];
     __shared__ uint32_t s_lstart[kRadix];
     __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_scan[W];
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_fail;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t w = tid / kWave;
     const uint32_t lane = lane_id();
-    const uint32_t n = uint32_t(*n_ptr);
-    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
 
-    {
-        uint32_t total;
-        const uint32_t g = block_exclusive_scan<kSortThreads>(tid < kRadix ? hist[tid] : 0u, s_scan, total);
-        if (tid < kRadix) s_gstart[tid] = g;
-    }
-
-    for (;;) {
-        if (tid == 0) {
-            s_tile = atomicAdd(tile_counter, 1u);
-            s_fail = 0;
-        }
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         for (int i = tid; i < W * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= ntiles) break;
-        const uint32_t tile_base = tile * kSortTile;
-        const uint32_t tile_n = min(uint32_t(kSortTile), n - tile_base);
+        const uint64_t tile_base = tile * kSortTile;
+        const uint32_t tile_n = uint32_t(min(uint64_t(kSortTile), n - tile_base));
 
         K key[kSortIpt];
         uint32_t val[PAIRS ? kSortIpt : 1];
@@ -196,13 +202,11 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__
                 total += c;
             }
         }
-        uint32_t excl = 0;
-        if (tid < kRadix && !digit_lookback(status, tile, tid, total, err, excl)) s_fail = 1;
         uint32_t unused;
         const uint32_t lstart = block_exclusive_scan<kSortThreads>(tid < kRadix ? total : 0u, s_scan, unused);
         if (tid < kRadix) {
             s_lstart[tid] = lstart;
-            s_base[tid] = s_gstart[tid] + excl - lstart;
+            s_base[tid] = offsets[uint64_t(tid) * stride + tile] - lstart;
         }
         __syncthreads();
 
@@ -217,8 +221,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__
             }
         }
         __syncthreads();
-        const bool ok = s_fail == 0;
-        for (uint32_t j = tid; ok && j < tile_n; j += kSortThreads) {
+        for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
             const K k = s_keys[j];
             const uint32_t d = uint32_t(k >> shift) & digit_mask;
             const uint32_t o = s_base[d] + j;
@@ -247,13 +250,11 @@ int device_cu_count()
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in, K* keys_a,
                 K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
-                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* tile_counters,
-                unsigned long long* key_or, unsigned long long* err, int grid_limit)
+                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts, unsigned long long* key_or)
 {
     if (plan.passes == 0 || n_bound == 0) return -1;
     uint32_t* hist = zero;
-    uint32_t* status = zero + plan.passes * kRadix;
-    const uint64_t tiles = sort_tiles(n_bound);
+    const uint64_t stride = sort_tiles(n_bound);
     HistPlan hp{};
     hp.passes = plan.passes;
     for (int p = 0; p < plan.passes; ++p) {
@@ -262,13 +263,14 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
     }
     const int cus = device_cu_count();
     const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 8));
-    hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist, key_or);
+    hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist,
+                       key_or);
 
     const bool pairs = vals_in != nullptr;
     const K* src = keys_in;
     const uint32_t* vsrc = vals_in;
     int where = -1;
-    uint64_t grid = std::min<uint64_t>(tiles, uint64_t(grid_limit > 0 ? grid_limit : cus * 2));
+    const uint32_t grid = uint32_t(std::min<uint64_t>(stride, uint64_t(cus) * 8));
     for (int p = 0; p < plan.passes; ++p) {
         const bool last = p + 1 == plan.passes;
         K* dst = (p % 2 == 0) ? keys_a : keys_b;
@@ -279,15 +281,17 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
             where = 2;
         }
         if (last && final_vals) vdst = final_vals;
-        uint32_t* st = status + uint64_t(p) * tiles * kRadix;
+        const uint32_t dmask = (1u << plan.bits[p]) - 1u;
+        hipLaunchKernelGGL(k_sort_count<K>, dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, plan.shift[p],
+                           dmask, counts, stride);
+        hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix, n_ptr,
+                           stride);
         if (pairs)
-            hipLaunchKernelGGL((k_onesweep<K, true>), dim3(uint32_t(grid)), dim3(kSortThreads), 0, stream, src, dst,
-                               vsrc, vdst, n_ptr, plan.shift[p], (1u << plan.bits[p]) - 1u, hist + p * kRadix, st,
-                               tile_counters + p, err);
+            hipLaunchKernelGGL((k_sort_scatter<K, true>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
+                               vdst, n_ptr, plan.shift[p], dmask, counts, stride);
         else
-            hipLaunchKernelGGL((k_onesweep<K, false>), dim3(uint32_t(grid)), dim3(kSortThreads), 0, stream, src, dst,
-                               nullptr, nullptr, n_ptr, plan.shift[p], (1u << plan.bits[p]) - 1u, hist + p * kRadix,
-                               st, tile_counters + p, err);
+            hipLaunchKernelGGL((k_sort_scatter<K, false>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
+                               nullptr, nullptr, n_ptr, plan.shift[p], dmask, counts, stride);
         src = dst;
         vsrc = vdst;
     }
@@ -296,9 +300,9 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
 
 template int launch_sort<uint64_t>(hipStream_t, const SortPlan&, const uint64_t*, const uint32_t*, uint64_t*,
                                    uint64_t*, uint32_t*, uint32_t*, uint64_t*, uint32_t*, const uint64_t*, uint64_t,
-                                   uint32_t*, uint32_t*, unsigned long long*, unsigned long long*, int);
+                                   uint32_t*, uint32_t*, unsigned long long*);
 template int launch_sort<uint32_t>(hipStream_t, const SortPlan&, const uint32_t*, const uint32_t*, uint32_t*,
                                    uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint64_t*, uint64_t,
-                                   uint32_t*, uint32_t*, unsigned long long*, unsigned long long*, int);
+                                   uint32_t*, uint32_t*, unsigned long long*);
 
 }  // namespace svdag

@@ -28,5 +28,7 @@ def gpu():
     from dag_compression_fast_amd import _lib
     if _lib.gpu_count() < 1:
         pytest.fail("no HIP device visible: GPU tests need an MI355X")
-    svd.default_context()
+    # The library fails a call whose device work does not finish in time and
+    # names the stage; keep that well inside the per-test timeout.
+    svd.default_context().set_option(_lib.OPT_WAIT_SECONDS, 60)
     return svd

@@ -76,11 +76,15 @@ T __shfl_xor(T v, int m, int = 64)
     return out;
 }
 
+#ifndef __HIP_MEMORY_SCOPE_AGENT
 #define __HIP_MEMORY_SCOPE_AGENT 3
+#endif
+#ifndef __clang__  // clang provides these as builtins
 template <class T>
 T __hip_atomic_load(const T* p, int, int) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
 template <class T, class U>
 void __hip_atomic_store(T* p, U v, int, int) { __atomic_store_n(p, T(v), __ATOMIC_RELAXED); }
+#endif
 
 template <class T, class U>
 T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, T(v), __ATOMIC_SEQ_CST); }
