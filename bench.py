@@ -1,0 +1,234 @@
+"""Benchmark of the plain GPU DAG build (one MI355X).
+
+Workload (default): triangle surfaces in a 2048^3 grid (depth 11): 20k seeded
+triangles (v0 uniform in the grid, edges lognormal mu=4 sigma=1 voxels),
+26-separating surface voxelization, 30% duplicates injected, shuffled uint64
+Morton keys.  One step = svdag_create_dag + svdag_create_final_dag on keys
+already resident in HBM (sort, leaf pack, leaf and level dedup, per-level arrays
+in ascending-hash order, flat uint32 layout), as DAGCompression::CreateDAG +
+CreateFinalDAG do (src/DAGCompression.cu:1003-1107, 1550-1610).
+
+Prints one JSON line: input voxels/s (keys including duplicates / step time),
+the roofline of the dominant kernel (algorithmic bytes per launch / its average
+launch time, measured with HIP events on the library's stream during the timed
+steps), and the CPU oracle timed on the same keys on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "input voxels/s to bit-exact DAG, batched vs plain on 1 GPU; p99 chunk latency"
+
+WORKLOADS = {
+    "triangles2048": dict(desc="triangle surfaces 2048^3 (depth 11): 20k triangles, edge lognormal(4,1), "
+                               "26-separating voxelization, 30% duplicates, shuffled u64 Morton keys",
+                          depth=11, gen=("triangles", 0x5EED0001, 20000, 4.0, 1.0, 0.3)),
+    "spheres256": dict(desc="sphere shells 256^3 (depth 8): 64 spheres r~U[8,40], 1-voxel shell, "
+                            "~20% duplicates, shuffled u64 Morton keys",
+                       depth=8, gen=("spheres", 0x5EED0000, 64, 8.0, 40.0, 0.25)),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_keys(svd, spec, threads):
+    kind, seed, count, a, b, dup = spec["gen"]
+    if kind == "triangles":
+        return svd.gen_triangle_surfaces(seed, spec["depth"], count, a, b, dup, threads=threads)
+    return svd.gen_sphere_shells(seed, spec["depth"], count, a, b, dup, threads=threads)
+
+
+def algorithmic_bytes(kernel, st, key_bytes, key_passes):
+    """Algorithmic HBM bytes one step moves through `kernel` (None if not modelled).
+
+    N keys, L0 leaf positions, L unique leaves, M_l / Q_l nodes of level l before /
+    after dedup, C_l children feeding level l, W words of the flat layout."""
+    n, l0, lu = st["n"], st["leaf_positions"], st["num_leaves"]
+    pre, uniq, words = st["pre"], st["uniq"], st["words"]
+    levels = len(pre)
+    child = [pre[l + 1] if l + 1 < levels else l0 for l in range(levels)]
+    leaf_passes = 8
+    table = {
+        # Every key pass reads and writes the keys; the unique-leaf sort moves (u64 mask, u32 slot) pairs.
+        "sort_scatter_keys": key_passes * 2 * key_bytes * n,
+        "sort_scatter_pairs": leaf_passes * 2 * 12 * lu,
+        "sort_count": key_passes * key_bytes * n + leaf_passes * 8 * lu,
+        "sort_histogram": key_bytes * n + 8 * lu,
+        "leaf_count": key_bytes * n,
+        "leaf_pack": key_bytes * n + 16 * l0,
+        # mask read, slot write, unique list append
+        "leaf_insert": 12 * l0 + 12 * lu,
+        # child code + slot + rank + hash reads; parent code, slot, mask, 8 children writes
+        "level_build": sum(24 * child[l] + 45 * pre[l] for l in range(levels)),
+        # rep + mask + children reads, rank + hash + mask + children writes per unique node
+        "level_extract": sum(82 * uniq[l] for l in range(levels)),
+        "final_write": sum(33 * uniq[l] for l in range(levels)) + 4 * sum(words) + 16 * lu,
+    }
+    return table.get(kernel)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"HBM bytes per launch \(scaled\):\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(keys, depth, threads, reps):
+    """The CPU oracle (a restatement of the reference algorithm) on the same keys."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes
+    oracle_ctypes.create_dag(keys, depth, threads=threads)  # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d = oracle_ctypes.create_dag(keys, depth, threads=threads)
+        oracle_ctypes.create_final_dag(d)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), d
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="triangles2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads for input generation and the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the build runs on a single "
+              "MI355X; sharding across GPUs is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import dag_compression_fast_amd as svd
+    from dag_compression_fast_amd import _lib
+
+    if _lib.gpu_count() < 1:
+        sys.exit("bench.py needs a visible GPU")
+    spec = WORKLOADS[args.workload]
+    depth = spec["depth"]
+    log(f"generating {args.workload} on {args.threads} host threads")
+    keys = make_keys(svd, spec, args.threads)
+    n = len(keys)
+    log(f"{n} keys, depth {depth}")
+
+    ctx = svd.Context()
+    L = _lib.lib()
+    d_keys = svd.DeviceBuffer.from_numpy(keys)
+    H = _lib.hip()
+
+    def step():
+        dag = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag(ctx.handle, d_keys.ptr, n, depth, ctypes.byref(dag)))
+        words, nw = ctypes.c_void_p(), ctypes.c_uint64()
+        _lib.check(L.svdag_create_final_dag(ctx.handle, ctypes.byref(dag), ctypes.byref(words), ctypes.byref(nw)))
+        return dag, words, int(nw.value)
+
+    def release(dag, words):
+        L.svdag_free_words(ctx.handle, words)
+        L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+
+    # Warm-up, with every kernel timed once to find the dominant one.
+    ctx.profile_enable(None)
+    for i in range(max(1, args.warmup)):
+        if i == max(1, args.warmup) - 1:
+            ctx.profile_reset()
+        dag, words, nw = step()
+        if i < max(1, args.warmup) - 1:
+            release(dag, words)
+    table = ctx.profile_read()
+    stats = {
+        "n": n, "leaf_positions": int(dag.leaf_positions), "num_leaves": int(dag.num_leaves),
+        "pre": [int(dag.pre_dedup[l]) for l in range(dag.num_levels)],
+        "uniq": [int(dag.levels[l].num_nodes) for l in range(dag.num_levels)],
+        "words": [int(dag.level_words[l]) for l in range(dag.num_levels)],
+    }
+    result_words = np.empty(nw, dtype=np.uint32)
+    _lib.hip_check(H.hipMemcpy(result_words.ctypes.data, words, 4 * nw, _lib.HIP_D2H), "hipMemcpy")
+    level_counts = stats["uniq"]
+    release(dag, words)
+    dominant = max(table, key=lambda k: table[k][0])
+    if args.kernel_table:
+        tot = sum(v[0] for v in table.values())
+        for k, (ms, cnt) in sorted(table.items(), key=lambda kv: -kv[1][0]):
+            log(f"  {k:20s} {ms:8.3f} ms  {cnt:4d} launches  {100 * ms / tot:5.1f}%")
+    log(f"dominant kernel: {dominant}")
+
+    # Timed steps: only the dominant kernel carries timing events.
+    ctx.profile_enable([dominant])
+    ctx.profile_reset()
+    _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dag, words, _nw = step()
+        release(dag, words)
+    _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    kms, launches = ctx.profile_read()[dominant]
+    ctx.profile_enable([])
+
+    key_passes = -(-3 * depth // 8)
+    alg = algorithmic_bytes(dominant, stats, 8, key_passes)
+    launches_per_step = launches / args.steps
+    roofline = None
+    if alg is not None:
+        per_launch = alg / launches_per_step
+        avg_s = kms / launches / 1e3
+        achieved = per_launch / avg_s / 1e9
+        traffic = read_traffic(dominant, args.workload)
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                    "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    cpu = None
+    check = "not checked"
+    if not args.no_cpu_baseline:
+        log(f"CPU oracle on the same {n} keys, {args.threads} threads")
+        cpu_s, ref = cpu_baseline(keys, depth, args.threads, args.cpu_reps)
+        cpu = {"value": round(n / cpu_s, 1), "unit": "voxels/s", "cores": args.threads, "kind": "port",
+               "sample": f"full workload ({n} keys), median of {args.cpu_reps} after 1 warm-up, "
+                         "create_dag + create_final_dag"}
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ctypes
+        ref_counts = [len(lv["hashes"]) for lv in ref["levels"]]
+        same = ref_counts == level_counts and np.array_equal(oracle_ctypes.create_final_dag(ref), result_words)
+        check = "bit-exact vs oracle (level counts + flat layout)" if same else "MISMATCH vs oracle"
+        if not same:
+            log("GPU result differs from the oracle")
+
+    out = {
+        "metric": METRIC, "value": round(n / (ms_per_step / 1e3), 1), "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "check": check,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0 if check != "MISMATCH vs oracle" else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -88,6 +88,38 @@ class Context:
     def free_all(self):
         _lib.check(_lib.lib().svdag_free_all(self.handle))
 
+    # Per-kernel timing (svdag_profile_*): HIP event pairs on the context's stream.
+    @staticmethod
+    def kernel_names():
+        names = []
+        while True:
+            n = _lib.lib().svdag_kernel_name(len(names))
+            if n is None:
+                return names
+            names.append(n.decode())
+
+    def profile_enable(self, kernels=None):
+        """Times every launch of the named kernels (None: all, []: off)."""
+        names = self.kernel_names()
+        sel = names if kernels is None else kernels
+        mask = 0
+        for k in sel:
+            mask |= 1 << names.index(k)
+        _lib.check(_lib.lib().svdag_profile_enable(self.handle, mask))
+
+    def profile_reset(self):
+        _lib.check(_lib.lib().svdag_profile_reset(self.handle))
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} for the kernels timed since the last reset."""
+        out = {}
+        ms, cnt = ctypes.c_double(), ctypes.c_uint64()
+        for i, name in enumerate(self.kernel_names()):
+            _lib.check(_lib.lib().svdag_profile_read(self.handle, i, ctypes.byref(ms), ctypes.byref(cnt)))
+            if cnt.value:
+                out[name] = (ms.value, int(cnt.value))
+        return out
+
     def close(self):
         if self.handle is not None and self.handle.value:
             _lib.lib().svdag_context_destroy(self.handle)

@@ -79,42 +79,45 @@ def lib():
 
 def bind_library(L):
     """Declares the C signatures of include/svdag.h on a loaded library."""
-    if True:
-        vp, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64
-        dagp = ctypes.POINTER(SvdagDag)
-        sig = {
-            "svdag_context_create": ([ctypes.POINTER(vp), vp], i32),
-            "svdag_context_destroy": ([vp], i32),
-            "svdag_set_option": ([vp, i32, i64], i32),
-            "svdag_scratch_bytes": ([u64, i32, i32], u64),
-            "svdag_create_dag": ([vp, vp, u64, i32, dagp], i32),
-            "svdag_create_dag_u32": ([vp, vp, u64, i32, dagp], i32),
-            "svdag_merge_dags": ([vp, dagp, u64, dagp], i32),
-            "svdag_create_dag_batched": ([vp, vp, ctypes.POINTER(u64), u64, i32, dagp], i32),
-            "svdag_create_final_dag": ([vp, dagp, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
-            "svdag_dag_to_host": ([vp, dagp, dagp], i32),
-            "svdag_dag_to_device": ([vp, dagp, dagp], i32),
-            "svdag_free_dag": ([vp, dagp], i32),
-            "svdag_free_words": ([vp, vp], i32),
-            "svdag_free_all": ([vp], i32),
-            "svdag_last_error": ([], ctypes.c_char_p),
-            "svdag_morton_encode": ([ctypes.c_uint32] * 3, u64),
-            "svdag_gen_sphere_shells": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, i32,
+    vp, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64
+    dagp = ctypes.POINTER(SvdagDag)
+    sig = {
+        "svdag_context_create": ([ctypes.POINTER(vp), vp], i32),
+        "svdag_context_destroy": ([vp], i32),
+        "svdag_set_option": ([vp, i32, i64], i32),
+        "svdag_scratch_bytes": ([u64, i32, i32], u64),
+        "svdag_create_dag": ([vp, vp, u64, i32, dagp], i32),
+        "svdag_create_dag_u32": ([vp, vp, u64, i32, dagp], i32),
+        "svdag_merge_dags": ([vp, dagp, u64, dagp], i32),
+        "svdag_create_dag_batched": ([vp, vp, ctypes.POINTER(u64), u64, i32, dagp], i32),
+        "svdag_create_final_dag": ([vp, dagp, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
+        "svdag_dag_to_host": ([vp, dagp, dagp], i32),
+        "svdag_dag_to_device": ([vp, dagp, dagp], i32),
+        "svdag_free_dag": ([vp, dagp], i32),
+        "svdag_free_words": ([vp, vp], i32),
+        "svdag_free_all": ([vp], i32),
+        "svdag_last_error": ([], ctypes.c_char_p),
+        "svdag_profile_enable": ([vp, u64], i32),
+        "svdag_profile_reset": ([vp], i32),
+        "svdag_profile_read": ([vp, i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)], i32),
+        "svdag_kernel_name": ([i32], ctypes.c_char_p),
+        "svdag_morton_encode": ([ctypes.c_uint32] * 3, u64),
+        "svdag_gen_sphere_shells": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, i32,
+                                     ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
+        "svdag_gen_triangle_surfaces": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, i32,
                                          ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
-            "svdag_gen_triangle_surfaces": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, i32,
-                                             ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
-            "svdag_gen_value_noise": ([u64, i32, i32, ctypes.c_uint32, u64, i32, ctypes.POINTER(vp),
-                                       ctypes.POINTER(u64)], i32),
-            "svdag_gen_chunk_plan": ([u64, u64, ctypes.c_double, ctypes.c_double, u64, u64, ctypes.POINTER(u64),
-                                      ctypes.POINTER(u64), ctypes.POINTER(i32)], i32),
-            "svdag_gen_chunk": ([u64, u64, i32, ctypes.c_double, ctypes.c_double, u64, u64, vp, u64,
-                                 ctypes.POINTER(u64)], i32),
-            "svdag_gen_free": ([vp], None),
-        }
-        for name, (args, res) in sig.items():
-            fn = getattr(L, name)
-            fn.argtypes = args
-            fn.restype = res
+        "svdag_gen_value_noise": ([u64, i32, i32, ctypes.c_uint32, u64, i32, ctypes.POINTER(vp),
+                                   ctypes.POINTER(u64)], i32),
+        "svdag_gen_chunk_plan": ([u64, u64, ctypes.c_double, ctypes.c_double, u64, u64, ctypes.POINTER(u64),
+                                  ctypes.POINTER(u64), ctypes.POINTER(i32)], i32),
+        "svdag_gen_chunk": ([u64, u64, i32, ctypes.c_double, ctypes.c_double, u64, u64, vp, u64,
+                             ctypes.POINTER(u64)], i32),
+        "svdag_gen_free": ([vp], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
     return L
 
 
@@ -145,22 +148,21 @@ def hip():
 
 def bind_hip(H):
     """Declares the HIP runtime signatures used by the Python layer."""
-    if True:
-        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-        H.hipFree.argtypes = [ctypes.c_void_p]
-        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        H.hipDeviceSynchronize.argtypes = []
-        H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
-        H.hipGetErrorString.restype = ctypes.c_char_p
-        H.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
-        H.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        H.hipEventSynchronize.argtypes = [ctypes.c_void_p]
-        H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
-        H.hipEventDestroy.argtypes = [ctypes.c_void_p]
-        H.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
-        H.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
-        H.hipStreamDestroy.argtypes = [ctypes.c_void_p]
-        H.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    H.hipFree.argtypes = [ctypes.c_void_p]
+    H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    H.hipDeviceSynchronize.argtypes = []
+    H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    H.hipGetErrorString.restype = ctypes.c_char_p
+    H.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    H.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    H.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    H.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    H.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    H.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    H.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    H.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     return H
 
 

@@ -25,6 +25,99 @@
 
 using namespace svdag;
 
+namespace {
+
+const char* const kKernelNames[kNumKernelIds] = {
+    "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
+    "scan_counts",    "prefix_chunks", "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
+    "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
+    "final_count",    "final_positions", "final_write", "merge_insert"};
+
+// Event pairs around the launches of the kernels selected by `mask`; resolved
+// into per-kernel totals once the stream has drained.
+struct EventRecorder final : KernelRecorder {
+    uint64_t mask = 0;
+    std::vector<hipEvent_t> pool;
+    struct Pending {
+        int id;
+        hipEvent_t begin, end;
+    };
+    std::vector<Pending> pending;
+    hipEvent_t open[kNumKernelIds] = {};
+    double total_ms[kNumKernelIds] = {};
+    uint64_t launches[kNumKernelIds] = {};
+
+    hipEvent_t take()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return e;
+    }
+    void begin(hipStream_t s, int id) override
+    {
+        if (!((mask >> id) & 1) || open[id]) return;
+        hipEvent_t e = take();
+        if (e && hipEventRecord(e, s) == hipSuccess) open[id] = e;
+        else if (e) pool.push_back(e);
+    }
+    void end(hipStream_t s, int id) override
+    {
+        if (!open[id]) return;
+        hipEvent_t e = take();
+        if (e && hipEventRecord(e, s) == hipSuccess) pending.push_back({id, open[id], e});
+        else {
+            pool.push_back(open[id]);
+            if (e) pool.push_back(e);
+        }
+        open[id] = nullptr;
+    }
+    // The stream has drained: every pending event is complete.
+    void resolve(bool keep)
+    {
+        for (const Pending& p : pending) {
+            float ms = 0.f;
+            if (keep && hipEventElapsedTime(&ms, p.begin, p.end) == hipSuccess) {
+                total_ms[p.id] += ms;
+                ++launches[p.id];
+            }
+            pool.push_back(p.begin);
+            pool.push_back(p.end);
+        }
+        pending.clear();
+        (void)hipGetLastError();
+    }
+    void reset()
+    {
+        for (int i = 0; i < kNumKernelIds; ++i) {
+            total_ms[i] = 0;
+            launches[i] = 0;
+        }
+    }
+    ~EventRecorder() override
+    {
+        resolve(false);
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        for (hipEvent_t e : open)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+thread_local KernelRecorder* t_recorder = nullptr;
+
+}  // namespace
+
+namespace svdag {
+KernelRecorder*& active_recorder() { return t_recorder; }
+}  // namespace svdag
+
 struct svdag_context {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -46,6 +139,7 @@ struct svdag_context {
     const char* mark_names[kMaxMarks] = {};
     int num_marks = 0;
     double wait_seconds = 120.0;
+    EventRecorder recorder;
 };
 
 namespace {
@@ -92,6 +186,7 @@ int finish(svdag_context* ctx, const char* what)
         if (e == hipSuccess) break;
         if (e != hipErrorNotReady) {
             ctx->num_marks = 0;
+            ctx->recorder.resolve(false);
             return set_error(SVDAG_ERR_HIP, "%s failed: %s", what, hipGetErrorString(e));
         }
         const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
@@ -110,6 +205,7 @@ int finish(svdag_context* ctx, const char* what)
         std::this_thread::sleep_for(std::chrono::microseconds(waited < 0.01 ? 20 : 500));
     }
     ctx->num_marks = 0;
+    ctx->recorder.resolve(true);
     return 0;
 }
 
@@ -496,6 +592,7 @@ int check_ctx(svdag_context* ctx)
 {
     if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "null context");
     HIP_TRY(hipSetDevice(ctx->device));
+    active_recorder() = ctx->recorder.mask ? &ctx->recorder : nullptr;
     return 0;
 }
 
@@ -770,6 +867,7 @@ int svdag_context_destroy(svdag_context* ctx)
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     for (auto& e : ctx->marks)
         if (e) (void)hipEventDestroy(e);
+    if (active_recorder() == &ctx->recorder) active_recorder() = nullptr;
     delete ctx;
     return 0;
 }
@@ -1059,5 +1157,36 @@ int svdag_free_all(svdag_context* ctx)
 }
 
 const char* svdag_last_error(void) { return g_error.c_str(); }
+
+int svdag_profile_enable(svdag_context* ctx, uint64_t kernel_mask)
+{
+    TRY(check_ctx(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->recorder.resolve(true);
+    ctx->recorder.mask = kernel_mask & ((1ull << kNumKernelIds) - 1);
+    active_recorder() = ctx->recorder.mask ? &ctx->recorder : nullptr;
+    return 0;
+}
+
+int svdag_profile_reset(svdag_context* ctx)
+{
+    if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "null context");
+    ctx->recorder.reset();
+    return 0;
+}
+
+int svdag_profile_read(svdag_context* ctx, int32_t kernel, double* total_ms, uint64_t* launches)
+{
+    if (!ctx || !total_ms || !launches) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    if (kernel < 0 || kernel >= kNumKernelIds) return set_error(SVDAG_ERR_ARGUMENT, "kernel %d out of range", kernel);
+    *total_ms = ctx->recorder.total_ms[kernel];
+    *launches = ctx->recorder.launches[kernel];
+    return 0;
+}
+
+const char* svdag_kernel_name(int32_t kernel)
+{
+    return kernel >= 0 && kernel < kNumKernelIds ? kKernelNames[kernel] : nullptr;
+}
 
 }  // extern "C"

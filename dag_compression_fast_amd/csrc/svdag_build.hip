@@ -665,6 +665,7 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
 void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
 {
     if (count == 0) return;
+    TimedLaunch t(kKFill, s);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, p, count,
                        value);
 }
@@ -672,6 +673,7 @@ void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
 void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
                         uint64_t per_tile, uint64_t* total_out)
 {
+    TimedLaunch t(kKScanCounts, s);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, s, counts, offsets, items, per_tile, total_out);
 }
 
@@ -680,6 +682,7 @@ void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint6
                           uint64_t n, uint32_t chunk_bits, BuildCounters* c)
 {
     if (n == 0) return;
+    TimedLaunch t(kKPrefixChunks, s);
     hipLaunchKernelGGL(k_prefix_chunks<K>, dim3(grid_for((n + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, in,
                        out, d_offsets, num_chunks, n, chunk_bits, c);
 }
@@ -692,11 +695,17 @@ void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t
                       uint64_t code_mask)
 {
     const int grid = grid_for((n_bound + kPackTile - 1) / kPackTile, 8);
-    hipLaunchKernelGGL(k_leaf_count<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_counts,
-                       &c->pre[leaf_level]);
+    {
+        TimedLaunch t(kKLeafCount, s);
+        hipLaunchKernelGGL(k_leaf_count<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_counts,
+                           &c->pre[leaf_level]);
+    }
     launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kPackTile, &c->leaf_positions);
-    hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_offsets,
-                       lp_code, lp_mask);
+    {
+        TimedLaunch t(kKLeafPack, s);
+        hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_offsets,
+                           lp_code, lp_mask);
+    }
 }
 template void launch_leaf_pack<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint64_t*, uint64_t*,
                                          uint32_t*, uint64_t*, uint64_t, int, uint64_t);
@@ -706,6 +715,7 @@ template void launch_leaf_pack<uint32_t>(hipStream_t, const uint32_t*, BuildCoun
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound)
 {
+    TimedLaunch t(kKTableClear, s);
     hipLaunchKernelGGL(k_table_clear, dim3(grid_for((slot_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        keys, count_ptr, log2_out, load_shift, overflow);
 }
@@ -714,6 +724,7 @@ void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* 
                         const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
                         uint32_t* list_slots, uint64_t* list_count, unsigned long long* err)
 {
+    TimedLaunch t(kKLeafInsert, s);
     hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, list_keys, list_slots, list_count, err);
 }
@@ -721,6 +732,7 @@ void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* 
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
                          uint32_t* rank_of_slot, uint64_t n_bound)
 {
+    TimedLaunch t(kKRankOfSlot, s);
     hipLaunchKernelGGL(k_rank_of_slot, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        sorted_slots, count_ptr, rank_of_slot);
 }
@@ -728,25 +740,38 @@ void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint
 void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound)
 {
     const int grid = grid_for((child_bound + kBuildTile - 1) / kBuildTile, 8);
-    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(kThreads), 0, s, a.child_codes, a.child_count, a.tile_counts,
-                       a.next_count);
+    {
+        TimedLaunch t(kKLevelCount, s);
+        hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(kThreads), 0, s, a.child_codes, a.child_count, a.tile_counts,
+                           a.next_count);
+    }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.child_count, kBuildTile, nullptr);
-    hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
+    {
+        TimedLaunch t(kKLevelBuild, s);
+        hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
+    }
 }
 
 void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound)
 {
     const int grid = grid_for((slot_bound + kExtractTile - 1) / kExtractTile, 8);
-    hipLaunchKernelGGL(k_extract_count, dim3(grid), dim3(kThreads), 0, s, a.table_keys, a.table_log2, a.tile_counts,
-                       a.slots_scratch);
+    {
+        TimedLaunch t(kKExtractCount, s);
+        hipLaunchKernelGGL(k_extract_count, dim3(grid), dim3(kThreads), 0, s, a.table_keys, a.table_log2, a.tile_counts,
+                           a.slots_scratch);
+    }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.slots_scratch, kExtractTile, a.q_count);
-    hipLaunchKernelGGL(k_level_extract, dim3(grid), dim3(kThreads), 0, s, a);
+    {
+        TimedLaunch t(kKLevelExtract, s);
+        hipLaunchKernelGGL(k_level_extract, dim3(grid), dim3(kThreads), 0, s, a);
+    }
 }
 
 void launch_level_check(hipStream_t s, const uint32_t* parent_slots, const uint32_t* table_rep,
                         const uint8_t* parent_masks, const uint32_t* parent_children, const uint64_t* count_ptr,
                         uint64_t* collisions, uint64_t n_bound)
 {
+    TimedLaunch t(kKLevelCheck, s);
     hipLaunchKernelGGL(k_level_check, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        parent_slots, table_rep, parent_masks, parent_children, count_ptr, collisions);
 }
@@ -756,12 +781,21 @@ void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes
     if (a.total_nodes == 0 && a.num_leaves == 0) return;
     if (a.total_nodes) {
         const int grid = grid_for((a.total_nodes + kFinalTile - 1) / kFinalTile, 8);
-        hipLaunchKernelGGL(k_final_count, dim3(grid), dim3(kThreads), 0, s, a);
+        {
+            TimedLaunch t(kKFinalCount, s);
+            hipLaunchKernelGGL(k_final_count, dim3(grid), dim3(kThreads), 0, s, a);
+        }
         launch_scan_counts(s, a.counts, a.offsets, total_nodes_dev, kFinalTile, nullptr);
-        hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
+        {
+            TimedLaunch t(kKFinalPositions, s);
+            hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
+        }
     }
     const uint64_t work = std::max(a.total_nodes, a.num_leaves);
-    hipLaunchKernelGGL(k_final_write, dim3(grid_for((work + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, a);
+    {
+        TimedLaunch t(kKFinalWrite, s);
+        hipLaunchKernelGGL(k_final_write, dim3(grid_for((work + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, a);
+    }
 }
 
 void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,
@@ -771,6 +805,7 @@ void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t n
                          uint32_t* list_slots)
 {
     if (total == 0) return;
+    TimedLaunch t(kKMergeInsert, s);
     hipLaunchKernelGGL(k_merge_insert, dim3(grid_for((total + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        d_sources, num_sources, total, child_slot_of, child_rank_of_slot, masks_out, children_out,
                        table_keys, table_rep, table_log2, slot_of, counters, ordered, list_keys, list_slots);

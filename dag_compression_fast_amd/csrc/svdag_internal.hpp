@@ -10,6 +10,60 @@
 
 namespace svdag {
 
+// ---- per-kernel timing ----------------------------------------------------
+// Counterpart of the reference's per-operation event timer
+// (src/CudaPerfRecorder.h:5-60, used at src/DAGCompression.cu:352, 384, 413...),
+// which synchronises after every operation.  Here a launch is bracketed by two
+// events on its stream when a recorder is active on the calling thread; the
+// pairs are resolved after the call's single synchronisation.
+enum KernelId : int {
+    kKHistogram = 0,
+    kKSortCount,
+    kKSortScan,
+    kKSortScatterKeys,
+    kKSortScatterPairs,
+    kKFill,
+    kKScanCounts,
+    kKPrefixChunks,
+    kKLeafCount,
+    kKLeafPack,
+    kKTableClear,
+    kKLeafInsert,
+    kKRankOfSlot,
+    kKLevelCount,
+    kKLevelBuild,
+    kKLevelCheck,
+    kKExtractCount,
+    kKLevelExtract,
+    kKFinalCount,
+    kKFinalPositions,
+    kKFinalWrite,
+    kKMergeInsert,
+    kNumKernelIds
+};
+
+struct KernelRecorder {
+    virtual void begin(hipStream_t s, int id) = 0;
+    virtual void end(hipStream_t s, int id) = 0;
+    virtual ~KernelRecorder() = default;
+};
+
+// The recorder of the calling thread (null: no timing).
+KernelRecorder*& active_recorder();
+
+struct TimedLaunch {
+    int id;
+    hipStream_t s;
+    TimedLaunch(int kernel, hipStream_t stream) : id(kernel), s(stream)
+    {
+        if (KernelRecorder* r = active_recorder()) r->begin(s, id);
+    }
+    ~TimedLaunch()
+    {
+        if (KernelRecorder* r = active_recorder()) r->end(s, id);
+    }
+};
+
 // ---- radix sort -----------------------------------------------------------
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;

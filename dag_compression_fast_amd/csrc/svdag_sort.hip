@@ -263,8 +263,11 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
     }
     const int cus = device_cu_count();
     const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 8));
-    hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist,
-                       key_or);
+    {
+        TimedLaunch t(kKHistogram, stream);
+        hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist,
+                           key_or);
+    }
 
     const bool pairs = vals_in != nullptr;
     const K* src = keys_in;
@@ -282,16 +285,25 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         }
         if (last && final_vals) vdst = final_vals;
         const uint32_t dmask = (1u << plan.bits[p]) - 1u;
-        hipLaunchKernelGGL(k_sort_count<K>, dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, plan.shift[p],
-                           dmask, counts, stride);
-        hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix, n_ptr,
-                           stride);
-        if (pairs)
+        {
+            TimedLaunch t(kKSortCount, stream);
+            hipLaunchKernelGGL(k_sort_count<K>, dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, plan.shift[p],
+                               dmask, counts, stride);
+        }
+        {
+            TimedLaunch t(kKSortScan, stream);
+            hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix,
+                               n_ptr, stride);
+        }
+        if (pairs) {
+            TimedLaunch t(kKSortScatterPairs, stream);
             hipLaunchKernelGGL((k_sort_scatter<K, true>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
                                vdst, n_ptr, plan.shift[p], dmask, counts, stride);
-        else
+        } else {
+            TimedLaunch t(kKSortScatterKeys, stream);
             hipLaunchKernelGGL((k_sort_scatter<K, false>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
                                nullptr, nullptr, n_ptr, plan.shift[p], dmask, counts, stride);
+        }
         src = dst;
         vsrc = vdst;
     }

@@ -149,6 +149,20 @@ int svdag_free_all(svdag_context* ctx);
 /* Message of the last failing call on this host thread. */
 const char* svdag_last_error(void);
 
+/* --- per-kernel timing ---------------------------------------------------- */
+
+/* Counterpart of the reference's per-operation timer FCudaScopePerfRecorder
+ * (src/CudaPerfRecorder.h:5-60, used at src/DAGCompression.cu:352, 384, 413 ...),
+ * which synchronises after each operation.  Here bit i of kernel_mask brackets
+ * every launch of kernel i (svdag_kernel_name(i)) with two HIP events on the
+ * context's stream; the pairs are resolved at the call's own synchronisation and
+ * summed per kernel.  0 switches timing off. */
+int svdag_profile_enable(svdag_context* ctx, uint64_t kernel_mask);
+int svdag_profile_reset(svdag_context* ctx);
+int svdag_profile_read(svdag_context* ctx, int32_t kernel, double* total_ms, uint64_t* launches);
+/* Name of kernel i, NULL past the last one. */
+const char* svdag_kernel_name(int32_t kernel);
+
 /* --- synthetic inputs (host) --------------------------------------------- */
 
 /* Morton code of (x, y, z) (src/DAG_Compression_Fast.cpp:223-236). */

@@ -141,6 +141,8 @@ hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 }
 enum { hipErrorNotReady = 600, hipEventDisableTiming = 2 };
 
