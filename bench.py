@@ -50,22 +50,21 @@ def make_keys(svd, spec, threads):
     return svd.gen_sphere_shells(seed, spec["depth"], count, a, b, dup, threads=threads)
 
 
-def algorithmic_bytes(kernel, st, key_bytes, key_passes):
+def algorithmic_bytes(kernel, st, key_bytes, launches):
     """Algorithmic HBM bytes one step moves through `kernel` (None if not modelled).
 
     N keys, L0 leaf positions, L unique leaves, M_l / Q_l nodes of level l before /
-    after dedup, C_l children feeding level l, W words of the flat layout."""
+    after dedup, C_l children feeding level l, W words of the flat layout;
+    `launches` = launches of the kernel per step (sort passes)."""
     n, l0, lu = st["n"], st["leaf_positions"], st["num_leaves"]
     pre, uniq, words = st["pre"], st["uniq"], st["words"]
     levels = len(pre)
     child = [pre[l + 1] if l + 1 < levels else l0 for l in range(levels)]
-    leaf_passes = 8
     table = {
-        # Every key pass reads and writes the keys; the unique-leaf sort moves (u64 mask, u32 slot) pairs.
-        "sort_scatter_keys": key_passes * 2 * key_bytes * n,
-        "sort_scatter_pairs": leaf_passes * 2 * 12 * lu,
-        "sort_count": key_passes * key_bytes * n + leaf_passes * 8 * lu,
-        "sort_histogram": key_bytes * n + 8 * lu,
+        # Every key-sort pass reads and writes each key once.
+        "sort_scatter_keys": launches * 2 * key_bytes * n,
+        # The unique-leaf sort moves (u64 mask, u32 slot) pairs.
+        "sort_scatter_pairs": launches * 2 * 12 * lu,
         "leaf_count": key_bytes * n,
         "leaf_pack": key_bytes * n + 16 * l0,
         # mask read, slot write, unique list append
@@ -113,6 +112,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,6 +133,8 @@ def main():
     log(f"{n} keys, depth {depth}")
 
     ctx = svd.Context()
+    if args.radix_bits is not None:
+        ctx.set_option(_lib.OPT_RADIX_BITS, args.radix_bits)
     L = _lib.lib()
     d_keys = svd.DeviceBuffer.from_numpy(keys)
     H = _lib.hip()
@@ -188,9 +190,8 @@ def main():
     kms, launches = ctx.profile_read()[dominant]
     ctx.profile_enable([])
 
-    key_passes = -(-3 * depth // 8)
-    alg = algorithmic_bytes(dominant, stats, 8, key_passes)
     launches_per_step = launches / args.steps
+    alg = algorithmic_bytes(dominant, stats, 8, launches_per_step)
     roofline = None
     if alg is not None:
         per_launch = alg / launches_per_step

@@ -132,6 +132,7 @@ struct svdag_context {
     int check_collisions = 0;
     uint64_t budget = 0;
     uint32_t load_shift = 1;
+    int radix_bits = 8;
     int host_threads = 1;
     // Stage markers of the work in flight (bounded completion wait).
     static constexpr int kMaxMarks = 64;
@@ -254,7 +255,7 @@ struct BuildLayout {
     SortPlan key_plan{}, leaf_plan{};
 };
 
-BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched)
+BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched, int radix_bits)
 {
     BuildLayout L;
     L.n = n;
@@ -262,8 +263,10 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.level_bound = std::min<uint64_t>(L.leaf_bound, pow8_capped(depth - 3, n));
     L.leaf_slots = 1ull << table_lg(L.leaf_bound, load_shift);
     L.level_slots = (1ull << table_lg(L.level_bound, load_shift)) + kTableOverflow;
-    L.key_plan = make_sort_plan(3 * depth);
-    L.leaf_plan = make_sort_plan(64);
+    // Keys only need grouping by code >> 6 (the leaf pack ORs the low 6 bits), so
+    // the sort skips bits [0, 6).
+    L.key_plan = make_sort_plan(6, 3 * depth, radix_bits);
+    L.leaf_plan = make_sort_plan(0, 64);
     const uint64_t stage_tiles = std::max({tiles(n, kPackTileItems), tiles(L.leaf_bound, kBuildTileItems),
                                            tiles(L.level_slots, kExtractTileItems), uint64_t(1)});
     Carver c;
@@ -416,7 +419,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     out->num_keys = n;
     if (n == 0) return 0;  // DAGCompression.cu:1010-1013
 
-    const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr);
+    const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits);
     const uint64_t offsets_bytes = batch ? align_up(8 * (batch->num_chunks + 1)) : 0;
     TRY(ensure_arena(ctx, L.total + offsets_bytes));
     char* A = ctx->arena;
@@ -635,7 +638,7 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     const uint32_t shift = ctx->load_shift;
     const uint64_t leaf_slots = 1ull << table_lg(total_leaves, shift);
     const uint64_t level_slots = (1ull << table_lg(std::max(max_total, nlive), shift)) + kTableOverflow;
-    const SortPlan leaf_plan = make_sort_plan(64);
+    const SortPlan leaf_plan = make_sort_plan(0, 64);
 
     const uint64_t stage_tiles =
         std::max({tiles(level_slots, kExtractTileItems), tiles(nlive, kBuildTileItems), uint64_t(1)});
@@ -884,6 +887,10 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         return 0;
     case SVDAG_OPT_HOST_THREADS: ctx->host_threads = int(std::max<int64_t>(1, value)); return 0;
     case SVDAG_OPT_WAIT_SECONDS: ctx->wait_seconds = double(std::max<int64_t>(1, value)); return 0;
+    case SVDAG_OPT_RADIX_BITS:
+        if (value < 4 || value > kMaxRadixBits) return set_error(SVDAG_ERR_ARGUMENT, "radix bits must be 4..%d", kMaxRadixBits);
+        ctx->radix_bits = int(value);
+        return 0;
     default: return set_error(SVDAG_ERR_ARGUMENT, "unknown option %d", option);
     }
 }
@@ -891,7 +898,7 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
 uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes)
 {
     if (depth < 3 || depth > 21 || n == 0) return 0;
-    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false).total;
+    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false, 4).total;  // narrowest digits: most passes
 }
 
 int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)

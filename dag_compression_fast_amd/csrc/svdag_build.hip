@@ -90,33 +90,54 @@ __global__ void k_prefix_chunks(const K* __restrict__ in, uint64_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Leaf pack: sorted keys -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block.
-// A run belongs to the thread holding its first key and may extend past the
-// thread's (and the tile's) range; duplicates OR harmlessly.
+// Leaf pack: sorted keys -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block,
+// i.e. per run of keys with equal code >> 6 (duplicates OR harmlessly).
+// Wave w of a tile owns kPackWaveKeys consecutive keys and reads them as
+// kPackGroups coalesced groups of 64.  A run belongs to the wave holding its
+// first key; it is OR-reduced by a segmented wave scan, carried across groups,
+// and finished past the wave's keys by reading on until the code changes.
 // ---------------------------------------------------------------------------
-constexpr int kPackIpt = 16;
-constexpr int kPackTile = kThreads * kPackIpt;
+constexpr int kPackGroups = 16;
+constexpr int kPackWaveKeys = kPackGroups * kWave;
+constexpr int kPackTile = (kThreads / kWave) * kPackWaveKeys;
 
-// Loads the thread's keys and marks run starts (code >> 6 changes) and
-// grand-parent starts (code >> 9 changes).
 template <typename K>
-__device__ __forceinline__ void leaf_runs(const K* __restrict__ sorted, uint64_t n, uint64_t first, uint64_t code_mask,
-                                          uint64_t (&k)[kPackIpt], uint32_t& starts, uint32_t& gcount)
+__device__ __forceinline__ uint64_t load_key(const K* __restrict__ p, uint64_t i, uint64_t n, uint64_t code_mask)
+{
+    return i < n ? (uint64_t(p[i]) & code_mask) : 0;
+}
+
+// Keys of the wave's groups and their run-head ballots; gheads counts the keys
+// that start a new code >> 9 (a new parent of the leaf level).
+template <typename K>
+__device__ __forceinline__ void wave_heads(const K* __restrict__ sorted, uint64_t n, uint64_t chunk,
+                                           uint64_t code_mask, uint64_t (&k)[kPackGroups],
+                                           uint64_t (&heads)[kPackGroups], uint32_t& gheads)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int g = 0; g < kPackGroups; ++g) k[g] = load_key(sorted, chunk + uint64_t(g) * kWave + lane, n, code_mask);
+    uint64_t carry = (chunk > 0 && chunk <= n) ? (uint64_t(sorted[chunk - 1]) & code_mask) : ~0ull;
+    gheads = 0;
+#pragma unroll
+    for (int g = 0; g < kPackGroups; ++g) {
+        uint64_t prev = __shfl_up(k[g], 1, kWave);
+        if (lane == 0) prev = carry;
+        const uint64_t i = chunk + uint64_t(g) * kWave + lane;
+        const bool valid = i < n;
+        const bool head = valid && (i == 0 || (k[g] >> 6) != (prev >> 6));
+        const bool ghead = valid && (i == 0 || (k[g] >> 9) != (prev >> 9));
+        heads[g] = __ballot(head);
+        gheads += uint32_t(__popcll(__ballot(ghead)));
+        carry = __shfl(k[g], kWave - 1, kWave);
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t v)
 {
 #pragma unroll
-    for (int j = 0; j < kPackIpt; ++j) k[j] = first + j < n ? (uint64_t(sorted[first + j]) & code_mask) : 0;
-    const uint64_t before = (first > 0 && first <= n) ? (uint64_t(sorted[first - 1]) & code_mask) : ~0ull;
-    starts = 0;
-    gcount = 0;
-#pragma unroll
-    for (int j = 0; j < kPackIpt; ++j) {
-        const uint64_t prev = j ? k[j - 1] : before;
-        const bool head = first + j == 0;
-        if (first + j < n && (head || (k[j] >> 6) != (prev >> 6))) {
-            starts |= 1u << j;
-            gcount += (head || (k[j] >> 9) != (prev >> 9)) ? 1u : 0u;
-        }
-    }
+    for (int d = kWave / 2; d >= 1; d >>= 1) v |= __shfl_xor(v, d, kWave);
+    return v;
 }
 
 template <typename K>
@@ -127,14 +148,19 @@ __global__ __launch_bounds__(kThreads) void k_leaf_count(const K* __restrict__ s
     __shared__ uint64_t s_red[kThreads / kWave];
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x / kWave;
     uint64_t gtot = 0;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t k[kPackIpt];
-        uint32_t starts, gcount;
-        leaf_runs(sorted, n, tile * kPackTile + uint64_t(threadIdx.x) * kPackIpt, code_mask, k, starts, gcount);
-        const uint64_t cnt = block_reduce_sum<kThreads>(__popc(starts), s_red);
-        gtot += gcount;
-        if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
+        uint64_t k[kPackGroups], heads[kPackGroups];
+        uint32_t gheads;
+        wave_heads(sorted, n, tile * kPackTile + uint64_t(w) * kPackWaveKeys, code_mask, k, heads, gheads);
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int g = 0; g < kPackGroups; ++g) cnt += uint32_t(__popcll(heads[g]));
+        const uint64_t tot = block_reduce_sum<kThreads>(lane == 0 ? cnt : 0u, s_red);
+        if (threadIdx.x == 0) counts[tile] = uint32_t(tot);
+        gtot += lane == 0 ? gheads : 0u;
     }
     gtot = block_reduce_sum<kThreads>(gtot, s_red);
     if (threadIdx.x == 0 && gtot) atomicAdd((unsigned long long*)next_count, (unsigned long long)gtot);
@@ -145,41 +171,78 @@ __global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ so
                                                         uint64_t code_mask, const uint64_t* __restrict__ offsets,
                                                         uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask)
 {
-    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint32_t s_wcount[kThreads / kWave];
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x / kWave;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t first = tile * kPackTile + uint64_t(threadIdx.x) * kPackIpt;
-        uint64_t k[kPackIpt];
-        uint32_t starts, gcount;
-        leaf_runs(sorted, n, first, code_mask, k, starts, gcount);
-        uint32_t unused;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, unused);
-        uint64_t q = offsets[tile] + local;
-        bool open = false;
-        uint64_t run_code = 0, run_mask = 0;
+        const uint64_t chunk = tile * kPackTile + uint64_t(w) * kPackWaveKeys;
+        uint64_t k[kPackGroups], heads[kPackGroups];
+        uint32_t gheads;
+        wave_heads(sorted, n, chunk, code_mask, k, heads, gheads);
+        uint32_t cnt = 0;
 #pragma unroll
-        for (int j = 0; j < kPackIpt; ++j) {
-            if ((starts >> j) & 1u) {
-                if (open) {
-                    lp_code[q] = run_code;
-                    lp_mask[q] = run_mask;
-                    ++q;
+        for (int g = 0; g < kPackGroups; ++g) cnt += uint32_t(__popcll(heads[g]));
+        if (lane == 0) s_wcount[w] = cnt;
+        __syncthreads();
+        uint64_t q = offsets[tile];
+        for (uint32_t i = 0; i < w; ++i) q += s_wcount[i];
+        __syncthreads();  // s_wcount is rewritten by the next tile
+
+        // Wave-uniform state of the run still open at the end of the last group.
+        bool open = false;
+        uint64_t open_code = 0, open_or = 0, open_q = 0;
+#pragma unroll
+        for (int g = 0; g < kPackGroups; ++g) {
+            const uint64_t i = chunk + uint64_t(g) * kWave + lane;
+            const uint64_t H = heads[g];
+            uint64_t v = i < n ? (1ull << (k[g] & 63)) : 0ull;
+            // Segmented OR over [lane, next head after lane).
+            const uint64_t after = lane == kWave - 1 ? 0ull : (H & (~0ull << (lane + 1)));
+            const uint32_t nh = after ? uint32_t(__ffsll((long long)after) - 1) : uint32_t(kWave);
+#pragma unroll
+            for (uint32_t off = 1; off < uint32_t(kWave); off <<= 1) {
+                const uint64_t o = __shfl_down(v, off, kWave);
+                if (lane + off < nh) v |= o;
+            }
+            // Lanes before the group's first head continue the open run (or a
+            // run owned by the previous wave, which is then ignored).
+            const uint64_t prefix = __shfl(v, 0, kWave);
+            if (!(H & 1ull)) open_or |= prefix;
+            if (H) {
+                if (open && lane == 0) {
+                    lp_code[open_q] = open_code;
+                    lp_mask[open_q] = open_or;
+                }
+                const uint32_t nheads = uint32_t(__popcll(H));
+                const uint32_t last = 63u - uint32_t(__clzll((long long)H));
+                if (((H >> lane) & 1ull) && lane != last) {
+                    const uint64_t r = q + uint32_t(__popcll(H & lanemask_lt()));
+                    lp_code[r] = k[g] >> 6;
+                    lp_mask[r] = v;
                 }
                 open = true;
-                run_code = k[j] >> 6;
-                run_mask = 0;
+                open_q = q + nheads - 1;
+                open_code = __shfl(k[g], last, kWave) >> 6;
+                open_or = __shfl(v, last, kWave);
+                q += nheads;
             }
-            if (open && first + j < n) run_mask |= 1ull << (k[j] & 63);
         }
         if (open) {
-            for (uint64_t i = first + kPackIpt; i < n; ++i) {
-                const uint64_t kk = uint64_t(sorted[i]) & code_mask;
-                if ((kk >> 6) != run_code) break;
-                run_mask |= 1ull << (kk & 63);
+            // The wave's last run may continue past its keys.
+            for (uint64_t i0 = chunk + kPackWaveKeys; i0 < n; i0 += kWave) {
+                const uint64_t i = i0 + lane;
+                const uint64_t kk = load_key(sorted, i, n, code_mask);
+                const uint64_t out = __ballot(!(i < n && (kk >> 6) == open_code));
+                const uint32_t stop = out ? uint32_t(__ffsll((long long)out) - 1) : uint32_t(kWave);
+                open_or |= wave_or(lane < stop ? (1ull << (kk & 63)) : 0ull);
+                if (stop < uint32_t(kWave)) break;
             }
-            lp_code[q] = run_code;
-            lp_mask[q] = run_mask;
+            if (lane == 0) {
+                lp_code[open_q] = open_code;
+                lp_mask[open_q] = open_or;
+            }
         }
     }
 }
@@ -197,13 +260,13 @@ __global__ void k_table_clear(uint64_t* __restrict__ keys, const uint64_t* __res
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) keys[i] = 0;
 }
 
-// Inserts key h (never 0) into a wrap-around table of 2^lg slots; the first
-// inserter of a key appends (key, slot) to the unique list.  A probe never
-// exceeds the table size; a full table reports kErrTableOverflow.
-__device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uint32_t lg, uint64_t h,
-                                                  uint64_t* __restrict__ list_keys, uint32_t* __restrict__ list_slots,
-                                                  uint64_t* __restrict__ list_count, unsigned long long* err)
+// Inserts key h (never 0) into a wrap-around table of 2^lg slots; `won` tells
+// whether this call placed the key.  A probe never exceeds the table size; a
+// full table reports kErrTableOverflow.
+__device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uint32_t lg, uint64_t h, bool& won,
+                                                  unsigned long long* err)
 {
+    won = false;
     const uint64_t wrap = (1ull << lg) - 1;
     uint64_t slot = fmix64(h) >> (64 - lg);
     for (uint64_t probe = 0; probe <= wrap; ++probe, slot = (slot + 1) & wrap) {
@@ -212,9 +275,7 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
         if (cur == 0) {
             const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
             if (prev == 0) {
-                const uint64_t idx = atomicAdd((unsigned long long*)list_count, 1ull);
-                list_keys[idx] = h;
-                list_slots[idx] = uint32_t(slot);
+                won = true;
                 return uint32_t(slot);
             }
             if (prev == h) return uint32_t(slot);
@@ -224,17 +285,59 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
     return 0;
 }
 
-__global__ void k_leaf_insert(const uint64_t* __restrict__ lp_mask, const uint64_t* __restrict__ count_ptr,
-                              uint64_t* __restrict__ tk, const uint32_t* __restrict__ log2_ptr,
-                              uint32_t* __restrict__ lp_slot, uint64_t* __restrict__ list_keys,
-                              uint32_t* __restrict__ list_slots, uint64_t* __restrict__ list_count,
-                              unsigned long long* __restrict__ err)
+// Every leaf position's mask enters the table; the masks placed first are
+// appended to the unique list, one list reservation per workgroup round.
+constexpr int kInsertIpt = 4;
+
+__global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __restrict__ lp_mask,
+                                                          const uint64_t* __restrict__ count_ptr,
+                                                          uint64_t* __restrict__ tk,
+                                                          const uint32_t* __restrict__ log2_ptr,
+                                                          uint32_t* __restrict__ lp_slot,
+                                                          uint64_t* __restrict__ list_keys,
+                                                          uint32_t* __restrict__ list_slots,
+                                                          uint64_t* __restrict__ list_count,
+                                                          unsigned long long* __restrict__ err)
 {
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_base;
     const uint64_t n = *count_ptr;
     const uint32_t lg = *log2_ptr;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
-        lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], list_keys, list_slots, list_count, err);
+    constexpr uint64_t per_round = uint64_t(kThreads) * kInsertIpt;
+    const uint64_t rounds = (n + per_round - 1) / per_round;
+    for (uint64_t r = blockIdx.x; r < rounds; r += gridDim.x) {
+        uint64_t key[kInsertIpt];
+        uint32_t slot[kInsertIpt];
+        uint32_t won = 0;
+#pragma unroll
+        for (int j = 0; j < kInsertIpt; ++j) {
+            const uint64_t q = r * per_round + uint64_t(j) * kThreads + threadIdx.x;
+            key[j] = 0;
+            slot[j] = 0;
+            if (q < n) {
+                key[j] = lp_mask[q];
+                bool w;
+                slot[j] = hashed_insert(tk, lg, key[j], w, err);
+                lp_slot[q] = slot[j];
+                won |= (w ? 1u : 0u) << j;
+            }
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kThreads>(uint32_t(__popc(won)), s_scan, total);
+        if (threadIdx.x == 0 && total) s_base = atomicAdd((unsigned long long*)list_count, (unsigned long long)total);
+        __syncthreads();
+        if (won) {
+            uint64_t idx = s_base + ex;
+#pragma unroll
+            for (int j = 0; j < kInsertIpt; ++j)
+                if ((won >> j) & 1u) {
+                    list_keys[idx] = key[j];
+                    list_slots[idx] = slot[j];
+                    ++idx;
+                }
+        }
+        __syncthreads();  // s_base is rewritten by the next round
+    }
 }
 
 __global__ void k_rank_of_slot(const uint32_t* __restrict__ sorted_slots, const uint64_t* __restrict__ count_ptr,
@@ -628,21 +731,45 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
     const uint32_t lg = *log2_ptr;
     const uint64_t limit = (1ull << lg) + kTableOverflow;
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t gi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < total; gi += stride) {
+    // Wave-uniform trip count: the leaf mode's list reservation is a wave ballot.
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < total; base += stride) {
+        const uint64_t gi = base + threadIdx.x;
+        const bool active = gi < total;
         uint32_t lo = 0, hi = nsrc;  // src[lo].first <= gi < src[hi].first
-        while (hi - lo > 1) {
+        while (active && hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
             if (src[mid].first <= gi) lo = mid;
             else hi = mid;
         }
         const MergeSource& s = src[lo];
         const uint64_t i = gi - s.first;
-        const uint64_t key = s.hashes[i];
         if (!ordered) {
-            slot_of[gi] = hashed_insert(tk, lg, key, list_keys, list_slots, &c->unique_leaves,
-                                        (unsigned long long*)&c->error);
+            bool won = false;
+            uint32_t slot = 0;
+            uint64_t key = 0;
+            if (active) {
+                key = s.hashes[i];
+                slot = hashed_insert(tk, lg, key, won, (unsigned long long*)&c->error);
+                slot_of[gi] = slot;
+            }
+            // One list reservation per wave for the keys this wave placed.
+            const uint64_t wins = __ballot(won);
+            if (wins) {
+                const uint32_t leader = uint32_t(__ffsll((long long)wins) - 1);
+                uint64_t start = 0;
+                if (lane_id() == leader)
+                    start = atomicAdd((unsigned long long*)&c->unique_leaves, (unsigned long long)__popcll(wins));
+                start = __shfl(start, leader, kWave);
+                if (won) {
+                    const uint64_t idx = start + uint32_t(__popcll(wins & lanemask_lt()));
+                    list_keys[idx] = key;
+                    list_slots[idx] = slot;
+                }
+            }
             continue;
         }
+        if (!active) continue;
+        const uint64_t key = s.hashes[i];
         const uint4* cs = reinterpret_cast<const uint4*>(s.children + 8 * i);
         const uint4 a = cs[0], b = cs[1];
         uint32_t ch[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -725,7 +852,8 @@ void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* 
                         uint32_t* list_slots, uint64_t* list_count, unsigned long long* err)
 {
     TimedLaunch t(kKLeafInsert, s);
-    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads * kInsertIpt - 1) / (kThreads * kInsertIpt), 8)),
+                       dim3(kThreads), 0, s,
                        lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, list_keys, list_slots, list_count, err);
 }
 

@@ -65,11 +65,13 @@ struct TimedLaunch {
 };
 
 // ---- radix sort -----------------------------------------------------------
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
+// Digits are at most kMaxRadixBits wide; kRadix digit slots are reserved in
+// every LDS and count array.
+constexpr int kMaxRadixBits = 9;
+constexpr int kRadix = 1 << kMaxRadixBits;
 constexpr int kMaxPasses = 8;
 // 256 threads x 16 keys: a tile's LDS (keys, values, per-wave digit counters)
-// stays under 64 KB, the per-workgroup limit of every CDNA generation.
+// stays under 64 KB.
 constexpr int kSortThreads = 256;
 constexpr int kSortIpt = 16;
 constexpr int kSortTile = kSortThreads * kSortIpt;  // keys per tile
@@ -80,18 +82,19 @@ struct SortPlan {
     uint32_t bits[kMaxPasses];
 };
 
-// Digits of width <= 8 covering bits [0, key_bits), spread evenly.
-SortPlan make_sort_plan(int key_bits);
+// Digits of width <= max_bits covering bits [lo, hi), spread evenly.
+SortPlan make_sort_plan(int lo, int hi, int max_bits = kMaxRadixBits);
 
 inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
-// Device scratch of one sort: the all-pass digit histogram (passes * 256 u32,
-// zero before launch_sort) and the per-tile digit counts (sort_count_words u32,
-// any contents).
+// Device scratch of one sort: the all-pass digit histogram (passes * kRadix
+// u32, zero before launch_sort) and the per-tile digit counts
+// (sort_count_words u32, any contents).
 inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) * kRadix; }
 inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound); }
 
-// LSD radix sort of keys (and optionally u32 values) on the plan's bits.
+// LSD radix sort of keys (and optionally u32 values) on the plan's bits; keys
+// equal on those bits keep their input order.
 // n is read from device memory (*n_ptr, < 2^30); n_bound is a host upper bound
 // sizing the grid and the count buffer.  Passes ping-pong between (keys_a,
 // keys_b); the first pass reads keys_in.  Returns which buffer holds the result

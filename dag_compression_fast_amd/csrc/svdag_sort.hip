@@ -1,9 +1,9 @@
 // svdag_sort.hip -- LSD radix sort for gfx950 (wave64), reduce-then-scan.
 //
 // Replaces the CUB DeviceRadixSort calls of the reference (SortKeys,
-// src/DAGCompression.cu:336-357; SortPairs, :359-394).  No workgroup ever waits
-// on another one: every pass is
-//   k_sort_count    tile t (a fixed range of 4096 keys) -> 256 digit counts
+// src/DAGCompression.cu:336-357; SortPairs, :359-394).  Digits are up to 9
+// bits wide.  No workgroup ever waits on another one: every pass is
+//   k_sort_count    tile t (a fixed range of 4096 keys) -> per-digit counts
 //   k_sort_scan     one workgroup per digit scans that digit's row of tile
 //                   counts and adds the digit's global start (from the
 //                   all-pass histogram, read once up front)
@@ -19,13 +19,15 @@
 
 namespace svdag {
 
-SortPlan make_sort_plan(int key_bits)
+SortPlan make_sort_plan(int lo, int hi, int max_bits)
 {
     SortPlan p{};
-    if (key_bits <= 0) return p;
-    p.passes = (key_bits + kRadixBits - 1) / kRadixBits;
+    const int key_bits = hi - lo;
+    if (key_bits <= 0 || max_bits < 1 || max_bits > kMaxRadixBits) return p;
+    p.passes = (key_bits + max_bits - 1) / max_bits;
+    if (p.passes > kMaxPasses) return SortPlan{};
     const int base = key_bits / p.passes, extra = key_bits % p.passes;
-    uint32_t shift = 0;
+    uint32_t shift = uint32_t(lo);
     for (int i = 0; i < p.passes; ++i) {
         p.bits[i] = uint32_t(base + (i < extra ? 1 : 0));
         p.shift[i] = shift;
@@ -35,6 +37,27 @@ SortPlan make_sort_plan(int key_bits)
 }
 
 namespace {
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Tile order per workgroup: XCD x owns the contiguous tile range
+// [x * per, (x + 1) * per), and its workgroups stride through that range, so
+// tiles processed at the same time on one XCD are neighbours and their writes
+// into a digit bucket land in the same L2 lines.
+constexpr uint32_t kXcds = 8;
+
+struct TileWalk {
+    uint64_t tile, end, step;
+};
+
+__device__ __forceinline__ TileWalk xcd_tiles(uint64_t ntiles)
+{
+    const uint32_t groups = gridDim.x / kXcds;  // workgroups per XCD (grid is a multiple of kXcds)
+    const uint32_t x = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const uint64_t per = (ntiles + kXcds - 1) / kXcds;
+    const uint64_t start = uint64_t(x) * per;
+    const uint64_t end = min(ntiles, start + per);
+    return TileWalk{start + local, end, groups};
+}
 
 struct HistPlan {
     int passes;
@@ -62,7 +85,7 @@ __global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, c
     }
     __syncthreads();
     for (int p = 0; p < plan.passes; ++p)
-        for (int d = threadIdx.x; d < kRadix; d += 256) {
+        for (uint32_t d = threadIdx.x; d <= plan.mask[p]; d += 256) {
             const uint32_t v = s_hist[p][d];
             if (v) atomicAdd(&hist[p * kRadix + d], v);
         }
@@ -84,8 +107,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
     __shared__ uint32_t s_cnt[kRadix];
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t radix = digit_mask + 1;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        for (int d = threadIdx.x; d < kRadix; d += kSortThreads) s_cnt[d] = 0;
+        for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) s_cnt[d] = 0;
         __syncthreads();
         const uint64_t base = tile * kSortTile;
 #pragma unroll 4
@@ -94,7 +118,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
             if (idx < n) atomicAdd(&s_cnt[uint32_t(keys[idx] >> shift) & digit_mask], 1u);
         }
         __syncthreads();
-        for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
+        for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
         __syncthreads();
     }
 }
@@ -136,6 +160,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                const uint32_t* __restrict__ offsets, uint64_t stride)
 {
     constexpr int W = kSortThreads / kWave;
+    constexpr int DPT = kRadix / kSortThreads;  // digit slots per thread in the tile scan
     __shared__ K s_keys[kSortTile];
     __shared__ uint32_t s_vals[PAIRS ? kSortTile : 1];
     __shared__ uint32_t s_wcnt[W][kRadix];
@@ -148,38 +173,52 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t lane = lane_id();
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t radix = digit_mask + 1;
+    const uint32_t bits = uint32_t(__popc(digit_mask));
 
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        for (int i = tid; i < W * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+    const uint32_t wave_base = w * kWave * kSortIpt;
+    TileWalk walk = xcd_tiles(ntiles);
+    K key[kSortIpt];
+    uint32_t val[PAIRS ? kSortIpt : 1];
+    // Keys of the tile at walk.tile (loaded one tile ahead).
+    auto load_tile = [&](uint64_t t) {
+        const uint64_t base = t * kSortTile;
+        const uint32_t cnt = t < walk.end ? uint32_t(min(uint64_t(kSortTile), n - base)) : 0u;
+#pragma unroll
+        for (int i = 0; i < kSortIpt; ++i) {
+            const uint32_t idx = wave_base + i * kWave + lane;
+            const bool valid = idx < cnt;
+            key[i] = valid ? keys_in[base + idx] : K(0);
+            if constexpr (PAIRS) val[i] = valid ? vals_in[base + idx] : 0u;
+        }
+    };
+    load_tile(walk.tile);
+
+    for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
+        for (uint32_t i = tid; i < W * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
         const uint64_t tile_base = tile * kSortTile;
         const uint32_t tile_n = uint32_t(min(uint64_t(kSortTile), n - tile_base));
 
-        K key[kSortIpt];
-        uint32_t val[PAIRS ? kSortIpt : 1];
         uint32_t dig[kSortIpt];
         uint32_t rnk[kSortIpt];
-        const uint32_t wave_base = w * kWave * kSortIpt;
 #pragma unroll
-        for (int i = 0; i < kSortIpt; ++i) {
-            const uint32_t idx = wave_base + i * kWave + lane;
-            const bool valid = idx < tile_n;
-            key[i] = valid ? keys_in[tile_base + idx] : K(0);
-            if constexpr (PAIRS) val[i] = valid ? vals_in[tile_base + idx] : 0u;
-            dig[i] = uint32_t(key[i] >> shift) & digit_mask;
-        }
+        for (int i = 0; i < kSortIpt; ++i) dig[i] = uint32_t(key[i] >> shift) & digit_mask;
         uint32_t* wc = s_wcnt[w];
 #pragma unroll
         for (int i = 0; i < kSortIpt; ++i) {
             const uint32_t idx = wave_base + i * kWave + lane;
             const bool valid = idx < tile_n;
             const uint32_t d = dig[i];
+            // Lanes holding the same digit: AND of one ballot per digit bit.
             uint64_t match = __ballot(valid);
 #pragma unroll
-            for (int b = 0; b < kRadixBits; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t vote = __ballot(bit);
-                match &= bit ? vote : ~vote;
+            for (uint32_t b = 0; b < uint32_t(kMaxRadixBits); ++b) {
+                if (b < bits) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t vote = __ballot(bit);
+                    match &= bit ? vote : ~vote;
+                }
             }
             const uint32_t below = __popcll(match & lanemask_lt());
             uint32_t c = 0;
@@ -193,20 +232,35 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         }
         __syncthreads();
 
-        uint32_t total = 0;
-        if (tid < kRadix) {
+        // Per digit: wave counts -> exclusive per-wave starts; tile-local digit
+        // starts by a block scan over DPT consecutive digits per thread.
+        uint32_t dtot[DPT];
+        uint32_t tsum = 0;
 #pragma unroll
-            for (int i = 0; i < W; ++i) {
-                const uint32_t c = s_wcnt[i][tid];
-                s_wcnt[i][tid] = total;
-                total += c;
+        for (int k = 0; k < DPT; ++k) {
+            const uint32_t d = tid * DPT + k;
+            uint32_t total = 0;
+            if (d < radix) {
+#pragma unroll
+                for (int i = 0; i < W; ++i) {
+                    const uint32_t c = s_wcnt[i][d];
+                    s_wcnt[i][d] = total;
+                    total += c;
+                }
             }
+            dtot[k] = total;
+            tsum += total;
         }
         uint32_t unused;
-        const uint32_t lstart = block_exclusive_scan<kSortThreads>(tid < kRadix ? total : 0u, s_scan, unused);
-        if (tid < kRadix) {
-            s_lstart[tid] = lstart;
-            s_base[tid] = offsets[uint64_t(tid) * stride + tile] - lstart;
+        uint32_t lstart = block_exclusive_scan<kSortThreads>(tsum, s_scan, unused);
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) {
+            const uint32_t d = tid * DPT + k;
+            if (d < radix) {
+                s_lstart[d] = lstart;
+                s_base[d] = offsets[uint64_t(d) * stride + tile] - lstart;
+            }
+            lstart += dtot[k];
         }
         __syncthreads();
 
@@ -221,6 +275,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
             }
         }
         __syncthreads();
+        // Next tile's loads go out before this tile's stores.
+        load_tile(tile + walk.step);
         for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
             const K k = s_keys[j];
             const uint32_t d = uint32_t(k >> shift) & digit_mask;
@@ -273,7 +329,9 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
     const K* src = keys_in;
     const uint32_t* vsrc = vals_in;
     int where = -1;
-    const uint32_t grid = uint32_t(std::min<uint64_t>(stride, uint64_t(cus) * 8));
+    // A multiple of the XCD count, as the scatter's tile walk assumes.
+    const uint32_t grid = uint32_t(
+        std::max<uint64_t>(kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
     for (int p = 0; p < plan.passes; ++p) {
         const bool last = p + 1 == plan.passes;
         K* dst = (p % 2 == 0) ? keys_a : keys_b;
@@ -292,7 +350,7 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         }
         {
             TimedLaunch t(kKSortScan, stream);
-            hipLaunchKernelGGL(k_sort_scan, dim3(kRadix), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix,
+            hipLaunchKernelGGL(k_sort_scan, dim3(dmask + 1), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix,
                                n_ptr, stride);
         }
         if (pairs) {

@@ -80,8 +80,9 @@ typedef enum svdag_option {
     SVDAG_OPT_MEMORY_BUDGET = 2,      /* bytes of scratch the context may hold, 0 = no limit */
     SVDAG_OPT_TABLE_LOAD_SHIFT = 3,   /* dedup table slots >= 2^shift * nodes (1..3)  */
     SVDAG_OPT_HOST_THREADS = 4,       /* host threads for host-side work (merge)      */
-    SVDAG_OPT_WAIT_SECONDS = 5        /* completion deadline of a call (default 120 s); past it
+    SVDAG_OPT_WAIT_SECONDS = 5,       /* completion deadline of a call (default 120 s); past it
                                          the call fails and names the stage that stalled */
+    SVDAG_OPT_RADIX_BITS = 6          /* widest digit of the key sort, 4..9 (default 8)      */
 } svdag_option;
 
 /* --- context ------------------------------------------------------------ */

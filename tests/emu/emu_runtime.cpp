@@ -2,20 +2,51 @@
 // <hip/hip_runtime.h> and host-side stand-ins for the HIP runtime calls.
 #include "hip/hip_runtime.h"
 
-#include <ucontext.h>
+#include <signal.h>
 
 #include <atomic>
+#include <cstdint>
+#include <cstring>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <thread>
+#include <unistd.h>
 #include <vector>
+
+// Stack switch between fibers (x86-64 System V): saves the callee-saved
+// registers on the current stack, stores its stack pointer in *from and resumes
+// the stack `to`.  A user-space switch: no signal-mask system call per switch.
+extern "C" void emu_switch(void** from, void* to);
+asm(R"(
+    .text
+    .globl emu_switch
+    .type emu_switch, @function
+emu_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size emu_switch, .-emu_switch
+)");
 
 namespace emu {
 namespace {
 
 struct Fiber {
-    ucontext_t ctx;
+    void* sp = nullptr;
     dim3 tid;
     bool done = false;
 };
@@ -28,9 +59,36 @@ struct Wave {
 constexpr size_t kStack = 512 * 1024;
 
 // Per OS thread (= per running block):
-thread_local ucontext_t g_main;
+thread_local void* g_main_sp = nullptr;
 thread_local std::vector<Fiber> g_fibers;
-thread_local std::vector<char*> g_stacks;
+// Fiber stacks: each launch starts fresh worker threads; a thread takes stacks
+// from the process-wide free list and returns them when it ends.
+std::mutex g_free_mutex;
+std::vector<char*> g_free_stacks;
+
+struct StackPool {
+    std::vector<char*> stacks;
+    ~StackPool()
+    {
+        std::lock_guard<std::mutex> lock(g_free_mutex);
+        g_free_stacks.insert(g_free_stacks.end(), stacks.begin(), stacks.end());
+    }
+};
+
+char* take_stack()
+{
+    {
+        std::lock_guard<std::mutex> lock(g_free_mutex);
+        if (!g_free_stacks.empty()) {
+            char* p = g_free_stacks.back();
+            g_free_stacks.pop_back();
+            return p;
+        }
+    }
+    return static_cast<char*>(std::malloc(kStack));
+}
+thread_local StackPool g_stack_pool;
+thread_local std::vector<char*>& g_stacks = g_stack_pool.stacks;
 thread_local std::vector<Wave> g_waves;
 thread_local int g_cur = -1;
 thread_local dim3 g_block;
@@ -44,7 +102,7 @@ int g_launch_threads = 8;
 void yield()
 {
     const int me = g_cur;
-    swapcontext(&g_fibers[me].ctx, &g_main);
+    emu_switch(&g_fibers[me].sp, g_main_sp);
 }
 
 // Every lane, the last arriver included, yields once the wave has met, so the
@@ -65,16 +123,70 @@ unsigned flat_tid() { return g_fibers[g_cur].tid.x + g_block_dim.x * (g_fibers[g
 
 Wave& my_wave() { return g_waves[flat_tid() / 64]; }
 
-void fiber_entry()
+[[noreturn]] void fiber_entry()
 {
     (*g_body)();
     g_fibers[g_cur].done = true;
+    void* dead = nullptr;
+    emu_switch(&dead, g_main_sp);
+    std::abort();  // a finished fiber is never resumed
+}
+
+// A fresh stack whose first switch-in "returns" into fiber_entry with the
+// stack aligned as after a call.
+void* fresh_stack(char* base)
+{
+    uintptr_t top = (reinterpret_cast<uintptr_t>(base) + kStack) & ~uintptr_t(15);
+    void** sp = reinterpret_cast<void**>(top - 64);
+    for (int i = 0; i < 6; ++i) sp[i] = nullptr;  // r15 r14 r13 r12 rbx rbp
+    sp[6] = reinterpret_cast<void*>(&fiber_entry);
+    return sp;
+}
+
+// Names the block, the lane and the faulting address when emulated kernel code
+// faults (runs on an alternate stack, so a fiber stack overflow reports too).
+void on_fault(int sig, siginfo_t* info, void*)
+{
+    char buf[256];
+    const unsigned t = g_cur >= 0 ? g_fibers[g_cur].tid.x : ~0u;
+    const char* base = g_cur >= 0 ? g_stacks[g_cur] : nullptr;
+    const int len = std::snprintf(buf, sizeof(buf),
+                                  "emu: signal %d at address %p in block %u thread %u (fiber stack %p..%p)\n", sig,
+                                  info->si_addr, g_block.x, t, static_cast<const void*>(base),
+                                  static_cast<const void*>(base ? base + kStack : nullptr));
+    if (len > 0) (void)!write(2, buf, size_t(len));
+    signal(sig, SIG_DFL);
+    raise(sig);
+}
+
+void install_fault_reporter()
+{
+    thread_local bool done = false;
+    if (done) return;
+    done = true;
+    stack_t ss{};
+    ss.ss_size = 1 << 16;
+    ss.ss_sp = std::malloc(ss.ss_size);
+    sigaltstack(&ss, nullptr);
+    struct sigaction sa{};
+    sa.sa_sigaction = on_fault;
+    sa.sa_flags = SA_SIGINFO | SA_ONSTACK;
+    sigaction(SIGSEGV, &sa, nullptr);
+    sigaction(SIGBUS, &sa, nullptr);
 }
 
 void run_block(unsigned nthreads)
 {
+    install_fault_reporter();
     if (g_stacks.size() < nthreads)
-        while (g_stacks.size() < nthreads) g_stacks.push_back(static_cast<char*>(std::malloc(kStack)));
+        while (g_stacks.size() < nthreads) {
+            char* p = take_stack();
+            if (!p) {
+                std::fprintf(stderr, "emu: cannot allocate a fiber stack\n");
+                std::abort();
+            }
+            g_stacks.push_back(p);
+        }
     g_fibers.assign(nthreads, Fiber{});
     g_waves.assign((nthreads + 63) / 64, Wave{});
     for (size_t w = 0; w < g_waves.size(); ++w) g_waves[w].size = std::min<unsigned>(64, nthreads - unsigned(w) * 64);
@@ -82,18 +194,14 @@ void run_block(unsigned nthreads)
     for (unsigned i = 0; i < nthreads; ++i) {
         Fiber& f = g_fibers[i];
         f.tid = dim3(i % g_block_dim.x, (i / g_block_dim.x) % g_block_dim.y, i / (g_block_dim.x * g_block_dim.y));
-        getcontext(&f.ctx);
-        f.ctx.uc_stack.ss_sp = g_stacks[i];
-        f.ctx.uc_stack.ss_size = kStack;
-        f.ctx.uc_link = &g_main;
-        makecontext(&f.ctx, fiber_entry, 0);
+        f.sp = fresh_stack(g_stacks[i]);
     }
     unsigned alive = nthreads;
     while (alive) {
         for (unsigned i = 0; i < nthreads; ++i) {
             if (g_fibers[i].done) continue;
             g_cur = int(i);
-            swapcontext(&g_main, &g_fibers[i].ctx);
+            emu_switch(&g_main_sp, g_fibers[i].sp);
             if (g_fibers[i].done) --alive;
         }
     }

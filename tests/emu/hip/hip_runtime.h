@@ -66,6 +66,28 @@ T __shfl_up(T v, unsigned d, int = 64)
     return out;
 }
 template <class T>
+T __shfl_down(T v, unsigned d, int = 64)
+{
+    unsigned long long bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    bits = emu::exchange(bits, int(emu::lane() + d));
+    T out;
+    std::memcpy(&out, &bits, sizeof(T));
+    return out;
+}
+template <class T>
+T __shfl(T v, int src, int = 64)
+{
+    unsigned long long bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    bits = emu::exchange(bits, src & 63);
+    T out;
+    std::memcpy(&out, &bits, sizeof(T));
+    return out;
+}
+inline int __ffsll(long long v) { return __builtin_ffsll(v); }
+inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
+template <class T>
 T __shfl_xor(T v, int m, int = 64)
 {
     unsigned long long bits = 0;
