@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
+    ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -135,6 +136,8 @@ def main():
     ctx = svd.Context()
     if args.radix_bits is not None:
         ctx.set_option(_lib.OPT_RADIX_BITS, args.radix_bits)
+    if args.sort_tile is not None:
+        ctx.set_option(_lib.OPT_SORT_TILE, args.sort_tile)
     L = _lib.lib()
     d_keys = svd.DeviceBuffer.from_numpy(keys)
     H = _lib.hip()

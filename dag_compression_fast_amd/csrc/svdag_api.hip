@@ -133,6 +133,7 @@ struct svdag_context {
     uint64_t budget = 0;
     uint32_t load_shift = 1;
     int radix_bits = 8;
+    int sort_ipt = kSortIptSmall;
     int host_threads = 1;
     // Stage markers of the work in flight (bounded completion wait).
     static constexpr int kMaxMarks = 64;
@@ -255,7 +256,8 @@ struct BuildLayout {
     SortPlan key_plan{}, leaf_plan{};
 };
 
-BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched, int radix_bits)
+BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched, int radix_bits,
+                       int sort_ipt)
 {
     BuildLayout L;
     L.n = n;
@@ -265,7 +267,7 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.level_slots = (1ull << table_lg(L.level_bound, load_shift)) + kTableOverflow;
     // Keys only need grouping by code >> 6 (the leaf pack ORs the low 6 bits), so
     // the sort skips bits [0, 6).
-    L.key_plan = make_sort_plan(6, 3 * depth, radix_bits);
+    L.key_plan = make_sort_plan(6, 3 * depth, radix_bits, sort_ipt);
     L.leaf_plan = make_sort_plan(0, 64);
     const uint64_t stage_tiles = std::max({tiles(n, kPackTileItems), tiles(L.leaf_bound, kBuildTileItems),
                                            tiles(L.level_slots, kExtractTileItems), uint64_t(1)});
@@ -419,7 +421,8 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     out->num_keys = n;
     if (n == 0) return 0;  // DAGCompression.cu:1010-1013
 
-    const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits);
+    const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits,
+                                     ctx->sort_ipt);
     const uint64_t offsets_bytes = batch ? align_up(8 * (batch->num_chunks + 1)) : 0;
     TRY(ensure_arena(ctx, L.total + offsets_bytes));
     char* A = ctx->arena;
@@ -891,6 +894,10 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         if (value < 4 || value > kMaxRadixBits) return set_error(SVDAG_ERR_ARGUMENT, "radix bits must be 4..%d", kMaxRadixBits);
         ctx->radix_bits = int(value);
         return 0;
+    case SVDAG_OPT_SORT_TILE:
+        if (value != 4096 && value != 8192) return set_error(SVDAG_ERR_ARGUMENT, "sort tile must be 4096 or 8192 keys");
+        ctx->sort_ipt = value == 8192 ? kSortIptLarge : kSortIptSmall;
+        return 0;
     default: return set_error(SVDAG_ERR_ARGUMENT, "unknown option %d", option);
     }
 }
@@ -898,7 +905,8 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
 uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes)
 {
     if (depth < 3 || depth > 21 || n == 0) return 0;
-    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false, 4).total;  // narrowest digits: most passes
+    // Narrowest digits: the most passes and the largest histogram region.
+    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false, 4, kSortIptSmall).total;
 }
 
 int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)

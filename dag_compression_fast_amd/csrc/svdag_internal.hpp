@@ -70,28 +70,33 @@ struct TimedLaunch {
 constexpr int kMaxRadixBits = 9;
 constexpr int kRadix = 1 << kMaxRadixBits;
 constexpr int kMaxPasses = 8;
-// 256 threads x 16 keys: a tile's LDS (keys, values, per-wave digit counters)
-// stays under 64 KB.
+// A tile is 256 threads x ipt keys: 16 (4096 keys; key/value sorts) or 32
+// (8192 keys; keys-only sorts).
 constexpr int kSortThreads = 256;
-constexpr int kSortIpt = 16;
-constexpr int kSortTile = kSortThreads * kSortIpt;  // keys per tile
+constexpr int kSortIptSmall = 16;
+constexpr int kSortIptLarge = 32;
 
 struct SortPlan {
     int passes;
     uint32_t shift[kMaxPasses];
     uint32_t bits[kMaxPasses];
+    int ipt;  // keys per thread and tile
 };
 
 // Digits of width <= max_bits covering bits [lo, hi), spread evenly.
-SortPlan make_sort_plan(int lo, int hi, int max_bits = kMaxRadixBits);
+SortPlan make_sort_plan(int lo, int hi, int max_bits = kMaxRadixBits, int ipt = kSortIptSmall);
 
-inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
+inline uint64_t sort_tiles(uint64_t n, int ipt)
+{
+    const uint64_t tile = uint64_t(kSortThreads) * ipt;
+    return (n + tile - 1) / tile;
+}
 
 // Device scratch of one sort: the all-pass digit histogram (passes * kRadix
 // u32, zero before launch_sort) and the per-tile digit counts
-// (sort_count_words u32, any contents).
+// (sort_count_words u32 for any tile size, any contents).
 inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) * kRadix; }
-inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound); }
+inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound, kSortIptSmall); }
 
 // LSD radix sort of keys (and optionally u32 values) on the plan's bits; keys
 // equal on those bits keep their input order.

@@ -19,13 +19,17 @@
 
 namespace svdag {
 
-SortPlan make_sort_plan(int lo, int hi, int max_bits)
+SortPlan make_sort_plan(int lo, int hi, int max_bits, int ipt)
 {
     SortPlan p{};
+    p.ipt = ipt == kSortIptLarge ? kSortIptLarge : kSortIptSmall;
     const int key_bits = hi - lo;
     if (key_bits <= 0 || max_bits < 1 || max_bits > kMaxRadixBits) return p;
     p.passes = (key_bits + max_bits - 1) / max_bits;
-    if (p.passes > kMaxPasses) return SortPlan{};
+    if (p.passes > kMaxPasses) {
+        p.passes = 0;
+        return p;
+    }
     const int base = key_bits / p.passes, extra = key_bits % p.passes;
     uint32_t shift = uint32_t(lo);
     for (int i = 0; i < p.passes; ++i) {
@@ -97,26 +101,33 @@ __global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, c
     }
 }
 
-// Per-tile digit counts, digit-major: counts[d * stride + tile].
-template <typename K>
+// Per-tile digit counts, digit-major: counts[d * stride + tile].  Tiles are
+// walked XCD by XCD, so the counts of neighbouring tiles are written from one L2.
+template <typename K, int IPT>
 __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict__ keys,
                                                              const uint64_t* __restrict__ n_ptr, uint32_t shift,
                                                              uint32_t digit_mask, uint32_t* __restrict__ counts,
                                                              uint64_t stride)
 {
+    constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
     __shared__ uint32_t s_cnt[kRadix];
     const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix = digit_mask + 1;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const TileWalk walk = xcd_tiles(ntiles);
+    for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
         for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) s_cnt[d] = 0;
         __syncthreads();
-        const uint64_t base = tile * kSortTile;
-#pragma unroll 4
-        for (int i = 0; i < kSortIpt; ++i) {
+        const uint64_t base = tile * kTile;
+        uint32_t dg[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
             const uint64_t idx = base + uint64_t(i) * kSortThreads + threadIdx.x;
-            if (idx < n) atomicAdd(&s_cnt[uint32_t(keys[idx] >> shift) & digit_mask], 1u);
+            dg[i] = idx < n ? uint32_t(keys[idx] >> shift) & digit_mask : ~0u;
         }
+#pragma unroll
+        for (int i = 0; i < IPT; ++i)
+            if (dg[i] != ~0u) atomicAdd(&s_cnt[dg[i]], 1u);
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
         __syncthreads();
@@ -129,13 +140,14 @@ constexpr int kScanThreads = 1024;
 
 __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ hist,
-                                                            const uint64_t* __restrict__ n_ptr, uint64_t stride)
+                                                            const uint64_t* __restrict__ n_ptr, uint64_t stride,
+                                                            uint64_t tile_keys)
 {
     __shared__ uint64_t s_red[kScanThreads / kWave];
     __shared__ uint32_t s_scan[kScanThreads / kWave];
     const uint32_t d = blockIdx.x;
     const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t ntiles = (n + tile_keys - 1) / tile_keys;
     // Global start of digit d.
     uint64_t below = 0;
     for (uint32_t i = threadIdx.x; i < d; i += kScanThreads) below += hist[i];
@@ -151,7 +163,7 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
     }
 }
 
-template <typename K, bool PAIRS>
+template <typename K, bool PAIRS, int IPT>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                                const uint32_t* __restrict__ vals_in,
                                                                uint32_t* __restrict__ vals_out,
@@ -160,10 +172,15 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                const uint32_t* __restrict__ offsets, uint64_t stride)
 {
     constexpr int W = kSortThreads / kWave;
+    constexpr uint32_t kTile = uint32_t(kSortThreads) * IPT;
     constexpr int DPT = kRadix / kSortThreads;  // digit slots per thread in the tile scan
-    __shared__ K s_keys[kSortTile];
-    __shared__ uint32_t s_vals[PAIRS ? kSortTile : 1];
+    // The staging area holds the tile in digit order; during ranking it holds
+    // the per-wave match table s_match[W][kRadix] instead.
+    static_assert(W * kRadix * sizeof(uint64_t) <= kTile * sizeof(K), "match table fits the staging area");
+    __shared__ K s_keys[kTile];
+    __shared__ uint32_t s_vals[PAIRS ? kTile : 1];
     __shared__ uint32_t s_wcnt[W][kRadix];
+    uint64_t* const s_match = reinterpret_cast<uint64_t*>(s_keys) + (threadIdx.x / kWave) * kRadix;
     __shared__ uint32_t s_lstart[kRadix];
     __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_scan[W];
@@ -172,61 +189,70 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     const uint32_t w = tid / kWave;
     const uint32_t lane = lane_id();
     const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix = digit_mask + 1;
-    const uint32_t bits = uint32_t(__popc(digit_mask));
 
-    const uint32_t wave_base = w * kWave * kSortIpt;
+    const uint32_t wave_base = w * kWave * IPT;
     TileWalk walk = xcd_tiles(ntiles);
-    K key[kSortIpt];
-    uint32_t val[PAIRS ? kSortIpt : 1];
-    // Keys of the tile at walk.tile (loaded one tile ahead).
+    K key[IPT];
+    uint32_t val[PAIRS ? IPT : 1];
+    // Keys of tile t (loaded one tile ahead).  Full tiles, the common case, load
+    // without per-key guards.
     auto load_tile = [&](uint64_t t) {
-        const uint64_t base = t * kSortTile;
-        const uint32_t cnt = t < walk.end ? uint32_t(min(uint64_t(kSortTile), n - base)) : 0u;
+        const uint64_t base = t * kTile;
+        const uint32_t cnt = t < walk.end ? uint32_t(min(uint64_t(kTile), n - base)) : 0u;
+        const K* src = keys_in + base + wave_base + lane;
+        const uint32_t* vs = PAIRS ? vals_in + base + wave_base + lane : nullptr;
+        if (cnt == kTile) {
 #pragma unroll
-        for (int i = 0; i < kSortIpt; ++i) {
-            const uint32_t idx = wave_base + i * kWave + lane;
-            const bool valid = idx < cnt;
-            key[i] = valid ? keys_in[base + idx] : K(0);
-            if constexpr (PAIRS) val[i] = valid ? vals_in[base + idx] : 0u;
+            for (int i = 0; i < IPT; ++i) {
+                key[i] = src[i * kWave];
+                if constexpr (PAIRS) val[i] = vs[i * kWave];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                const bool valid = wave_base + i * kWave + lane < cnt;
+                key[i] = valid ? src[i * kWave] : K(0);
+                if constexpr (PAIRS) val[i] = valid ? vs[i * kWave] : 0u;
+            }
         }
     };
     load_tile(walk.tile);
 
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
-        for (uint32_t i = tid; i < W * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0;
+        for (uint32_t i = tid; i < W * kRadix; i += kSortThreads) {
+            (&s_wcnt[0][0])[i] = 0;
+            reinterpret_cast<uint64_t*>(s_keys)[i] = 0;
+        }
         __syncthreads();
-        const uint64_t tile_base = tile * kSortTile;
-        const uint32_t tile_n = uint32_t(min(uint64_t(kSortTile), n - tile_base));
+        const uint64_t tile_base = tile * kTile;
+        const uint32_t tile_n = uint32_t(min(uint64_t(kTile), n - tile_base));
 
-        uint32_t dig[kSortIpt];
-        uint32_t rnk[kSortIpt];
-#pragma unroll
-        for (int i = 0; i < kSortIpt; ++i) dig[i] = uint32_t(key[i] >> shift) & digit_mask;
+        // Per key: (rank among the wave's keys of its digit) << 16 | digit.
+        uint32_t dr[IPT];
         uint32_t* wc = s_wcnt[w];
 #pragma unroll
-        for (int i = 0; i < kSortIpt; ++i) {
+        for (int i = 0; i < IPT; ++i) {
             const uint32_t idx = wave_base + i * kWave + lane;
             const bool valid = idx < tile_n;
-            const uint32_t d = dig[i];
-            // Lanes holding the same digit: AND of one ballot per digit bit.
-            uint64_t match = __ballot(valid);
-#pragma unroll
-            for (uint32_t b = 0; b < uint32_t(kMaxRadixBits); ++b) {
-                if (b < bits) {
-                    const bool bit = (d >> b) & 1u;
-                    const uint64_t vote = __ballot(bit);
-                    match &= bit ? vote : ~vote;
-                }
-            }
+            const uint32_t d = uint32_t(key[i] >> shift) & digit_mask;
+            // Lanes holding the same digit: every lane ORs its bit into the wave's
+            // word for its digit (the OR does not depend on the order of the
+            // lanes), reads the word back and clears it for the next key.
+            uint64_t* const slot = s_match + d;
+            if (valid) atomicOr((unsigned long long*)slot, 1ull << lane);
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t match = valid ? *slot : 0ull;
+            __builtin_amdgcn_wave_barrier();
+            if (valid) *slot = 0ull;
             const uint32_t below = __popcll(match & lanemask_lt());
             uint32_t c = 0;
             if (valid) c = wc[d];
             // Every lane of the group reads the counter before its highest lane
             // (the leader) writes it back.
             __builtin_amdgcn_wave_barrier();
-            rnk[i] = c + below;
+            dr[i] = (c + below) << 16 | d;
             if (valid && (match >> lane) == 1ull) wc[d] = c + uint32_t(__popcll(match));
             __builtin_amdgcn_wave_barrier();
         }
@@ -265,11 +291,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         __syncthreads();
 
 #pragma unroll
-        for (int i = 0; i < kSortIpt; ++i) {
+        for (int i = 0; i < IPT; ++i) {
             const uint32_t idx = wave_base + i * kWave + lane;
             if (idx < tile_n) {
-                const uint32_t d = dig[i];
-                const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + rnk[i];
+                const uint32_t d = dr[i] & 0xFFFFu;
+                const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + (dr[i] >> 16);
                 s_keys[pos] = key[i];
                 if constexpr (PAIRS) s_vals[pos] = val[i];
             }
@@ -277,12 +303,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         __syncthreads();
         // Next tile's loads go out before this tile's stores.
         load_tile(tile + walk.step);
-        for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
+        auto store = [&](uint32_t j) {
             const K k = s_keys[j];
             const uint32_t d = uint32_t(k >> shift) & digit_mask;
             const uint32_t o = s_base[d] + j;
             keys_out[o] = k;
             if constexpr (PAIRS) vals_out[o] = s_vals[j];
+        };
+        if (tile_n == kTile) {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) store(tid + i * kSortThreads);
+        } else {
+            for (uint32_t j = tid; j < tile_n; j += kSortThreads) store(j);
         }
         __syncthreads();
     }
@@ -303,14 +335,43 @@ int device_cu_count()
     return cus;
 }
 
+template <typename K, int IPT>
+static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
+                        const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
+                        uint64_t stride, uint32_t grid)
+{
+    {
+        TimedLaunch t(kKSortCount, stream);
+        hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
+                           counts, stride);
+    }
+    {
+        TimedLaunch t(kKSortScan, stream);
+        hipLaunchKernelGGL(k_sort_scan, dim3(dmask + 1), dim3(kScanThreads), 0, stream, counts, hist, n_ptr, stride,
+                           uint64_t(kSortThreads) * IPT);
+    }
+    if (vsrc) {
+        TimedLaunch t(kKSortScatterPairs, stream);
+        hipLaunchKernelGGL((k_sort_scatter<K, true, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
+                           vdst, n_ptr, shift, dmask, counts, stride);
+    } else {
+        TimedLaunch t(kKSortScatterKeys, stream);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
+                           nullptr, nullptr, n_ptr, shift, dmask, counts, stride);
+    }
+}
+
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in, K* keys_a,
                 K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
                 const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts, unsigned long long* key_or)
 {
     if (plan.passes == 0 || n_bound == 0) return -1;
+    const bool pairs = vals_in != nullptr;
+    // Key/value tiles stay at 4096 keys (their LDS would exceed 64 KB at 8192).
+    const int ipt = pairs ? kSortIptSmall : plan.ipt;
     uint32_t* hist = zero;
-    const uint64_t stride = sort_tiles(n_bound);
+    const uint64_t stride = sort_tiles(n_bound, ipt);
     HistPlan hp{};
     hp.passes = plan.passes;
     for (int p = 0; p < plan.passes; ++p) {
@@ -318,18 +379,18 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         hp.mask[p] = (1u << plan.bits[p]) - 1u;
     }
     const int cus = device_cu_count();
-    const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 8));
+    // One histogram flush per workgroup: a few workgroups per CU suffice.
+    const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 2));
     {
         TimedLaunch t(kKHistogram, stream);
         hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist,
                            key_or);
     }
 
-    const bool pairs = vals_in != nullptr;
     const K* src = keys_in;
     const uint32_t* vsrc = vals_in;
     int where = -1;
-    // A multiple of the XCD count, as the scatter's tile walk assumes.
+    // A multiple of the XCD count, as the tile walk assumes.
     const uint32_t grid = uint32_t(
         std::max<uint64_t>(kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
     for (int p = 0; p < plan.passes; ++p) {
@@ -343,25 +404,12 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         }
         if (last && final_vals) vdst = final_vals;
         const uint32_t dmask = (1u << plan.bits[p]) - 1u;
-        {
-            TimedLaunch t(kKSortCount, stream);
-            hipLaunchKernelGGL(k_sort_count<K>, dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, plan.shift[p],
-                               dmask, counts, stride);
-        }
-        {
-            TimedLaunch t(kKSortScan, stream);
-            hipLaunchKernelGGL(k_sort_scan, dim3(dmask + 1), dim3(kScanThreads), 0, stream, counts, hist + p * kRadix,
-                               n_ptr, stride);
-        }
-        if (pairs) {
-            TimedLaunch t(kKSortScatterPairs, stream);
-            hipLaunchKernelGGL((k_sort_scatter<K, true>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
-                               vdst, n_ptr, plan.shift[p], dmask, counts, stride);
-        } else {
-            TimedLaunch t(kKSortScatterKeys, stream);
-            hipLaunchKernelGGL((k_sort_scatter<K, false>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
-                               nullptr, nullptr, n_ptr, plan.shift[p], dmask, counts, stride);
-        }
+        if (ipt == kSortIptLarge)
+            launch_pass<K, kSortIptLarge>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
+                                          hist + p * kRadix, stride, grid);
+        else
+            launch_pass<K, kSortIptSmall>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
+                                          hist + p * kRadix, stride, grid);
         src = dst;
         vsrc = vdst;
     }

@@ -194,19 +194,20 @@ def test_batched_matches_merge(gpu, oracle):
     assert np.array_equal(gpu.create_final_dag(batched), gpu.create_final_dag(merged))
 
 
-@pytest.mark.parametrize("bits", [4, 6, 7, 9])
-def test_radix_digit_widths(gpu, oracle, bits):
+@pytest.mark.parametrize("bits,tile", [(4, 4096), (6, 8192), (7, 4096), (9, 4096), (9, 8192)])
+def test_radix_digit_widths(gpu, oracle, bits, tile):
     from dag_compression_fast_amd import _lib
     ctx = gpu.Context()
     try:
         ctx.set_option(_lib.OPT_RADIX_BITS, bits)
+        ctx.set_option(_lib.OPT_SORT_TILE, tile)
         ctx.set_option(_lib.OPT_WAIT_SECONDS, 60)
-        rng = np.random.default_rng(3000 + bits)
+        rng = np.random.default_rng(3000 + bits + tile)
         keys = clustered_keys(rng, 10, 400000)
-        print(f"[gpu-parity] radix bits {bits}: depth=10 n={len(keys)}", flush=True)
+        print(f"[gpu-parity] radix bits {bits} tile {tile}: depth=10 n={len(keys)}", flush=True)
         exp = oracle.create_dag(keys, 10, threads=8)
         dag = gpu.create_dag(keys, 10, ctx=ctx)
-        assert_dag_equal(dag.to_numpy(), exp, f"radix {bits}")
+        assert_dag_equal(dag.to_numpy(), exp, f"radix {bits} tile {tile}")
         assert np.array_equal(gpu.create_final_dag(dag), oracle.create_final_dag(exp))
         dag.free()
     finally:
