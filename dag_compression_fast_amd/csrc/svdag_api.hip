@@ -132,7 +132,7 @@ struct svdag_context {
     int check_collisions = 0;
     uint64_t budget = 0;
     uint32_t load_shift = 1;
-    int radix_bits = 8;
+    int radix_bits = 9;
     int sort_ipt = kSortIptSmall;
     int host_threads = 1;
     // Stage markers of the work in flight (bounded completion wait).

@@ -166,16 +166,21 @@ __global__ __launch_bounds__(kThreads) void k_leaf_count(const K* __restrict__ s
     if (threadIdx.x == 0 && gtot) atomicAdd((unsigned long long*)next_count, (unsigned long long)gtot);
 }
 
+// Each key ORs its bit into its run's word in the wave's LDS area (the OR does
+// not depend on lane order); head lanes write their run's code, and the words
+// are written out after the wave's last group.
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ sorted, const uint64_t* __restrict__ n_ptr,
                                                         uint64_t code_mask, const uint64_t* __restrict__ offsets,
                                                         uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask)
 {
     __shared__ uint32_t s_wcount[kThreads / kWave];
+    __shared__ uint64_t s_or[kThreads / kWave][kPackWaveKeys];
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / kWave;
+    uint64_t* const wor = s_or[w];
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t chunk = tile * kPackTile + uint64_t(w) * kPackWaveKeys;
         uint64_t k[kPackGroups], heads[kPackGroups];
@@ -185,65 +190,45 @@ __global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ so
 #pragma unroll
         for (int g = 0; g < kPackGroups; ++g) cnt += uint32_t(__popcll(heads[g]));
         if (lane == 0) s_wcount[w] = cnt;
+#pragma unroll
+        for (int i = 0; i < kPackGroups; ++i) wor[i * kWave + lane] = 0ull;
         __syncthreads();
         uint64_t q = offsets[tile];
         for (uint32_t i = 0; i < w; ++i) q += s_wcount[i];
         __syncthreads();  // s_wcount is rewritten by the next tile
 
-        // Wave-uniform state of the run still open at the end of the last group.
-        bool open = false;
-        uint64_t open_code = 0, open_or = 0, open_q = 0;
+        // Run r of this wave = the r-th head; keys before the wave's first head
+        // continue a run owned by the previous wave.
+        int32_t run_base = -1;
 #pragma unroll
         for (int g = 0; g < kPackGroups; ++g) {
             const uint64_t i = chunk + uint64_t(g) * kWave + lane;
             const uint64_t H = heads[g];
-            uint64_t v = i < n ? (1ull << (k[g] & 63)) : 0ull;
-            // Segmented OR over [lane, next head after lane).
-            const uint64_t after = lane == kWave - 1 ? 0ull : (H & (~0ull << (lane + 1)));
-            const uint32_t nh = after ? uint32_t(__ffsll((long long)after) - 1) : uint32_t(kWave);
-#pragma unroll
-            for (uint32_t off = 1; off < uint32_t(kWave); off <<= 1) {
-                const uint64_t o = __shfl_down(v, off, kWave);
-                if (lane + off < nh) v |= o;
-            }
-            // Lanes before the group's first head continue the open run (or a
-            // run owned by the previous wave, which is then ignored).
-            const uint64_t prefix = __shfl(v, 0, kWave);
-            if (!(H & 1ull)) open_or |= prefix;
-            if (H) {
-                if (open && lane == 0) {
-                    lp_code[open_q] = open_code;
-                    lp_mask[open_q] = open_or;
-                }
-                const uint32_t nheads = uint32_t(__popcll(H));
-                const uint32_t last = 63u - uint32_t(__clzll((long long)H));
-                if (((H >> lane) & 1ull) && lane != last) {
-                    const uint64_t r = q + uint32_t(__popcll(H & lanemask_lt()));
-                    lp_code[r] = k[g] >> 6;
-                    lp_mask[r] = v;
-                }
-                open = true;
-                open_q = q + nheads - 1;
-                open_code = __shfl(k[g], last, kWave) >> 6;
-                open_or = __shfl(v, last, kWave);
-                q += nheads;
-            }
+            const int32_t r = run_base + int32_t(__popcll(H & (lanemask_lt() | (1ull << lane))));
+            if (i < n && r >= 0) atomicOr((unsigned long long*)&wor[r], 1ull << (k[g] & 63));
+            if ((H >> lane) & 1ull) lp_code[q + uint32_t(r)] = k[g] >> 6;
+            run_base += int32_t(__popcll(H));
         }
-        if (open) {
+        if (run_base >= 0) {
             // The wave's last run may continue past its keys.
-            for (uint64_t i0 = chunk + kPackWaveKeys; i0 < n; i0 += kWave) {
-                const uint64_t i = i0 + lane;
-                const uint64_t kk = load_key(sorted, i, n, code_mask);
-                const uint64_t out = __ballot(!(i < n && (kk >> 6) == open_code));
-                const uint32_t stop = out ? uint32_t(__ffsll((long long)out) - 1) : uint32_t(kWave);
-                open_or |= wave_or(lane < stop ? (1ull << (kk & 63)) : 0ull);
-                if (stop < uint32_t(kWave)) break;
-            }
-            if (lane == 0) {
-                lp_code[open_q] = open_code;
-                lp_mask[open_q] = open_or;
+            const uint64_t open_code = __shfl(k[kPackGroups - 1], kWave - 1, kWave) >> 6;
+            uint64_t extra = 0;
+            const uint64_t end = chunk + kPackWaveKeys;
+            const bool runs_on = end < n && (uint64_t(sorted[end]) & code_mask) >> 6 == open_code;
+            if (runs_on) {
+                for (uint64_t i0 = end; i0 < n; i0 += kWave) {
+                    const uint64_t i = i0 + lane;
+                    const uint64_t kk = load_key(sorted, i, n, code_mask);
+                    const uint64_t out = __ballot(!(i < n && (kk >> 6) == open_code));
+                    const uint32_t stop = out ? uint32_t(__ffsll((long long)out) - 1) : uint32_t(kWave);
+                    extra |= wave_or(lane < stop ? (1ull << (kk & 63)) : 0ull);
+                    if (stop < uint32_t(kWave)) break;
+                }
+                if (lane == 0) atomicOr((unsigned long long*)&wor[run_base], extra);
             }
         }
+        __builtin_amdgcn_wave_barrier();
+        for (int32_t r = int32_t(lane); r <= run_base; r += kWave) lp_mask[q + uint32_t(r)] = wor[r];
     }
 }
 
@@ -449,33 +434,50 @@ __device__ __forceinline__ void acc_add(ParentAcc& a, uint32_t slot, uint32_t ra
     }
 }
 
-__device__ __forceinline__ void acc_emit(const ParentAcc& a, uint64_t p, const LevelBuildArgs& g, uint32_t lg,
-                                         uint64_t limit)
+// Writes a parent's record and returns its hash.
+__device__ __forceinline__ uint64_t acc_emit(const ParentAcc& a, uint64_t p, const LevelBuildArgs& g)
 {
-    const uint64_t h = hash_children(a.hash);
     g.parent_codes[p] = a.code;
     g.parent_masks[p] = uint8_t(a.mask);
     uint4* dst = reinterpret_cast<uint4*>(g.parent_children + 8 * p);
     dst[0] = make_uint4(a.child[0], a.child[1], a.child[2], a.child[3]);
     dst[1] = make_uint4(a.child[4], a.child[5], a.child[6], a.child[7]);
-    g.parent_slots[p] = ordered_insert(g.table_keys, g.table_rep, lg, limit, h, uint32_t(p), g.counters);
+    return hash_children(a.hash);
 }
 
+// A thread owns kBuildIpt consecutive children; their rank and hash gathers are
+// issued together.  The tile's parent hashes are staged in LDS and inserted into
+// the level's table one thread per parent.
 __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ uint64_t s_hash[kBuildTile];
     const uint64_t n = *g.child_count;
     const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
     const uint32_t lg = *g.table_log2;
     const uint64_t limit = (1ull << lg) + kTableOverflow;
+    const uint32_t* __restrict__ slots = g.child_slots;
+    const uint32_t* __restrict__ rank_of_slot = g.child_rank_of_slot;
+    const uint64_t* __restrict__ hashes = g.child_hashes;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t first = tile * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt;
         uint64_t code[kBuildIpt];
         uint32_t starts, gcount;
         parent_runs(g.child_codes, n, first, code, starts, gcount);
-        uint32_t unused;
-        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, unused);
-        uint64_t p = g.offsets[tile] + local;
+        uint32_t tile_parents;
+        const uint32_t local = block_exclusive_scan<kThreads>(__popc(starts), s_scan, tile_parents);
+        const uint64_t p0 = g.offsets[tile];
+
+        // Indices are clamped into [0, n): loads past the end read a valid child
+        // that is then ignored.
+        uint32_t rank[kBuildIpt];
+        uint64_t hsh[kBuildIpt];
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) rank[j] = rank_of_slot[slots[min(first + j, n - 1)]];
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) hsh[j] = hashes[rank[j]];
+
+        uint32_t q = local;
         bool open = false;
         ParentAcc acc;
         acc_reset(acc, 0);
@@ -483,26 +485,30 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
         for (int j = 0; j < kBuildIpt; ++j) {
             if ((starts >> j) & 1u) {
                 if (open) {
-                    acc_emit(acc, p, g, lg, limit);
-                    ++p;
+                    s_hash[q] = acc_emit(acc, p0 + q, g);
+                    ++q;
                 }
                 open = true;
                 acc_reset(acc, code[j] >> 3);
             }
-            if (open && first + j < n) {
-                const uint32_t rank = g.child_rank_of_slot[g.child_slots[first + j]];
-                acc_add(acc, uint32_t(code[j] & 7), rank, g.child_hashes[rank]);
-            }
+            if (open && first + j < n) acc_add(acc, uint32_t(code[j] & 7), rank[j], hsh[j]);
         }
         if (open) {
+            // The last parent may take children past this thread's (at most 7).
             for (uint64_t i = first + kBuildIpt; i < n; ++i) {
                 const uint64_t cc = g.child_codes[i];
                 if ((cc >> 3) != acc.code) break;
-                const uint32_t rank = g.child_rank_of_slot[g.child_slots[i]];
-                acc_add(acc, uint32_t(cc & 7), rank, g.child_hashes[rank]);
+                const uint32_t r = rank_of_slot[slots[i]];
+                acc_add(acc, uint32_t(cc & 7), r, hashes[r]);
             }
-            acc_emit(acc, p, g, lg, limit);
+            s_hash[q] = acc_emit(acc, p0 + q, g);
         }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < tile_parents; k += kThreads) {
+            const uint64_t p = p0 + k;
+            g.parent_slots[p] = ordered_insert(g.table_keys, g.table_rep, lg, limit, s_hash[k], uint32_t(p), g.counters);
+        }
+        __syncthreads();  // s_hash is rewritten by the next tile
     }
 }
 
@@ -583,6 +589,19 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
         uint32_t unused;
         const uint32_t local = block_exclusive_scan<kThreads>(__popc(occ), s_scan, unused);
         uint64_t before = g.offsets[tile] + local;  // occupied slots before the current one
+        // Representatives' records, gathered up front (unoccupied slots read
+        // parent 0, which is then ignored).
+        uint32_t mask[kExtractIpt];
+        uint4 ch0[kExtractIpt], ch1[kExtractIpt];
+#pragma unroll
+        for (int j = 0; j < kExtractIpt; ++j) {
+            const uint32_t r = g.table_rep[min(uint64_t(first + j), total - 1)];
+            const uint64_t rep = ((occ >> j) & 1u) ? r : 0u;
+            mask[j] = g.parent_masks[rep];
+            const uint4* src = reinterpret_cast<const uint4*>(g.parent_children + 8 * rep);
+            ch0[j] = src[0];
+            ch1[j] = src[1];
+        }
 #pragma unroll
         for (int j = 0; j < kExtractIpt; ++j) {
             if ((occ >> j) & 1u) {
@@ -597,14 +616,11 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
                 const uint64_t rank = before - uint64_t(k - js) + lt;
                 g.rank_of_slot[k] = uint32_t(rank);
                 g.out_hashes[rank] = key;
-                const uint32_t rep = g.table_rep[k];
-                const uint8_t m = g.parent_masks[rep];
-                g.out_masks[rank] = m;
-                const uint4* src = reinterpret_cast<const uint4*>(g.parent_children + 8 * uint64_t(rep));
+                g.out_masks[rank] = uint8_t(mask[j]);
                 uint4* dst = reinterpret_cast<uint4*>(g.out_children + 8 * rank);
-                dst[0] = src[0];
-                dst[1] = src[1];
-                wsum += 1 + __popc(uint32_t(m));
+                dst[0] = ch0[j];
+                dst[1] = ch1[j];
+                wsum += 1 + __popc(mask[j]);
                 ++before;
             }
         }

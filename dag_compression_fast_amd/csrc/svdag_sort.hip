@@ -69,27 +69,60 @@ struct HistPlan {
     uint32_t mask[kMaxPasses];
 };
 
-// Digit totals of every pass in one read of the keys; OR of the keys for the
-// range check.
-template <typename K>
-__global__ __launch_bounds__(256) void k_histogram(const K* __restrict__ keys, const uint64_t* __restrict__ n_ptr,
-                                                   HistPlan plan, uint32_t* __restrict__ hist,
-                                                   unsigned long long* __restrict__ key_or)
+// One read of the keys gives the first pass's per-tile digit counts
+// (counts[d * stride + tile], walked XCD by XCD like the count kernel), the
+// digit totals of every pass, and the OR of the keys (range check).
+template <typename K, int IPT>
+__global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict__ keys,
+                                                            const uint64_t* __restrict__ n_ptr, HistPlan plan,
+                                                            uint32_t* __restrict__ hist,
+                                                            unsigned long long* __restrict__ key_or,
+                                                            uint32_t* __restrict__ counts, uint64_t stride)
 {
+    constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
     __shared__ uint32_t s_hist[kMaxPasses][kRadix];
-    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += 256) (&s_hist[0][0])[i] = 0;
-    __syncthreads();
+    __shared__ uint32_t s_cnt[kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortThreads) (&s_hist[0][0])[i] = 0;
     const uint64_t n = *n_ptr;
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    const uint32_t radix0 = plan.mask[0] + 1;
+    const TileWalk walk = xcd_tiles(ntiles);
     K acc = 0;
-    const uint64_t stride = uint64_t(gridDim.x) * 256;
-    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
-        const K k = keys[i];
-        acc |= k;
-        for (int p = 0; p < plan.passes; ++p) atomicAdd(&s_hist[p][uint32_t(k >> plan.shift[p]) & plan.mask[p]], 1u);
+    for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
+        for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) s_cnt[d] = 0;
+        __syncthreads();
+        const uint64_t base = tile * kTile;
+        const uint32_t cnt = uint32_t(min(kTile, n - base));
+        K k[IPT];
+        if (cnt == kTile) {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) k[i] = keys[base + uint64_t(i) * kSortThreads + threadIdx.x];
+        } else {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
+                k[i] = idx < cnt ? keys[base + idx] : K(0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            if (uint32_t(i) * kSortThreads + threadIdx.x < cnt) {
+                acc |= k[i];
+                atomicAdd(&s_cnt[uint32_t(k[i] >> plan.shift[0]) & plan.mask[0]], 1u);
+                for (int p = 1; p < plan.passes; ++p)
+                    atomicAdd(&s_hist[p][uint32_t(k[i] >> plan.shift[p]) & plan.mask[p]], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) {
+            const uint32_t c = s_cnt[d];
+            counts[uint64_t(d) * stride + tile] = c;
+            s_hist[0][d] += c;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int p = 0; p < plan.passes; ++p)
-        for (uint32_t d = threadIdx.x; d <= plan.mask[p]; d += 256) {
+        for (uint32_t d = threadIdx.x; d <= plan.mask[p]; d += kSortThreads) {
             const uint32_t v = s_hist[p][d];
             if (v) atomicAdd(&hist[p * kRadix + d], v);
         }
@@ -338,9 +371,9 @@ int device_cu_count()
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
-                        uint64_t stride, uint32_t grid)
+                        uint64_t stride, uint32_t grid, bool counted)
 {
-    {
+    if (!counted) {
         TimedLaunch t(kKSortCount, stream);
         hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
                            counts, stride);
@@ -379,20 +412,22 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         hp.mask[p] = (1u << plan.bits[p]) - 1u;
     }
     const int cus = device_cu_count();
-    // One histogram flush per workgroup: a few workgroups per CU suffice.
-    const int hgrid = int(std::min<uint64_t>((n_bound + 255) / 256, uint64_t(cus) * 2));
-    {
-        TimedLaunch t(kKHistogram, stream);
-        hipLaunchKernelGGL(k_histogram<K>, dim3(std::max(1, hgrid)), dim3(256), 0, stream, keys_in, n_ptr, hp, hist,
-                           key_or);
-    }
-
     const K* src = keys_in;
     const uint32_t* vsrc = vals_in;
     int where = -1;
     // A multiple of the XCD count, as the tile walk assumes.
     const uint32_t grid = uint32_t(
         std::max<uint64_t>(kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
+    {
+        // Also writes the first pass's tile counts.
+        TimedLaunch t(kKHistogram, stream);
+        if (ipt == kSortIptLarge)
+            hipLaunchKernelGGL((k_histogram<K, kSortIptLarge>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
+                               n_ptr, hp, hist, key_or, counts, stride);
+        else
+            hipLaunchKernelGGL((k_histogram<K, kSortIptSmall>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
+                               n_ptr, hp, hist, key_or, counts, stride);
+    }
     for (int p = 0; p < plan.passes; ++p) {
         const bool last = p + 1 == plan.passes;
         K* dst = (p % 2 == 0) ? keys_a : keys_b;
@@ -406,10 +441,10 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         const uint32_t dmask = (1u << plan.bits[p]) - 1u;
         if (ipt == kSortIptLarge)
             launch_pass<K, kSortIptLarge>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid);
+                                          hist + p * kRadix, stride, grid, p == 0);
         else
             launch_pass<K, kSortIptSmall>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid);
+                                          hist + p * kRadix, stride, grid, p == 0);
         src = dst;
         vsrc = vdst;
     }

@@ -82,7 +82,7 @@ typedef enum svdag_option {
     SVDAG_OPT_HOST_THREADS = 4,       /* host threads for host-side work (merge)      */
     SVDAG_OPT_WAIT_SECONDS = 5,       /* completion deadline of a call (default 120 s); past it
                                          the call fails and names the stage that stalled */
-    SVDAG_OPT_RADIX_BITS = 6,         /* widest digit of the key sort, 4..9 (default 8)      */
+    SVDAG_OPT_RADIX_BITS = 6,         /* widest digit of the key sort, 4..9 (default 9)      */
     SVDAG_OPT_SORT_TILE = 7           /* keys per key-sort tile: 4096 or 8192 (default 4096) */
 } svdag_option;
 
