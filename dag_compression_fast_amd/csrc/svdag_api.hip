@@ -31,7 +31,7 @@ const char* const kKernelNames[kNumKernelIds] = {
     "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
     "scan_counts",    "prefix_chunks", "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
-    "final_count",    "final_positions", "final_write", "merge_insert"};
+    "final_count",    "final_positions", "final_write", "merge_insert",      "sketch"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -134,6 +134,7 @@ struct svdag_context {
     uint32_t load_shift = 1;
     int radix_bits = 9;
     int sort_ipt = kSortIptSmall;
+    bool table_estimate = true;
     int host_threads = 1;
     // Stage markers of the work in flight (bounded completion wait).
     static constexpr int kMaxMarks = 64;
@@ -245,7 +246,7 @@ inline uint32_t table_lg(uint64_t count, uint32_t shift)
 struct BuildLayout {
     uint64_t total = 0;
     uint64_t zero_begin = 0, zero_end = 0;
-    uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0;
+    uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0, sketch = 0;
     uint64_t sort_counts = 0, tile_counts = 0, tile_offsets = 0;
     uint64_t gkeys = 0, keys_a = 0, keys_b = 0;
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
@@ -276,6 +277,7 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.counters = c.take(sizeof(BuildCounters));
     L.key_sort_zero = c.take(4 * sort_zero_words(L.key_plan));
     L.leaf_sort_zero = c.take(4 * sort_zero_words(L.leaf_plan));
+    L.sketch = c.take(4 * kSketchRegs);
     L.zero_end = c.off;
     L.sort_counts = c.take(4 * sort_count_words(n));
     L.tile_counts = c.take(4 * stage_tiles);
@@ -412,8 +414,13 @@ struct BatchInfo {
     int chunk_depth;
 };
 
+// Returned by build() when a table sized from an estimate overflowed; the
+// caller repeats the build with bound-sized tables.
+constexpr int kRetryWithBounds = -100;
+
 template <typename K>
-int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag* out, const BatchInfo* batch)
+int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag* out, const BatchInfo* batch,
+          bool estimate)
 {
     hipStream_t st = ctx->stream;
     const int levels = depth - 2;
@@ -494,7 +501,16 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + L.list_slots[0]);
     uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + L.list_slots[1]);
     uint32_t* sorted_slots = reinterpret_cast<uint32_t*>(A + L.sorted_slots);
-    launch_table_clear(st, leaf_table, &c->leaf_positions, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
+    // The leaf table is sized from a distinct-count estimate of the masks (or,
+    // on a retry, from the leaf positions, an upper bound).
+    const uint64_t* leaf_size = &c->leaf_positions;
+    if (estimate) {
+        uint32_t* regs = reinterpret_cast<uint32_t*>(A + L.sketch);
+        launch_sketch(st, lp_mask, &c->leaf_positions, regs, L.leaf_bound);
+        launch_sketch_estimate(st, regs, &c->leaf_positions, &c->leaf_estimate);
+        leaf_size = &c->leaf_estimate;
+    }
+    launch_table_clear(st, leaf_table, leaf_size, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
     launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, lk0, ls0,
                        &c->unique_leaves, err);
     uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
@@ -575,6 +591,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         return fail(set_error(SVDAG_ERR_ARGUMENT, "a key is >= 2^(3*depth) for depth %d", depth));
     if (h.error & kErrKeyRange)
         return fail(set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)"));
+    if ((h.error & kErrTableOverflow) && estimate) return fail(kRetryWithBounds);
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
 
@@ -894,6 +911,7 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         if (value < 4 || value > kMaxRadixBits) return set_error(SVDAG_ERR_ARGUMENT, "radix bits must be 4..%d", kMaxRadixBits);
         ctx->radix_bits = int(value);
         return 0;
+    case SVDAG_OPT_TABLE_ESTIMATE: ctx->table_estimate = value != 0; return 0;
     case SVDAG_OPT_SORT_TILE:
         if (value != 4096 && value != 8192) return set_error(SVDAG_ERR_ARGUMENT, "sort tile must be 4096 or 8192 keys");
         ctx->sort_ipt = value == 8192 ? kSortIptLarge : kSortIptSmall;
@@ -916,7 +934,8 @@ int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int
     if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
     if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
     if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
-    return build<uint64_t>(ctx, d_keys, n, depth, out, nullptr);
+    const int rc = build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, ctx->table_estimate);
+    return rc == kRetryWithBounds ? build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, false) : rc;
 }
 
 int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)
@@ -926,7 +945,8 @@ int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n,
     if (depth < 3 || depth > 10) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 10] for 32-bit keys", depth);
     if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
     if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
-    return build<uint32_t>(ctx, d_keys, n, depth, out, nullptr);
+    const int rc = build<uint32_t>(ctx, d_keys, n, depth, out, nullptr, ctx->table_estimate);
+    return rc == kRetryWithBounds ? build<uint32_t>(ctx, d_keys, n, depth, out, nullptr, false) : rc;
 }
 
 int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys, const uint64_t* offsets, uint64_t num_chunks,
@@ -949,7 +969,8 @@ int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys, const u
     if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
     if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
     BatchInfo b{offsets, num_chunks, chunk_depth};
-    return build<uint64_t>(ctx, d_keys, n, depth, out, &b);
+    const int rc = build<uint64_t>(ctx, d_keys, n, depth, out, &b, ctx->table_estimate);
+    return rc == kRetryWithBounds ? build<uint64_t>(ctx, d_keys, n, depth, out, &b, false) : rc;
 }
 
 int svdag_merge_dags(svdag_context* ctx, const svdag_dag* dags, uint64_t count, svdag_dag* out)

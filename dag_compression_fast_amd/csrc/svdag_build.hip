@@ -325,6 +325,75 @@ __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Distinct-count estimate of the leaf masks (HyperLogLog: Flajolet, Fusy,
+// Gandouet, Meunier 2007), so the leaf table is sized by unique leaves rather
+// than by leaf positions and its slot arrays stay cache-sized.  Only the table
+// size depends on it; the result does not.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_sketch(const uint64_t* __restrict__ values,
+                                                     const uint64_t* __restrict__ count_ptr,
+                                                     uint32_t* __restrict__ regs)
+{
+    __shared__ uint32_t s_reg[kSketchRegs];
+    for (uint32_t i = threadIdx.x; i < uint32_t(kSketchRegs); i += kThreads) s_reg[i] = 0;
+    __syncthreads();
+    const uint64_t n = *count_ptr;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
+        // A mixer unrelated to the table's home-slot hash.
+        const uint64_t h = fmix64(values[q] * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+        const uint32_t idx = uint32_t(h >> (64 - kSketchBits));
+        const uint64_t rest = h << kSketchBits;
+        const uint32_t rho = rest ? uint32_t(__clzll((long long)rest)) + 1u : uint32_t(64 - kSketchBits + 1);
+        atomicMax(&s_reg[idx], rho);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < uint32_t(kSketchRegs); i += kThreads)
+        if (s_reg[i]) atomicMax(&regs[i], s_reg[i]);
+}
+
+constexpr int kEstimateThreads = 1024;
+
+__global__ __launch_bounds__(kEstimateThreads) void k_sketch_estimate(const uint32_t* __restrict__ regs,
+                                                                      const uint64_t* __restrict__ bound_ptr,
+                                                                      uint64_t* __restrict__ estimate)
+{
+    __shared__ double s_sum[kEstimateThreads / kWave];
+    __shared__ uint32_t s_zero[kEstimateThreads / kWave];
+    double sum = 0.0;
+    uint32_t zeros = 0;
+    for (uint32_t i = threadIdx.x; i < uint32_t(kSketchRegs); i += kEstimateThreads) {
+        sum += ldexp(1.0, -int(regs[i]));
+        zeros += regs[i] == 0 ? 1u : 0u;
+    }
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1) {
+        sum += __shfl_xor(sum, d, kWave);
+        zeros += __shfl_xor(zeros, d, kWave);
+    }
+    const uint32_t w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        s_sum[w] = sum;
+        s_zero[w] = zeros;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        uint32_t z = 0;
+        for (int i = 0; i < kEstimateThreads / kWave; ++i) {
+            total += s_sum[i];
+            z += s_zero[i];
+        }
+        const double m = double(kSketchRegs);
+        double e = 0.7213 / (1.0 + 1.079 / m) * m * m / total;
+        if (e <= 2.5 * m && z) e = m * log(m / double(z));  // small-range (linear counting) correction
+        const uint64_t want = uint64_t(e * 1.25) + 16;
+        const uint64_t bound = *bound_ptr;
+        *estimate = want < bound ? want : bound;
+    }
+}
+
 __global__ void k_rank_of_slot(const uint32_t* __restrict__ sorted_slots, const uint64_t* __restrict__ count_ptr,
                                uint32_t* __restrict__ rank_of_slot)
 {
@@ -871,6 +940,20 @@ void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* 
     hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads * kInsertIpt - 1) / (kThreads * kInsertIpt), 8)),
                        dim3(kThreads), 0, s,
                        lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, list_keys, list_slots, list_count, err);
+}
+
+void launch_sketch(hipStream_t s, const uint64_t* values, const uint64_t* count_ptr, uint32_t* regs, uint64_t n_bound)
+{
+    TimedLaunch t(kKSketch, s);
+    // Few workgroups: each merges its registers into the global ones once.
+    hipLaunchKernelGGL(k_sketch, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 1)), dim3(kThreads), 0, s, values,
+                       count_ptr, regs);
+}
+
+void launch_sketch_estimate(hipStream_t s, const uint32_t* regs, const uint64_t* bound_ptr, uint64_t* estimate)
+{
+    TimedLaunch t(kKSketch, s);
+    hipLaunchKernelGGL(k_sketch_estimate, dim3(1), dim3(kEstimateThreads), 0, s, regs, bound_ptr, estimate);
 }
 
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,

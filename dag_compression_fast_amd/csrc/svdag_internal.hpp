@@ -39,6 +39,7 @@ enum KernelId : int {
     kKFinalPositions,
     kKFinalWrite,
     kKMergeInsert,
+    kKSketch,
     kNumKernelIds
 };
 
@@ -127,6 +128,7 @@ struct BuildCounters {
     uint64_t error;                           // kErr* bits
     uint32_t table_log2[SVDAG_MAX_LEVELS + 1];  // [l] level tables, [MAX] leaf table
     uint64_t table_slots;                     // slots of the table being extracted
+    uint64_t leaf_estimate;                   // table sizing count of the leaf dedup table
 };
 
 constexpr uint64_t kErrTableOverflow = 1;
@@ -168,6 +170,14 @@ void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr
 void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
                         const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
                         uint32_t* list_slots, uint64_t* list_count, unsigned long long* err);
+
+// Distinct-count sketch (HyperLogLog, kSketchRegs registers, zero before the
+// first launch) of values[0, *count_ptr), and the count a hashed table is sized
+// from: 1.25 x the estimate, at most *bound_ptr.
+constexpr int kSketchBits = 12;
+constexpr int kSketchRegs = 1 << kSketchBits;
+void launch_sketch(hipStream_t s, const uint64_t* values, const uint64_t* count_ptr, uint32_t* regs, uint64_t n_bound);
+void launch_sketch_estimate(hipStream_t s, const uint32_t* regs, const uint64_t* bound_ptr, uint64_t* estimate);
 
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
                          uint32_t* rank_of_slot, uint64_t n_bound);

@@ -83,7 +83,11 @@ typedef enum svdag_option {
     SVDAG_OPT_WAIT_SECONDS = 5,       /* completion deadline of a call (default 120 s); past it
                                          the call fails and names the stage that stalled */
     SVDAG_OPT_RADIX_BITS = 6,         /* widest digit of the key sort, 4..9 (default 9)      */
-    SVDAG_OPT_SORT_TILE = 7           /* keys per key-sort tile: 4096 or 8192 (default 4096) */
+    SVDAG_OPT_SORT_TILE = 7,          /* keys per key-sort tile: 4096 or 8192 (default 4096) */
+    SVDAG_OPT_TABLE_ESTIMATE = 8      /* 1: size the leaf dedup table from a distinct-count
+                                         estimate of the leaf masks (default); 0: from the leaf
+                                         positions.  An estimate that proves too small makes the
+                                         call rebuild with the bound-sized table. */
 } svdag_option;
 
 /* --- context ------------------------------------------------------------ */

@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <functional>
 
@@ -112,6 +113,14 @@ template <class T, class U>
 T atomicAdd(T* p, U v) { return __atomic_fetch_add(p, T(v), __ATOMIC_SEQ_CST); }
 template <class T, class U>
 T atomicOr(T* p, U v) { return __atomic_fetch_or(p, T(v), __ATOMIC_SEQ_CST); }
+template <class T, class U>
+T atomicMax(T* p, U v)
+{
+    T old = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (old < T(v) && !__atomic_compare_exchange_n(p, &old, T(v), false, __ATOMIC_SEQ_CST, __ATOMIC_RELAXED)) {
+    }
+    return old;
+}
 template <class T, class U, class V>
 T atomicCAS(T* p, U cmp, V val)
 {
