@@ -31,7 +31,7 @@ const char* const kKernelNames[kNumKernelIds] = {
     "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
     "scan_counts",    "prefix_chunks", "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
-    "final_count",    "final_positions", "final_write", "merge_insert",      "sketch"};
+    "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -521,10 +521,14 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
     mark(ctx, "leaf dedup");
 
-    // Levels, bottom-up (DAGCompression.cu:1077-1092).
-    launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[0]), &c->pre[depth - 3], &c->table_log2[depth - 3],
-                       ctx->load_shift, kTableOverflow, L.level_slots);
-    for (int l = depth - 3; l >= 0; --l) {
+    // Levels, bottom-up (DAGCompression.cu:1077-1092): hashed-table levels down
+    // to tail_top + 1, then levels tail_top..0 (at most 4096 nodes each) in one
+    // workgroup.
+    const int tail_top = std::min(depth - 3, kTailTopLevel);
+    if (depth - 3 > tail_top)
+        launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[0]), &c->pre[depth - 3],
+                           &c->table_log2[depth - 3], ctx->load_shift, kTableOverflow, L.level_slots);
+    for (int l = depth - 3; l > tail_top; --l) {
         const int step = depth - 3 - l;
         const int cur = step & 1;
         LevelBuildArgs a{};
@@ -574,9 +578,43 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.w_count = &c->words[l];
         launch_level_extract(st, e, L.level_slots);
         mark(ctx, "level extract");
-        if (l > 0)
+        if (l - 1 > tail_top)
             launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[cur ^ 1]), &c->pre[l - 1],
                                &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, L.level_slots);
+    }
+    {
+        TailArgs t{};
+        t.top = tail_top;
+        if (tail_top == depth - 3) {
+            t.child_codes = lp_code;
+            t.child_slots = lp_slot;
+            t.child_rank_of_slot = leaf_rank;
+            t.child_hashes = leaves_out;
+            t.child_count = &c->leaf_positions;
+        } else {
+            const int cur = (depth - 3 - (tail_top + 1)) & 1;  // buffers of level tail_top + 1
+            t.child_codes = reinterpret_cast<const uint64_t*>(A + L.codes[cur]);
+            t.child_slots = reinterpret_cast<const uint32_t*>(A + L.slots[cur]);
+            t.child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur]);
+            t.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[tail_top + 1]);
+            t.child_count = &c->pre[tail_top + 1];
+        }
+        // The level tables' key and representative arrays are free by now.
+        for (int i = 0; i < 2; ++i) {
+            t.codes[i] = reinterpret_cast<uint64_t*>(A + L.tkeys[i]);
+            t.ranks[i] = reinterpret_cast<uint32_t*>(A + L.trep[i]);
+        }
+        t.masks = reinterpret_cast<uint8_t*>(A + L.masks);
+        t.children = reinterpret_cast<uint32_t*>(A + L.children);
+        for (int l = 0; l <= tail_top; ++l) {
+            t.out_hashes[l] = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+            t.out_masks[l] = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+            t.out_children[l] = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        }
+        t.counters = c;
+        t.check_collisions = ctx->check_collisions;
+        launch_level_tail(st, t);
+        mark(ctx, "top levels");
     }
     {
         const hipError_t e = hipGetLastError();

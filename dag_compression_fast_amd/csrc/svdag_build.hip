@@ -255,7 +255,7 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
     const uint64_t wrap = (1ull << lg) - 1;
     uint64_t slot = fmix64(h) >> (64 - lg);
     for (uint64_t probe = 0; probe <= wrap; ++probe, slot = (slot + 1) & wrap) {
-        const uint64_t cur = atomic_load_relaxed(&tk[slot]);
+        const uint64_t cur = probe_load(&tk[slot]);
         if (cur == h) return uint32_t(slot);
         if (cur == 0) {
             const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
@@ -410,7 +410,7 @@ __device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, ui
 {
     uint64_t slot = h >> (64 - lg);
     for (; slot < limit; ++slot) {
-        const uint64_t cur = atomic_load_relaxed(&tk[slot]);
+        const uint64_t cur = probe_load(&tk[slot]);
         if (cur == h) return uint32_t(slot);
         if (cur == 0) {
             const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
@@ -581,6 +581,162 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Small top levels in one workgroup.  Per level (top down to 0): parents are
+// the runs of equal code >> 3 among the children; one thread per parent reads
+// its (at most 8) children and forms its mask, child ranks and hash; the
+// (hash, parent) pairs are sorted in LDS (bitonic), equal hashes get one rank,
+// and the level is written in ascending hash order.  The parents' ranks are
+// the next level's child ranks.
+// ---------------------------------------------------------------------------
+constexpr int kTailThreads = 1024;
+
+__global__ __launch_bounds__(kTailThreads) void k_level_tail(TailArgs a)
+{
+    __shared__ uint64_t s_key[kTailMaxParents];
+    __shared__ uint16_t s_idx[kTailMaxParents];
+    __shared__ uint32_t s_start[kTailMaxParents + 1];
+    __shared__ uint32_t s_scan[kTailThreads / kWave];
+    __shared__ uint64_t s_red[kTailThreads / kWave];
+    const uint32_t tid = threadIdx.x;
+    BuildCounters* c = a.counters;
+    uint32_t C = uint32_t(*a.child_count);
+    int cur = 0;
+    for (int l = a.top; l >= 0; --l, cur ^= 1) {
+        const bool first = l == a.top;
+        const uint64_t* ccodes = first ? a.child_codes : a.codes[cur ^ 1];
+        const uint32_t* cranks = first ? nullptr : a.ranks[cur ^ 1];
+        const uint64_t* chashes = first ? a.child_hashes : a.out_hashes[l + 1];
+        // Child rank and hash of child i.
+        auto child_rank = [&](uint32_t i) -> uint32_t {
+            return first ? a.child_rank_of_slot[a.child_slots[i]] : cranks[i];
+        };
+
+        // 1. Run heads (parents) in child order.
+        uint32_t M = 0;
+        for (uint32_t base = 0; base < C; base += kTailThreads) {
+            const uint32_t i = base + tid;
+            bool head = false;
+            if (i < C) head = i == 0 || (ccodes[i] >> 3) != (ccodes[i - 1] >> 3);
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<kTailThreads>(head ? 1u : 0u, s_scan, total);
+            if (head) s_start[M + ex] = i;
+            M += total;
+        }
+        if (tid == 0) s_start[M] = C;
+        __syncthreads();
+
+        // 2. One thread per parent.
+        uint64_t* pcodes = a.codes[cur];
+        for (uint32_t p = tid; p < M; p += kTailThreads) {
+            const uint32_t b = s_start[p], e = s_start[p + 1];
+            uint32_t mask = 0;
+            uint32_t ch[8];
+            uint64_t hs[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ch[k] = kEmptyChild;
+                hs[k] = 0;
+            }
+            for (uint32_t i = b; i < e; ++i) {
+                const uint64_t code = ccodes[i];
+                const uint32_t slot = uint32_t(code & 7);
+                const uint32_t r = child_rank(i);
+                mask |= 1u << slot;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    ch[k] = slot == uint32_t(k) ? r : ch[k];
+                    hs[k] = slot == uint32_t(k) ? chashes[r] : hs[k];
+                }
+            }
+            pcodes[p] = ccodes[b] >> 3;
+            a.masks[p] = uint8_t(mask);
+            uint4* dst = reinterpret_cast<uint4*>(a.children + 8 * uint64_t(p));
+            dst[0] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
+            dst[1] = make_uint4(ch[4], ch[5], ch[6], ch[7]);
+            s_key[p] = hash_children(hs);
+            s_idx[p] = uint16_t(p);
+        }
+        // Pad to a power of two; (max key, index 0xFFFF) sorts after every parent.
+        uint32_t n2 = 1;
+        while (n2 < M) n2 <<= 1;
+        for (uint32_t p = M + tid; p < n2; p += kTailThreads) {
+            s_key[p] = ~0ull;
+            s_idx[p] = 0xFFFF;
+        }
+        __syncthreads();
+
+        // 3. Bitonic sort of (key, index), ascending.
+        for (uint32_t size = 2; size <= n2; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t t = tid; t < n2 / 2; t += kTailThreads) {
+                    const uint32_t lo = 2 * t - (t & (stride - 1));
+                    const uint32_t hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const uint64_t ka = s_key[lo], kb = s_key[hi];
+                    const uint16_t ia = s_idx[lo], ib = s_idx[hi];
+                    const bool greater = ka > kb || (ka == kb && ia > ib);
+                    if (greater == up) {
+                        s_key[lo] = kb;
+                        s_key[hi] = ka;
+                        s_idx[lo] = ib;
+                        s_idx[hi] = ia;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // 4. Unique ranks; the first of each equal-hash run represents it.
+        uint32_t* pranks = a.ranks[cur];
+        uint32_t Q = 0;
+        uint64_t words = 0, bad = 0;
+        for (uint32_t base = 0; base < M; base += kTailThreads) {
+            const uint32_t q = base + tid;
+            bool lead = false;
+            if (q < M) lead = q == 0 || s_key[q] != s_key[q - 1];
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<kTailThreads>(lead ? 1u : 0u, s_scan, total);
+            if (q < M) {
+                const uint32_t r = Q + ex + (lead ? 1u : 0u) - 1u;
+                const uint32_t p = s_idx[q];
+                pranks[p] = r;
+                if (lead) {
+                    const uint32_t m = a.masks[p];
+                    a.out_hashes[l][r] = s_key[q];
+                    a.out_masks[l][r] = uint8_t(m);
+                    const uint4* src = reinterpret_cast<const uint4*>(a.children + 8 * uint64_t(p));
+                    uint4* dst = reinterpret_cast<uint4*>(a.out_children[l] + 8 * uint64_t(r));
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                    words += 1 + __popc(m);
+                } else if (a.check_collisions) {
+                    // Compare with the run's representative (content must be equal).
+                    uint32_t f = q;
+                    while (f > 0 && s_key[f - 1] == s_key[q]) --f;
+                    const uint32_t rp = s_idx[f];
+                    bool diff = a.masks[p] != a.masks[rp];
+                    for (int k = 0; k < 8; ++k) diff |= a.children[8 * p + k] != a.children[8 * rp + k];
+                    bad += diff ? 1 : 0;
+                }
+            }
+            Q += total;
+        }
+        words = block_reduce_sum<kTailThreads>(words, s_red);
+        bad = block_reduce_sum<kTailThreads>(bad, s_red);
+        if (tid == 0) {
+            c->pre[l] = M;
+            c->uniq[l] = Q;
+            c->words[l] = words;
+            if (bad) c->collisions += bad;
+        }
+        // The next level reads this level's codes, ranks and hashes (the
+        // barrier orders the global writes within the workgroup).
+        __syncthreads();
+        C = M;
+    }
+}
+
 // Exact-content check of every parent against its table representative.
 __global__ void k_level_check(const uint32_t* __restrict__ parent_slots, const uint32_t* __restrict__ table_rep,
                               const uint8_t* __restrict__ masks, const uint32_t* __restrict__ children,
@@ -605,9 +761,9 @@ __global__ void k_level_check(const uint32_t* __restrict__ parent_slots, const u
 // Slots are read through an LDS window with a halo; clusters reaching past it
 // fall back to global loads.
 // ---------------------------------------------------------------------------
-constexpr int kExtractIpt = 8;
+constexpr int kExtractIpt = 2;
 constexpr int kExtractTile = kThreads * kExtractIpt;
-constexpr int kHalo = 256;
+constexpr int kHalo = 128;
 
 __global__ __launch_bounds__(kThreads) void k_extract_count(const uint64_t* __restrict__ table_keys,
                                                             const uint32_t* __restrict__ table_log2,
@@ -977,6 +1133,12 @@ void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_b
         TimedLaunch t(kKLevelBuild, s);
         hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
     }
+}
+
+void launch_level_tail(hipStream_t s, const TailArgs& a)
+{
+    TimedLaunch t(kKLevelTail, s);
+    hipLaunchKernelGGL(k_level_tail, dim3(1), dim3(kTailThreads), 0, s, a);
 }
 
 void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound)

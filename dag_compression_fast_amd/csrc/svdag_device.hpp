@@ -20,6 +20,13 @@ __device__ __forceinline__ uint32_t atomic_load_relaxed(const uint32_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Probe read of a dedup-table slot.  Slots only ever change from 0 to their
+// final key, so a value read through the CU's L1 is either final or a stale 0,
+// which the caller settles with a CAS; hot slots then stay in L1.
+__device__ __forceinline__ uint64_t probe_load(const uint64_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ __forceinline__ void atomic_store_relaxed(uint64_t* p, uint64_t v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

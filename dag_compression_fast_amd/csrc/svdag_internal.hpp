@@ -40,6 +40,7 @@ enum KernelId : int {
     kKFinalWrite,
     kKMergeInsert,
     kKSketch,
+    kKLevelTail,
     kNumKernelIds
 };
 
@@ -138,7 +139,7 @@ constexpr uint64_t kErrKeyRange = 2;
 // tiles(n, kXxxTile) u32 counts and u64 offsets of scratch.
 constexpr uint64_t kPackTileItems = 4096;     // sorted keys per leaf-pack tile
 constexpr uint64_t kBuildTileItems = 2048;    // child codes per level-build tile
-constexpr uint64_t kExtractTileItems = 2048;  // table slots per extract tile
+constexpr uint64_t kExtractTileItems = 512;   // table slots per extract tile
 constexpr uint64_t kFinalTileItems = 4096;    // nodes per final-layout tile
 inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
 
@@ -201,6 +202,37 @@ struct LevelBuildArgs {
     BuildCounters* counters;
 };
 void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound);
+
+// ---- small top levels -------------------------------------------------------
+// Levels 0..top with top <= kTailTopLevel hold at most 8^top <= 4096 nodes; one
+// workgroup builds, dedups (LDS sort by hash) and writes all of them.
+constexpr int kTailTopLevel = 4;
+constexpr uint32_t kTailMaxParents = 1u << (3 * kTailTopLevel);      // 4096
+constexpr uint32_t kTailMaxChildren = 1u << (3 * (kTailTopLevel + 1));  // 32768
+
+struct TailArgs {
+    int top;                              // highest level built here
+    // Children of level `top`: codes (sorted), table slots and the child level's
+    // slot -> rank map, hashes by rank, and their count.
+    const uint64_t* child_codes;
+    const uint32_t* child_slots;
+    const uint32_t* child_rank_of_slot;
+    const uint64_t* child_hashes;
+    const uint64_t* child_count;
+    // Scratch, each at least kTailMaxParents entries (two ping-pong halves for
+    // codes and ranks).
+    uint64_t* codes[2];
+    uint32_t* ranks[2];
+    uint8_t* masks;
+    uint32_t* children;                   // 8 per parent
+    // Output levels 0..top.
+    uint64_t* out_hashes[kTailTopLevel + 1];
+    uint8_t* out_masks[kTailTopLevel + 1];
+    uint32_t* out_children[kTailTopLevel + 1];
+    BuildCounters* counters;
+    int check_collisions;
+};
+void launch_level_tail(hipStream_t s, const TailArgs& a);
 
 struct LevelExtractArgs {
     const uint64_t* table_keys;

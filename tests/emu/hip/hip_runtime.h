@@ -99,6 +99,9 @@ T __shfl_xor(T v, int m, int = 64)
     return out;
 }
 
+#ifndef __HIP_MEMORY_SCOPE_WORKGROUP
+#define __HIP_MEMORY_SCOPE_WORKGROUP 2
+#endif
 #ifndef __HIP_MEMORY_SCOPE_AGENT
 #define __HIP_MEMORY_SCOPE_AGENT 3
 #endif
