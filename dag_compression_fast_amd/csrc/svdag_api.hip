@@ -251,7 +251,7 @@ struct BuildLayout {
     uint64_t gkeys = 0, keys_a = 0, keys_b = 0;
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
-    uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0;
+    uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0, tail_hash = 0;
     uint64_t tkeys[2] = {}, trep[2] = {}, trank[2] = {};
     uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
     SortPlan key_plan{}, leaf_plan{};
@@ -302,6 +302,7 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     }
     L.masks = c.take(L.level_bound);
     L.children = c.take(32 * L.level_bound);
+    L.tail_hash = c.take(8 * 8 * std::min<uint64_t>(L.level_bound, kTailMaxParents));
     for (int i = 0; i < 2; ++i) {
         L.tkeys[i] = c.take(8 * L.level_slots);
         L.trep[i] = c.take(4 * L.level_slots);
@@ -606,6 +607,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         }
         t.masks = reinterpret_cast<uint8_t*>(A + L.masks);
         t.children = reinterpret_cast<uint32_t*>(A + L.children);
+        t.child_hash = reinterpret_cast<uint64_t*>(A + L.tail_hash);
         for (int l = 0; l <= tail_top; ++l) {
             t.out_hashes[l] = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
             t.out_masks[l] = reinterpret_cast<uint8_t*>(O + S.masks[l]);

@@ -626,34 +626,43 @@ __global__ __launch_bounds__(kTailThreads) void k_level_tail(TailArgs a)
         if (tid == 0) s_start[M] = C;
         __syncthreads();
 
-        // 2. One thread per parent.
+        // 2. Children scatter their rank and hash into their parent's slot (the
+        //    gathers of all children are independent), then one thread per
+        //    parent reads its 8 slots and forms mask, record and hash.
         uint64_t* pcodes = a.codes[cur];
+        for (uint32_t k = tid; k < 8 * M; k += kTailThreads) {
+            a.children[k] = kEmptyChild;
+            a.child_hash[k] = 0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t i = tid; i < C; i += kTailThreads) {
+            // Parent of child i: the last run start <= i.
+            uint32_t lo = 0, hi = M;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (s_start[mid] <= i) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t slot = uint32_t(ccodes[i] & 7);
+            const uint32_t r = child_rank(i);
+            a.children[8 * lo + slot] = r;
+            a.child_hash[8 * lo + slot] = chashes[r];
+        }
+        __syncthreads();
         for (uint32_t p = tid; p < M; p += kTailThreads) {
-            const uint32_t b = s_start[p], e = s_start[p + 1];
+            const uint4* cs = reinterpret_cast<const uint4*>(a.children + 8 * uint64_t(p));
+            const uint4 c0 = cs[0], c1 = cs[1];
+            const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
             uint32_t mask = 0;
-            uint32_t ch[8];
             uint64_t hs[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                ch[k] = kEmptyChild;
-                hs[k] = 0;
+                mask |= ch[k] != kEmptyChild ? 1u << k : 0u;
+                hs[k] = a.child_hash[8 * uint64_t(p) + k];
             }
-            for (uint32_t i = b; i < e; ++i) {
-                const uint64_t code = ccodes[i];
-                const uint32_t slot = uint32_t(code & 7);
-                const uint32_t r = child_rank(i);
-                mask |= 1u << slot;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    ch[k] = slot == uint32_t(k) ? r : ch[k];
-                    hs[k] = slot == uint32_t(k) ? chashes[r] : hs[k];
-                }
-            }
-            pcodes[p] = ccodes[b] >> 3;
+            pcodes[p] = ccodes[s_start[p]] >> 3;
             a.masks[p] = uint8_t(mask);
-            uint4* dst = reinterpret_cast<uint4*>(a.children + 8 * uint64_t(p));
-            dst[0] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
-            dst[1] = make_uint4(ch[4], ch[5], ch[6], ch[7]);
             s_key[p] = hash_children(hs);
             s_idx[p] = uint16_t(p);
         }

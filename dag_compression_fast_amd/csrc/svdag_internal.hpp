@@ -225,6 +225,7 @@ struct TailArgs {
     uint32_t* ranks[2];
     uint8_t* masks;
     uint32_t* children;                   // 8 per parent
+    uint64_t* child_hash;                 // 8 per parent
     // Output levels 0..top.
     uint64_t* out_hashes[kTailTopLevel + 1];
     uint8_t* out_masks[kTailTopLevel + 1];
