@@ -115,8 +115,13 @@ __device__ __forceinline__ void wave_heads(const K* __restrict__ sorted, uint64_
                                            uint64_t (&heads)[kPackGroups], uint32_t& gheads)
 {
     const uint32_t lane = lane_id();
+    if (chunk + kPackWaveKeys <= n) {
 #pragma unroll
-    for (int g = 0; g < kPackGroups; ++g) k[g] = load_key(sorted, chunk + uint64_t(g) * kWave + lane, n, code_mask);
+        for (int g = 0; g < kPackGroups; ++g) k[g] = uint64_t(sorted[chunk + uint64_t(g) * kWave + lane]) & code_mask;
+    } else {
+#pragma unroll
+        for (int g = 0; g < kPackGroups; ++g) k[g] = load_key(sorted, chunk + uint64_t(g) * kWave + lane, n, code_mask);
+    }
     uint64_t carry = (chunk > 0 && chunk <= n) ? (uint64_t(sorted[chunk - 1]) & code_mask) : ~0ull;
     gheads = 0;
 #pragma unroll

@@ -204,11 +204,11 @@ struct LevelBuildArgs {
 void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound);
 
 // ---- small top levels -------------------------------------------------------
-// Levels 0..top with top <= kTailTopLevel hold at most 8^top <= 4096 nodes; one
+// Levels 0..top with top <= kTailTopLevel hold at most 8^top <= 512 nodes; one
 // workgroup builds, dedups (LDS sort by hash) and writes all of them.
-constexpr int kTailTopLevel = 4;
-constexpr uint32_t kTailMaxParents = 1u << (3 * kTailTopLevel);      // 4096
-constexpr uint32_t kTailMaxChildren = 1u << (3 * (kTailTopLevel + 1));  // 32768
+constexpr int kTailTopLevel = 3;
+constexpr uint32_t kTailMaxParents = 1u << (3 * kTailTopLevel);
+constexpr uint32_t kTailMaxChildren = 1u << (3 * (kTailTopLevel + 1));
 
 struct TailArgs {
     int top;                              // highest level built here
