@@ -9,7 +9,8 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libsvdag.so")
+# SVDAG_LIBRARY selects another build of the library (default: the in-tree one).
+LIB_PATH = os.environ.get("SVDAG_LIBRARY") or os.path.join(HERE, "_build", "libsvdag.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "svdag.h")
 
 MAX_LEVELS = 24

@@ -16,6 +16,7 @@
 #include "svdag_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace svdag {
 
@@ -210,11 +211,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     // The staging area holds the tile in digit order; during ranking it holds
     // the per-wave match table s_match[W][kRadix] instead.
     static_assert(W * kRadix * sizeof(uint64_t) <= kTile * sizeof(K), "match table fits the staging area");
+    // Per-wave digit counts and tile-local digit starts never exceed the tile
+    // size, so 16 bits hold them (keeps the LDS under 40 KB: 4 workgroups per CU).
+    static_assert(kTile <= 65535, "16-bit tile-local counters");
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[PAIRS ? kTile : 1];
-    __shared__ uint32_t s_wcnt[W][kRadix];
+    __shared__ uint16_t s_wcnt[W][kRadix];
     uint64_t* const s_match = reinterpret_cast<uint64_t*>(s_keys) + (threadIdx.x / kWave) * kRadix;
-    __shared__ uint32_t s_lstart[kRadix];
+    __shared__ uint16_t s_lstart[kRadix];
     __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_scan[W];
 
@@ -262,13 +266,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         const uint64_t tile_base = tile * kTile;
         const uint32_t tile_n = uint32_t(min(uint64_t(kTile), n - tile_base));
 
+        // Ranking, digit starts and staging; FULL (every key of the tile present,
+        // the common case) drops the per-key bounds checks and their registers.
+        auto rank_and_stage = [&](auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
         // Per key: (rank among the wave's keys of its digit) << 16 | digit.
         uint32_t dr[IPT];
-        uint32_t* wc = s_wcnt[w];
+        uint16_t* wc = s_wcnt[w];
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
-            const uint32_t idx = wave_base + i * kWave + lane;
-            const bool valid = idx < tile_n;
+            const bool valid = FULL || wave_base + i * kWave + lane < tile_n;
             const uint32_t d = uint32_t(key[i] >> shift) & digit_mask;
             // Lanes holding the same digit: every lane ORs its bit into the wave's
             // word for its digit (the OR does not depend on the order of the
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
             // (the leader) writes it back.
             __builtin_amdgcn_wave_barrier();
             dr[i] = (c + below) << 16 | d;
-            if (valid && (match >> lane) == 1ull) wc[d] = c + uint32_t(__popcll(match));
+            if (valid && (match >> lane) == 1ull) wc[d] = uint16_t(c + uint32_t(__popcll(match)));
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
 #pragma unroll
                 for (int i = 0; i < W; ++i) {
                     const uint32_t c = s_wcnt[i][d];
-                    s_wcnt[i][d] = total;
+                    s_wcnt[i][d] = uint16_t(total);
                     total += c;
                 }
             }
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         for (int k = 0; k < DPT; ++k) {
             const uint32_t d = tid * DPT + k;
             if (d < radix) {
-                s_lstart[d] = lstart;
+                s_lstart[d] = uint16_t(lstart);
                 s_base[d] = offsets[uint64_t(d) * stride + tile] - lstart;
             }
             lstart += dtot[k];
@@ -325,14 +332,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
 
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
-            const uint32_t idx = wave_base + i * kWave + lane;
-            if (idx < tile_n) {
+            if (FULL || wave_base + i * kWave + lane < tile_n) {
                 const uint32_t d = dr[i] & 0xFFFFu;
                 const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + (dr[i] >> 16);
                 s_keys[pos] = key[i];
                 if constexpr (PAIRS) s_vals[pos] = val[i];
             }
         }
+        };
+        if (tile_n == kTile) rank_and_stage(std::true_type{});
+        else rank_and_stage(std::false_type{});
         __syncthreads();
         // Next tile's loads go out before this tile's stores.
         load_tile(tile + walk.step);
