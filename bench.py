@@ -87,6 +87,118 @@ def read_traffic(kernel, workload):
     return float(m.group(1)) if m else None
 
 
+CHUNKS = dict(desc="512 sub-volumes of 512^3 (chunk depth 9 -> depth 12), lognormal keys per chunk "
+                   "(median 2e5, sigma 1, 1e3..4e6), sphere-shell or triangle-surface content, 20-30% duplicates",
+              seed=0x5EED0003, num=512, depth=9)
+
+
+def chunk_workload(svd, threads):
+    """The 512-chunk workload: host arrays per chunk (chunk-local keys)."""
+    from concurrent.futures import ThreadPoolExecutor
+    n, _, _ = svd.gen_chunk_plan(CHUNKS["seed"], CHUNKS["num"])
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda c: svd.gen_chunk(CHUNKS["seed"], c, CHUNKS["depth"], capacity=int(n[c])),
+                           range(CHUNKS["num"])))
+
+
+def final_words(L, _lib, ctx, dag):
+    """Flat layout of a device DAG, copied to the host."""
+    words, nw = ctypes.c_void_p(), ctypes.c_uint64()
+    _lib.check(L.svdag_create_final_dag(ctx.handle, ctypes.byref(dag), ctypes.byref(words), ctypes.byref(nw)))
+    out = np.empty(int(nw.value), dtype=np.uint32)
+    _lib.hip_check(_lib.hip().hipMemcpy(out.ctypes.data, words, out.nbytes, _lib.HIP_D2H), "hipMemcpy")
+    L.svdag_free_words(ctx.handle, words)
+    return out
+
+
+def bench_chunks(svd, _lib, args):
+    """Plain (CreateDAG per chunk + MergeDAGs + CreateFinalDAG, the reference's
+    flow) against batched (one create_dag_batched) and served (continuous
+    admission) on the same 512 uneven chunks; all three must give the same DAG."""
+    from dag_compression_fast_amd import serve
+    L, H = _lib.lib(), _lib.hip()
+    chunks = chunk_workload(svd, args.threads)
+    sizes = np.array([len(c) for c in chunks], dtype=np.uint64)
+    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum(sizes)
+    total = int(offsets[-1])
+    log(f"chunks: {len(chunks)} chunks, {total} keys")
+    d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
+    ctx = svd.Context()
+    depth = CHUNKS["depth"]
+    k_levels = 3
+
+    def plain():
+        dags = (_lib.SvdagDag * len(chunks))()
+        for i in range(len(chunks)):
+            ptr = ctypes.c_void_p(d_keys.address + 8 * int(offsets[i]))
+            _lib.check(L.svdag_create_dag(ctx.handle, ptr, int(sizes[i]), depth, ctypes.byref(dags[i])))
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_merge_dags(ctx.handle, dags, len(chunks), ctypes.byref(out)))
+        for i in range(len(chunks)):
+            L.svdag_free_dag(ctx.handle, ctypes.byref(dags[i]))
+        return out
+
+    def batched():
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag_batched(ctx.handle, d_keys.ptr,
+                                              offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                              len(chunks), depth, ctypes.byref(out)))
+        return out
+
+    def timed(fn, reps):
+        """Best of `reps` runs after one warm-up; build + flat layout, device-resident keys."""
+        best, words = None, None
+        for r in range(reps + 1):
+            _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+            t = time.perf_counter()
+            dag = fn()
+            words = final_words(L, _lib, ctx, dag)
+            dt = time.perf_counter() - t
+            if r > 0:
+                best = dt if best is None else min(best, dt)
+            L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+        return best, words
+
+    t_plain, w_plain = timed(plain, args.chunk_reps)
+    log(f"chunks plain: {t_plain * 1e3:.1f} ms")
+    t_batch, w_batch = timed(batched, args.chunk_reps)
+    log(f"chunks batched: {t_batch * 1e3:.1f} ms")
+    same = np.array_equal(w_plain, w_batch)
+
+    served = []
+    server = serve.ChunkServer(depth, slots=args.serve_slots, max_chunk_keys=int(sizes.max()))
+    try:
+        for rate in args.serve_rates:
+            arrivals = serve.poisson_arrivals(rate, len(chunks), seed=int(rate))
+            order = np.random.default_rng(7).permutation(len(chunks))
+            reqs = [serve.Request(i, int(order[i]), chunks[order[i]], float(arrivals[i])) for i in range(len(chunks))]
+            t = time.perf_counter()
+            results = server.run(reqs)
+            span = max(r.done for r in results) - min(r.arrival for r in results)
+            merged = server.merge(results, len(chunks))
+            w_served = final_words(L, _lib, merged.ctx, merged.raw)
+            same_served = np.array_equal(w_served, w_batch)
+            stats = serve.latency_stats(results)
+            served.append({"rate_per_s": rate, "requests": len(reqs), "voxels_per_s": round(total / span, 1),
+                           **{k: round(v, 3) for k, v in stats.items()}, "bit_exact": bool(same_served)})
+            log(f"served {rate}/s: {total / span / 1e9:.2f} Gvox/s p50 {stats['p50_ms']:.2f} ms "
+                f"p99 {stats['p99_ms']:.2f} ms ({time.perf_counter() - t:.1f} s)")
+            merged.free()
+            for r in results:
+                r.dag.free()
+            same = same and same_served
+    finally:
+        server.close()
+    d_keys.free()
+    ctx.close()
+    return {"workload": CHUNKS["desc"], "n_keys": total, "plain_voxels_per_s": round(total / t_plain, 1),
+            "plain_ms": round(t_plain * 1e3, 2), "batched_voxels_per_s": round(total / t_batch, 1),
+            "batched_ms": round(t_batch * 1e3, 2), "batched_vs_plain": round(t_plain / t_batch, 2),
+            "served": served, "serve_slots": args.serve_slots,
+            "check": "plain, batched and served final DAGs identical" if same else "MISMATCH"}
+
+
 def cpu_baseline(keys, depth, threads, reps):
     """The CPU oracle (a restatement of the reference algorithm) on the same keys."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -114,6 +226,11 @@ def main():
     ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
     ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
+    ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
+    ap.add_argument("--chunk-reps", type=int, default=2)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0],
+                    help="Poisson arrival rates (requests/s) of the served run")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -221,17 +338,24 @@ def main():
         if not same:
             log("GPU result differs from the oracle")
 
+    chunk_stats = None
+    if not args.no_chunks:
+        chunk_stats = bench_chunks(svd, _lib, args)
+        if chunk_stats["check"] == "MISMATCH":
+            log("chunk workload: plain, batched and served results differ")
+
     out = {
         "metric": METRIC, "value": round(n / (ms_per_step / 1e3), 1), "unit": "voxels/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "check": check,
+        "roofline": roofline, "cpu_baseline": cpu, "check": check, "chunks": chunk_stats,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
-    return 0 if check != "MISMATCH vs oracle" else 1
+    ok = check != "MISMATCH vs oracle" and (chunk_stats is None or chunk_stats["check"] != "MISMATCH")
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

@@ -260,8 +260,11 @@ unsigned long long exchange(unsigned long long v, int src)
     return r;
 }
 
+std::mutex g_launch_mutex;  // launches from several host threads run one at a time
+
 void launch(dim3 grid, dim3 block, std::function<void()> body)
 {
+    std::lock_guard<std::mutex> serial(g_launch_mutex);
     if (g_cur >= 0) {
         std::fprintf(stderr, "emu: launch from inside a kernel\n");
         std::abort();
