@@ -205,7 +205,10 @@ int finish(svdag_context* ctx, const char* what)
             return set_error(SVDAG_ERR_INTERNAL, "%s: device work did not complete within %.0f s (stalled in: %s)", what,
                              ctx->wait_seconds, stalled);
         }
-        std::this_thread::sleep_for(std::chrono::microseconds(waited < 0.01 ? 20 : 500));
+        // Short calls are noticed promptly (a sleep rounds up to tens of
+        // microseconds); long ones poll every 200 us.
+        if (waited < 0.05) std::this_thread::yield();
+        else std::this_thread::sleep_for(std::chrono::microseconds(200));
     }
     ctx->num_marks = 0;
     ctx->recorder.resolve(true);
