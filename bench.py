@@ -176,6 +176,14 @@ def bench_chunks(svd, _lib, args):
     log(f"chunks plain: {t_plain * 1e3:.1f} ms")
     t_batch, w_batch = timed(batched, args.chunk_reps)
     log(f"chunks batched: {t_batch * 1e3:.1f} ms")
+    if args.kernel_table:
+        ctx.profile_enable(None)
+        ctx.profile_reset()
+        L.svdag_free_dag(ctx.handle, ctypes.byref(batched()))
+        table = ctx.profile_read()
+        ctx.profile_enable([])
+        for k, (ms, cnt) in sorted(table.items(), key=lambda kv: -kv[1][0])[:12]:
+            log(f"  batched {k:20s} {ms:8.3f} ms  {cnt:4d} launches")
     same = np.array_equal(w_plain, w_batch)
 
     served = []

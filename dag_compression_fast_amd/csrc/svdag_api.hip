@@ -29,7 +29,7 @@ namespace {
 
 const char* const kKernelNames[kNumKernelIds] = {
     "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
-    "scan_counts",    "prefix_chunks", "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
+    "scan_counts",    "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail"};
 
@@ -251,7 +251,7 @@ struct BuildLayout {
     uint64_t zero_begin = 0, zero_end = 0;
     uint64_t counters = 0, key_sort_zero = 0, leaf_sort_zero = 0, sketch = 0;
     uint64_t sort_counts = 0, tile_counts = 0, tile_offsets = 0;
-    uint64_t gkeys = 0, keys_a = 0, keys_b = 0;
+    uint64_t keys_a = 0, keys_b = 0;
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
     uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0, tail_hash = 0;
@@ -286,7 +286,6 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.tile_counts = c.take(4 * stage_tiles);
     L.tile_offsets = c.take(8 * stage_tiles);
     const int kb = batched ? 8 : key_bytes;
-    if (batched) L.gkeys = c.take(8 * n);
     L.keys_a = c.take(uint64_t(kb) * n);
     L.keys_b = c.take(uint64_t(kb) * n);
     L.lp_code = c.take(8 * L.leaf_bound);
@@ -465,13 +464,18 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         uint64_t* d_off = reinterpret_cast<uint64_t*>(A + L.total);
         if (hipMemcpyAsync(d_off, batch->h_offsets, 8 * (batch->num_chunks + 1), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(set_error(SVDAG_ERR_HIP, "offset upload failed"));
-        uint64_t* g = reinterpret_cast<uint64_t*>(A + L.gkeys);
-        launch_prefix_chunks<uint64_t>(st, reinterpret_cast<const uint64_t*>(d_keys), g, d_off, batch->num_chunks, n,
-                                       uint32_t(3 * batch->chunk_depth), c);
+        // The sort's first reads put each chunk's index above its chunk-local keys.
+        ChunkPrefix prefix;
+        prefix.offsets = d_off;
+        prefix.num_chunks = batch->num_chunks;
+        prefix.chunk_bits = uint32_t(3 * batch->chunk_depth);
+        prefix.err = err;
         uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
         uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
-        where = launch_sort<uint64_t>(st, L.key_plan, g, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
-                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or);
+        where = launch_sort<uint64_t>(st, L.key_plan, reinterpret_cast<const uint64_t*>(d_keys), nullptr, ka, kb,
+                                      nullptr, nullptr, nullptr, nullptr, &c->n, n,
+                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or,
+                                      prefix);
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     } else {
         K* ka = reinterpret_cast<K*>(A + L.keys_a);

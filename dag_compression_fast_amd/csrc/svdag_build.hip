@@ -67,28 +67,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const uint32_t* __
     if (total_out && threadIdx.x == 0) *total_out = running;
 }
 
-// Chunk-local keys -> global keys: chunk index (Morton order of the sub-volumes,
-// src/Utils/Aabb.h:83-105) above the chunk's 3*chunk_depth bits.
-template <typename K>
-__global__ void k_prefix_chunks(const K* __restrict__ in, uint64_t* __restrict__ out, const uint64_t* __restrict__ offsets,
-                                uint64_t num_chunks, uint64_t n, uint32_t chunk_bits, BuildCounters* __restrict__ c)
-{
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    bool bad = false;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t lo = 0, hi = num_chunks;  // offsets[lo] <= i < offsets[hi]
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) / 2;
-            if (offsets[mid] <= i) lo = mid;
-            else hi = mid;
-        }
-        const uint64_t k = uint64_t(in[i]);
-        bad |= (k >> chunk_bits) != 0;
-        out[i] = (lo << chunk_bits) | k;
-    }
-    if (bad) atomicOr((unsigned long long*)&c->error, (unsigned long long)kErrKeyRange);
-}
-
 // ---------------------------------------------------------------------------
 // Leaf pack: sorted keys -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block,
 // i.e. per run of keys with equal code >> 6 (duplicates OR harmlessly).
@@ -1058,18 +1036,6 @@ void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets
     TimedLaunch t(kKScanCounts, s);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, s, counts, offsets, items, per_tile, total_out);
 }
-
-template <typename K>
-void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint64_t* d_offsets, uint64_t num_chunks,
-                          uint64_t n, uint32_t chunk_bits, BuildCounters* c)
-{
-    if (n == 0) return;
-    TimedLaunch t(kKPrefixChunks, s);
-    hipLaunchKernelGGL(k_prefix_chunks<K>, dim3(grid_for((n + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, in,
-                       out, d_offsets, num_chunks, n, chunk_bits, c);
-}
-template void launch_prefix_chunks<uint64_t>(hipStream_t, const uint64_t*, uint64_t*, const uint64_t*, uint64_t,
-                                             uint64_t, uint32_t, BuildCounters*);
 
 template <typename K>
 void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,

@@ -24,7 +24,6 @@ enum KernelId : int {
     kKSortScatterPairs,
     kKFill,
     kKScanCounts,
-    kKPrefixChunks,
     kKLeafCount,
     kKLeafPack,
     kKTableClear,
@@ -107,11 +106,21 @@ inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * s
 // keys_b); the first pass reads keys_in.  Returns which buffer holds the result
 // (0 = a, 1 = b, 2 = final); if final_keys / final_vals are non-null the last
 // pass writes there instead.  key_or (optional) receives the OR of all keys.
+// Chunk-local input of a batched build: key i, with offsets[c] <= i <
+// offsets[c + 1], becomes (c << chunk_bits) | key when the sort first reads it
+// (chunk order = Morton order of the sub-volumes, src/Utils/Aabb.h:83-105).
+struct ChunkPrefix {
+    const uint64_t* offsets = nullptr;  // device, num_chunks + 1 entries; null: keys are global
+    uint64_t num_chunks = 0;
+    uint32_t chunk_bits = 0;
+    unsigned long long* err = nullptr;  // receives kErrKeyRange for a key >= 2^chunk_bits
+};
+
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,
                 K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
                 const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
-                unsigned long long* key_or);
+                unsigned long long* key_or, const ChunkPrefix& prefix = ChunkPrefix{});
 
 // ---- build ----------------------------------------------------------------
 constexpr uint32_t kTableOverflow = 4096;  // slots past 2^log2 for ordered tables
@@ -152,9 +161,6 @@ void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
 void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
                         uint64_t per_tile, uint64_t* total_out);
 
-template <typename K>
-void launch_prefix_chunks(hipStream_t s, const K* in, uint64_t* out, const uint64_t* d_offsets, uint64_t num_chunks,
-                          uint64_t n, uint32_t chunk_bits, BuildCounters* c);
 
 template <typename K>
 void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,

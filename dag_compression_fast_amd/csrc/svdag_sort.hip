@@ -64,6 +64,35 @@ __device__ __forceinline__ TileWalk xcd_tiles(uint64_t ntiles)
     return TileWalk{start + local, end, groups};
 }
 
+// Chunk of input position i: the c with offsets[c] <= i < offsets[c + 1],
+// searched in [lo, hi).
+__device__ __forceinline__ uint64_t chunk_search(const ChunkPrefix& p, uint64_t i, uint64_t lo, uint64_t hi)
+{
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (p.offsets[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Chunks of a tile's first and last key (the same for every thread).
+struct TileChunks {
+    uint64_t first, last;
+};
+__device__ __forceinline__ TileChunks tile_chunks(const ChunkPrefix& p, uint64_t base, uint32_t cnt)
+{
+    const uint64_t first = chunk_search(p, base, 0, p.num_chunks);
+    return TileChunks{first, chunk_search(p, base + cnt - 1, first, p.num_chunks)};
+}
+
+// Global key of input key k at position i of a tile with chunks tc.
+__device__ __forceinline__ uint64_t prefixed(const ChunkPrefix& p, const TileChunks& tc, uint64_t i, uint64_t k)
+{
+    const uint64_t c = tc.first == tc.last ? tc.first : chunk_search(p, i, tc.first, tc.last + 1);
+    return (c << p.chunk_bits) | k;
+}
+
 struct HistPlan {
     int passes;
     uint32_t shift[kMaxPasses];
@@ -78,7 +107,8 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
                                                             const uint64_t* __restrict__ n_ptr, HistPlan plan,
                                                             uint32_t* __restrict__ hist,
                                                             unsigned long long* __restrict__ key_or,
-                                                            uint32_t* __restrict__ counts, uint64_t stride)
+                                                            uint32_t* __restrict__ counts, uint64_t stride,
+                                                            ChunkPrefix prefix)
 {
     constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
     __shared__ uint32_t s_hist[kMaxPasses][kRadix];
@@ -104,6 +134,20 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
                 const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
                 k[i] = idx < cnt ? keys[base + idx] : K(0);
             }
+        }
+        if (prefix.offsets) {
+            // Chunk-local keys: range check, then the chunk index above them.
+            const TileChunks tc = tile_chunks(prefix, base, cnt);
+            bool bad = false;
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
+                if (idx < cnt) {
+                    bad |= (uint64_t(k[i]) >> prefix.chunk_bits) != 0;
+                    k[i] = K(prefixed(prefix, tc, base + idx, uint64_t(k[i])));
+                }
+            }
+            if (bad) atomicOr(prefix.err, (unsigned long long)kErrKeyRange);
         }
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
@@ -203,7 +247,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint64_t* __restrict__ n_ptr, uint32_t shift,
                                                                uint32_t digit_mask,
-                                                               const uint32_t* __restrict__ offsets, uint64_t stride)
+                                                               const uint32_t* __restrict__ offsets, uint64_t stride,
+                                                               ChunkPrefix prefix)
 {
     constexpr int W = kSortThreads / kWave;
     constexpr uint32_t kTile = uint32_t(kSortThreads) * IPT;
@@ -252,6 +297,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                 const bool valid = wave_base + i * kWave + lane < cnt;
                 key[i] = valid ? src[i * kWave] : K(0);
                 if constexpr (PAIRS) val[i] = valid ? vs[i * kWave] : 0u;
+            }
+        }
+        if (prefix.offsets && cnt) {
+            const TileChunks tc = tile_chunks(prefix, base, cnt);
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                const uint32_t idx = wave_base + i * kWave + lane;
+                if (idx < cnt) key[i] = K(prefixed(prefix, tc, base + idx, uint64_t(key[i])));
             }
         }
     };
@@ -380,7 +433,7 @@ int device_cu_count()
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
-                        uint64_t stride, uint32_t grid, bool counted)
+                        uint64_t stride, uint32_t grid, bool counted, const ChunkPrefix& prefix)
 {
     if (!counted) {
         TimedLaunch t(kKSortCount, stream);
@@ -395,18 +448,19 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, true, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
-                           vdst, n_ptr, shift, dmask, counts, stride);
+                           vdst, n_ptr, shift, dmask, counts, stride, prefix);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, false, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
-                           nullptr, nullptr, n_ptr, shift, dmask, counts, stride);
+                           nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     }
 }
 
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in, K* keys_a,
                 K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,
-                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts, unsigned long long* key_or)
+                const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts, unsigned long long* key_or,
+                const ChunkPrefix& prefix)
 {
     if (plan.passes == 0 || n_bound == 0) return -1;
     const bool pairs = vals_in != nullptr;
@@ -432,10 +486,10 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         TimedLaunch t(kKHistogram, stream);
         if (ipt == kSortIptLarge)
             hipLaunchKernelGGL((k_histogram<K, kSortIptLarge>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
-                               n_ptr, hp, hist, key_or, counts, stride);
+                               n_ptr, hp, hist, key_or, counts, stride, prefix);
         else
             hipLaunchKernelGGL((k_histogram<K, kSortIptSmall>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
-                               n_ptr, hp, hist, key_or, counts, stride);
+                               n_ptr, hp, hist, key_or, counts, stride, prefix);
     }
     for (int p = 0; p < plan.passes; ++p) {
         const bool last = p + 1 == plan.passes;
@@ -450,10 +504,12 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         const uint32_t dmask = (1u << plan.bits[p]) - 1u;
         if (ipt == kSortIptLarge)
             launch_pass<K, kSortIptLarge>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid, p == 0);
+                                          hist + p * kRadix, stride, grid, p == 0,
+                                          p == 0 ? prefix : ChunkPrefix{});
         else
             launch_pass<K, kSortIptSmall>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid, p == 0);
+                                          hist + p * kRadix, stride, grid, p == 0,
+                                          p == 0 ? prefix : ChunkPrefix{});
         src = dst;
         vsrc = vdst;
     }
@@ -462,9 +518,9 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
 
 template int launch_sort<uint64_t>(hipStream_t, const SortPlan&, const uint64_t*, const uint32_t*, uint64_t*,
                                    uint64_t*, uint32_t*, uint32_t*, uint64_t*, uint32_t*, const uint64_t*, uint64_t,
-                                   uint32_t*, uint32_t*, unsigned long long*);
+                                   uint32_t*, uint32_t*, unsigned long long*, const ChunkPrefix&);
 template int launch_sort<uint32_t>(hipStream_t, const SortPlan&, const uint32_t*, const uint32_t*, uint32_t*,
                                    uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint64_t*, uint64_t,
-                                   uint32_t*, uint32_t*, unsigned long long*);
+                                   uint32_t*, uint32_t*, unsigned long long*, const ChunkPrefix&);
 
 }  // namespace svdag
