@@ -241,8 +241,8 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
     }
 }
 
-template <typename K, bool PAIRS, int IPT>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+template <typename K, bool PAIRS, int IPT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                                const uint32_t* __restrict__ vals_in,
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint64_t* __restrict__ n_ptr, uint32_t shift,
@@ -250,9 +250,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
                                                                const uint32_t* __restrict__ offsets, uint64_t stride,
                                                                ChunkPrefix prefix)
 {
-    constexpr int W = kSortThreads / kWave;
-    constexpr uint32_t kTile = uint32_t(kSortThreads) * IPT;
-    constexpr int DPT = kRadix / kSortThreads;  // digit slots per thread in the tile scan
+    constexpr int W = THREADS / kWave;
+    constexpr uint32_t kTile = uint32_t(THREADS) * IPT;
+    constexpr int DPT = kRadix / THREADS;  // digit slots per thread in the tile scan
     // The staging area holds the tile in digit order; during ranking it holds
     // the per-wave match table s_match[W][kRadix] instead.
     static_assert(W * kRadix * sizeof(uint64_t) <= kTile * sizeof(K), "match table fits the staging area");
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
     load_tile(walk.tile);
 
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
-        for (uint32_t i = tid; i < W * kRadix; i += kSortThreads) {
+        for (uint32_t i = tid; i < W * kRadix; i += THREADS) {
             (&s_wcnt[0][0])[i] = 0;
             reinterpret_cast<uint64_t*>(s_keys)[i] = 0;
         }
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
             tsum += total;
         }
         uint32_t unused;
-        uint32_t lstart = block_exclusive_scan<kSortThreads>(tsum, s_scan, unused);
+        uint32_t lstart = block_exclusive_scan<THREADS>(tsum, s_scan, unused);
 #pragma unroll
         for (int k = 0; k < DPT; ++k) {
             const uint32_t d = tid * DPT + k;
@@ -407,9 +407,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const K* __restri
         };
         if (tile_n == kTile) {
 #pragma unroll
-            for (int i = 0; i < IPT; ++i) store(tid + i * kSortThreads);
+            for (int i = 0; i < IPT; ++i) store(tid + i * THREADS);
         } else {
-            for (uint32_t j = tid; j < tile_n; j += kSortThreads) store(j);
+            for (uint32_t j = tid; j < tile_n; j += THREADS) store(j);
         }
         __syncthreads();
     }
@@ -430,6 +430,11 @@ int device_cu_count()
     return cus;
 }
 
+#ifndef SVDAG_SCATTER_WIDE
+#define SVDAG_SCATTER_WIDE 0
+#endif
+constexpr bool kScatterWide = SVDAG_SCATTER_WIDE != 0;
+
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
@@ -447,12 +452,17 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     }
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, true, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst, vsrc,
-                           vdst, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
+                           src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
+    } else if constexpr (kScatterWide && sizeof(K) == 8 && IPT == kSortIptSmall) {
+        // The same 4096-key tile on 512 threads x 8 keys.
+        TimedLaunch t(kKSortScatterKeys, stream);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT / 2, 2 * kSortThreads>), dim3(grid), dim3(2 * kSortThreads),
+                           0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, dst,
-                           nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
+                           src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     }
 }
 
