@@ -4,7 +4,7 @@ Workload (default): triangle surfaces in a 2048^3 grid (depth 11): 20k seeded
 triangles (v0 uniform in the grid, edges lognormal mu=4 sigma=1 voxels),
 26-separating surface voxelization, 30% duplicates injected, shuffled uint64
 Morton keys.  One step = svdag_create_dag + svdag_create_final_dag on keys
-already resident in HBM (sort, leaf pack, leaf and level dedup, per-level arrays
+already resident in HBM (sort, leaf reduce, leaf and level dedup, per-level arrays
 in ascending-hash order, flat uint32 layout), as DAGCompression::CreateDAG +
 CreateFinalDAG do (src/DAGCompression.cu:1003-1107, 1550-1610).
 
@@ -65,8 +65,9 @@ def algorithmic_bytes(kernel, st, key_bytes, launches):
         "sort_scatter_keys": launches * 2 * key_bytes * n,
         # The unique-leaf sort moves (u64 mask, u32 slot) pairs.
         "sort_scatter_pairs": launches * 2 * 12 * lu,
-        "leaf_count": key_bytes * n,
-        "leaf_pack": key_bytes * n + 16 * l0,
+        # grouped keys read once; (code, mask) per leaf into the chunk regions
+        "leaf_reduce": key_bytes * n + 16 * l0,
+        "leaf_compact": 32 * l0,
         # mask read, slot write, unique list append
         "leaf_insert": 12 * l0 + 12 * lu,
         # child code + slot + rank + hash reads; parent code, slot, mask, 8 children writes

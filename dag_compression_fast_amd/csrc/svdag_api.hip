@@ -29,7 +29,7 @@ namespace {
 
 const char* const kKernelNames[kNumKernelIds] = {
     "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
-    "scan_counts",    "leaf_count",  "leaf_pack",         "table_clear",        "leaf_insert",
+    "scan_counts",    "leaf_reduce", "leaf_compact",      "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail"};
 
@@ -253,10 +253,12 @@ struct BuildLayout {
     uint64_t sort_counts = 0, tile_counts = 0, tile_offsets = 0;
     uint64_t keys_a = 0, keys_b = 0;
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
+    uint64_t reg_code = 0, reg_mask = 0;  // leaf-reduce regions, over the later stages' buffers
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
     uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0, tail_hash = 0;
     uint64_t tkeys[2] = {}, trep[2] = {}, trank[2] = {};
     uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
+    uint32_t group_bits = 0;
     SortPlan key_plan{}, leaf_plan{};
 };
 
@@ -269,9 +271,14 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.level_bound = std::min<uint64_t>(L.leaf_bound, pow8_capped(depth - 3, n));
     L.leaf_slots = 1ull << table_lg(L.leaf_bound, load_shift);
     L.level_slots = (1ull << table_lg(L.level_bound, load_shift)) + kTableOverflow;
-    // Keys only need grouping by code >> 6 (the leaf pack ORs the low 6 bits), so
-    // the sort skips bits [0, 6).
-    L.key_plan = make_sort_plan(6, 3 * depth, radix_bits, sort_ipt);
+    // Keys only need grouping by code >> (6 + group_bits): the leaf reduce ORs
+    // the low 6 bits into a mask and indexes its table by the next group_bits,
+    // so the sort skips both.  It keeps at least one bit so that its first read
+    // (range check, chunk prefix) always runs; at depth <= 5 the whole key list
+    // is then one group.
+    const uint32_t code_bits = uint32_t(3 * depth - 6);
+    L.group_bits = std::min(kLeafGroupBits, code_bits);
+    L.key_plan = make_sort_plan(int(6 + std::min(L.group_bits, code_bits - 1)), 3 * depth, radix_bits, sort_ipt);
     L.leaf_plan = make_sort_plan(0, 64);
     const uint64_t stage_tiles = std::max({tiles(n, kPackTileItems), tiles(L.leaf_bound, kBuildTileItems),
                                            tiles(L.level_slots, kExtractTileItems), uint64_t(1)});
@@ -311,6 +318,12 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
         L.trank[i] = c.take(4 * L.level_slots);
     }
     L.total = c.off;
+    // The leaf-reduce regions are read only by the compaction, before any of
+    // the buffers from the leaf table on is written.
+    const uint64_t region_bytes = 8 * tiles(n, kPackTileItems) * kLeafRegionItems;
+    L.reg_code = L.leaf_table;
+    L.reg_mask = L.reg_code + region_bytes;
+    L.total = std::max(L.total, L.reg_mask + region_bytes);
     return L;
 }
 
@@ -492,14 +505,18 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     uint32_t* lp_slot = reinterpret_cast<uint32_t*>(A + L.lp_slot);
     const uint64_t code_mask = depth >= 22 ? ~0ull : ((1ull << (3 * depth)) - 1);
     // Leaf ReduceByKey (DAGCompression.cu:1044-1068).
+    uint64_t* reg_code = reinterpret_cast<uint64_t*>(A + L.reg_code);
+    uint64_t* reg_mask = reinterpret_cast<uint64_t*>(A + L.reg_mask);
     if (batch || sizeof(K) == 8)
-        launch_leaf_pack<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, lp_code, lp_mask, tile_counts,
-                                   tile_offsets, n, depth - 3, code_mask);
+        launch_leaf_reduce<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, L.group_bits, reg_code, reg_mask,
+                                     lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
+                                     code_mask);
     else
-        launch_leaf_pack<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, lp_code, lp_mask, tile_counts,
-                                   tile_offsets, n, depth - 3, code_mask);
+        launch_leaf_reduce<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, L.group_bits, reg_code, reg_mask,
+                                     lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
+                                     code_mask);
 
-    mark(ctx, "leaf pack");
+    mark(ctx, "leaf reduce");
     // SortLevel(Leaves) (DAGCompression.cu:1070-1071): dedup table, then order.
     uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + L.leaf_table);
     uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + L.leaf_rank);
@@ -640,6 +657,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         return fail(set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)"));
     if ((h.error & kErrTableOverflow) && estimate) return fail(kRetryWithBounds);
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
+    if (h.error & kErrInternal) return fail(set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
 
     out->storage = storage;
@@ -888,6 +906,7 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     TRY(finish(ctx, "merge_dags"));
     const BuildCounters& h = *ctx->h_counters;
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
+    if (h.error & kErrInternal) return fail(set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "merged root level holds %llu nodes", (unsigned long long)h.uniq[0]));
     out->storage = storage;
     out->leaves = leaves_out;

@@ -1,9 +1,9 @@
 // svdag_build.hip -- gfx950 kernels of the bottom-up DAG build.
 //
 // Reference path (src/DAGCompression.cu):
-//   SelectUnique + leaf ReduceByKey  :1035-1068  -> k_leaf_count / k_leaf_pack
-//       (one pass over the sorted keys: duplicates OR into the same 4^3 mask,
-//       so no separate unique step is needed)
+//   SelectUnique + leaf ReduceByKey  :1035-1068  -> k_leaf_reduce / k_leaf_compact
+//       (one pass over the keys grouped by code >> 15: duplicates OR into the
+//       same 4^3 mask, so no separate unique step is needed)
 //   SortLevel(Leaves)               :1070-1071  -> k_table_clear / k_leaf_insert
 //       (hash table keyed by the mask; the first inserter of a mask appends it
 //       to the unique list) + a key/value radix sort of that list +
@@ -68,150 +68,187 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const uint32_t* __
 }
 
 // ---------------------------------------------------------------------------
-// Leaf pack: sorted keys -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block,
-// i.e. per run of keys with equal code >> 6 (duplicates OR harmlessly).
-// Wave w of a tile owns kPackWaveKeys consecutive keys and reads them as
-// kPackGroups coalesced groups of 64.  A run belongs to the wave holding its
-// first key; it is OR-reduced by a segmented wave scan, carried across groups,
-// and finished past the wave's keys by reading on until the code changes.
+// Leaf reduce: keys sorted by code >> (6 + r), with r = min(9, 3 * depth - 6)
+// -> (code >> 6, OR of 1 << (code & 63)) per 4^3 block, in code order.
+// A run of keys with equal code >> (6 + r) (a "block group", at most 2^r
+// leaves) is reduced in a wave-private LDS table of 2^r words indexed by the
+// leaf's low r bits; the keys of a group may come in any order, so the sort
+// skips the low r bits of the leaf code as well as the 6 voxel bits.
+// Wave work unit: a chunk of kPackTileItems consecutive keys.  A wave owns the
+// groups whose first key lies in its chunk (reading on past the chunk to
+// finish its last one) and writes their leaves into its own region of
+// kPackTileItems + 512 entries: groups inside the chunk have at most one leaf
+// per key, the last group at most 2^r leaves.  k_leaf_compact then moves the
+// regions together.
 // ---------------------------------------------------------------------------
-constexpr int kPackGroups = 16;
-constexpr int kPackWaveKeys = kPackGroups * kWave;
-constexpr int kPackTile = (kThreads / kWave) * kPackWaveKeys;
+constexpr uint32_t kReduceChunk = uint32_t(kPackTileItems);
+constexpr uint32_t kReduceTable = 1u << kLeafGroupBits;
+constexpr uint32_t kReduceRegion = uint32_t(kLeafRegionItems);
+static_assert(kReduceRegion == kReduceChunk + kReduceTable, "region bound");
+constexpr int kReduceGroups = 8;  // groups of 64 keys loaded together
+constexpr int kReduceWaves = kThreads / kWave;
+static_assert(kReduceTable == 8 * kWave, "one lane holds the 8 leaves of one parent");
 
 template <typename K>
-__device__ __forceinline__ uint64_t load_key(const K* __restrict__ p, uint64_t i, uint64_t n, uint64_t code_mask)
+__global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ sorted,
+                                                          const uint64_t* __restrict__ n_ptr, uint64_t code_mask,
+                                                          uint32_t r, uint64_t* __restrict__ reg_code,
+                                                          uint64_t* __restrict__ reg_mask,
+                                                          uint32_t* __restrict__ counts,
+                                                          uint64_t* __restrict__ next_count,
+                                                          unsigned long long* __restrict__ err)
 {
-    return i < n ? (uint64_t(p[i]) & code_mask) : 0;
-}
-
-// Keys of the wave's groups and their run-head ballots; gheads counts the keys
-// that start a new code >> 9 (a new parent of the leaf level).
-template <typename K>
-__device__ __forceinline__ void wave_heads(const K* __restrict__ sorted, uint64_t n, uint64_t chunk,
-                                           uint64_t code_mask, uint64_t (&k)[kPackGroups],
-                                           uint64_t (&heads)[kPackGroups], uint32_t& gheads)
-{
-    const uint32_t lane = lane_id();
-    if (chunk + kPackWaveKeys <= n) {
-#pragma unroll
-        for (int g = 0; g < kPackGroups; ++g) k[g] = uint64_t(sorted[chunk + uint64_t(g) * kWave + lane]) & code_mask;
-    } else {
-#pragma unroll
-        for (int g = 0; g < kPackGroups; ++g) k[g] = load_key(sorted, chunk + uint64_t(g) * kWave + lane, n, code_mask);
-    }
-    uint64_t carry = (chunk > 0 && chunk <= n) ? (uint64_t(sorted[chunk - 1]) & code_mask) : ~0ull;
-    gheads = 0;
-#pragma unroll
-    for (int g = 0; g < kPackGroups; ++g) {
-        uint64_t prev = __shfl_up(k[g], 1, kWave);
-        if (lane == 0) prev = carry;
-        const uint64_t i = chunk + uint64_t(g) * kWave + lane;
-        const bool valid = i < n;
-        const bool head = valid && (i == 0 || (k[g] >> 6) != (prev >> 6));
-        const bool ghead = valid && (i == 0 || (k[g] >> 9) != (prev >> 9));
-        heads[g] = __ballot(head);
-        gheads += uint32_t(__popcll(__ballot(ghead)));
-        carry = __shfl(k[g], kWave - 1, kWave);
-    }
-}
-
-__device__ __forceinline__ uint64_t wave_or(uint64_t v)
-{
-#pragma unroll
-    for (int d = kWave / 2; d >= 1; d >>= 1) v |= __shfl_xor(v, d, kWave);
-    return v;
-}
-
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_leaf_count(const K* __restrict__ sorted, const uint64_t* __restrict__ n_ptr,
-                                                         uint64_t code_mask, uint32_t* __restrict__ counts,
-                                                         uint64_t* __restrict__ next_count)
-{
-    __shared__ uint64_t s_red[kThreads / kWave];
-    const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
+    __shared__ uint64_t s_tab[kReduceWaves][kReduceTable];
+    __shared__ uint64_t s_parents[kReduceWaves];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x / kWave;
-    uint64_t gtot = 0;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t k[kPackGroups], heads[kPackGroups];
-        uint32_t gheads;
-        wave_heads(sorted, n, tile * kPackTile + uint64_t(w) * kPackWaveKeys, code_mask, k, heads, gheads);
-        uint32_t cnt = 0;
+    uint64_t* const tab = s_tab[w];
 #pragma unroll
-        for (int g = 0; g < kPackGroups; ++g) cnt += uint32_t(__popcll(heads[g]));
-        const uint64_t tot = block_reduce_sum<kThreads>(lane == 0 ? cnt : 0u, s_red);
-        if (threadIdx.x == 0) counts[tile] = uint32_t(tot);
-        gtot += lane == 0 ? gheads : 0u;
-    }
-    gtot = block_reduce_sum<kThreads>(gtot, s_red);
-    if (threadIdx.x == 0 && gtot) atomicAdd((unsigned long long*)next_count, (unsigned long long)gtot);
-}
-
-// Each key ORs its bit into its run's word in the wave's LDS area (the OR does
-// not depend on lane order); head lanes write their run's code, and the words
-// are written out after the wave's last group.
-template <typename K>
-__global__ __launch_bounds__(kThreads) void k_leaf_pack(const K* __restrict__ sorted, const uint64_t* __restrict__ n_ptr,
-                                                        uint64_t code_mask, const uint64_t* __restrict__ offsets,
-                                                        uint64_t* __restrict__ lp_code, uint64_t* __restrict__ lp_mask)
-{
-    __shared__ uint32_t s_wcount[kThreads / kWave];
-    __shared__ uint64_t s_or[kThreads / kWave][kPackWaveKeys];
+    for (int e = 0; e < 8; ++e) tab[lane * 8 + e] = 0ull;
+    __builtin_amdgcn_wave_barrier();
     const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kPackTile - 1) / kPackTile;
-    const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x / kWave;
-    uint64_t* const wor = s_or[w];
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t chunk = tile * kPackTile + uint64_t(w) * kPackWaveKeys;
-        uint64_t k[kPackGroups], heads[kPackGroups];
-        uint32_t gheads;
-        wave_heads(sorted, n, chunk, code_mask, k, heads, gheads);
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int g = 0; g < kPackGroups; ++g) cnt += uint32_t(__popcll(heads[g]));
-        if (lane == 0) s_wcount[w] = cnt;
-#pragma unroll
-        for (int i = 0; i < kPackGroups; ++i) wor[i * kWave + lane] = 0ull;
-        __syncthreads();
-        uint64_t q = offsets[tile];
-        for (uint32_t i = 0; i < w; ++i) q += s_wcount[i];
-        __syncthreads();  // s_wcount is rewritten by the next tile
+    const uint64_t nchunks = (n + kReduceChunk - 1) / kReduceChunk;
+    const uint32_t sh = 6 + r;
+    const uint32_t low_mask = (1u << r) - 1u;
+    uint64_t parents = 0;
+    bool overflow = false;
 
-        // Run r of this wave = the r-th head; keys before the wave's first head
-        // continue a run owned by the previous wave.
-        int32_t run_base = -1;
+    for (uint64_t chunk = uint64_t(blockIdx.x) * kReduceWaves + w; chunk < nchunks;
+         chunk += uint64_t(gridDim.x) * kReduceWaves) {
+        const uint64_t c0 = chunk * kReduceChunk;
+        const uint64_t c1 = min(c0 + kReduceChunk, n);
+        uint64_t* const oc = reg_code + chunk * kReduceRegion;
+        uint64_t* const om = reg_mask + chunk * kReduceRegion;
+        uint32_t written = 0;
+        // Group of the key before the chunk (~0 never equals a group index).
+        uint64_t prev_b = c0 > 0 ? (uint64_t(sorted[c0 - 1]) & code_mask) >> sh : ~0ull;
+        bool open = false;  // a group owned by this wave is being accumulated
+        uint64_t cur_b = 0;
+        bool done = false;
+
+        // Writes the open group's leaves in order and clears the table.  Lane l
+        // holds leaves 8l..8l+7, i.e. the children of one parent.
+        auto flush = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            uint64_t m[8];
+            uint32_t cnt = 0;
 #pragma unroll
-        for (int g = 0; g < kPackGroups; ++g) {
-            const uint64_t i = chunk + uint64_t(g) * kWave + lane;
-            const uint64_t H = heads[g];
-            const int32_t r = run_base + int32_t(__popcll(H & (lanemask_lt() | (1ull << lane))));
-            if (i < n && r >= 0) atomicOr((unsigned long long*)&wor[r], 1ull << (k[g] & 63));
-            if ((H >> lane) & 1ull) lp_code[q + uint32_t(r)] = k[g] >> 6;
-            run_base += int32_t(__popcll(H));
-        }
-        if (run_base >= 0) {
-            // The wave's last run may continue past its keys.
-            const uint64_t open_code = __shfl(k[kPackGroups - 1], kWave - 1, kWave) >> 6;
-            uint64_t extra = 0;
-            const uint64_t end = chunk + kPackWaveKeys;
-            const bool runs_on = end < n && (uint64_t(sorted[end]) & code_mask) >> 6 == open_code;
-            if (runs_on) {
-                for (uint64_t i0 = end; i0 < n; i0 += kWave) {
-                    const uint64_t i = i0 + lane;
-                    const uint64_t kk = load_key(sorted, i, n, code_mask);
-                    const uint64_t out = __ballot(!(i < n && (kk >> 6) == open_code));
-                    const uint32_t stop = out ? uint32_t(__ffsll((long long)out) - 1) : uint32_t(kWave);
-                    extra |= wave_or(lane < stop ? (1ull << (kk & 63)) : 0ull);
-                    if (stop < uint32_t(kWave)) break;
+            for (int e = 0; e < 8; ++e) {
+                m[e] = tab[lane * 8 + e];
+                cnt += m[e] != 0ull ? 1u : 0u;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tab[lane * 8 + e] = 0ull;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t incl = wave_inclusive_scan(cnt);
+            const uint32_t total = __shfl(incl, kWave - 1, kWave);
+            if (written + total > kReduceRegion) {
+                overflow = true;
+            } else {
+                uint32_t pos = written + incl - cnt;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    if (m[e]) {
+                        oc[pos] = (cur_b << r) | uint64_t(lane * 8 + e);
+                        om[pos] = m[e];
+                        ++pos;
+                    }
                 }
-                if (lane == 0) atomicOr((unsigned long long*)&wor[run_base], extra);
+                written += total;
+            }
+            parents += uint64_t(__popcll(__ballot(cnt != 0)));
+            open = false;
+        };
+
+        for (uint64_t base = c0; !done; base += uint64_t(kReduceGroups) * kWave) {
+            uint64_t k[kReduceGroups];
+            if (base + uint64_t(kReduceGroups) * kWave <= n) {
+#pragma unroll
+                for (int g = 0; g < kReduceGroups; ++g) k[g] = uint64_t(sorted[base + g * kWave + lane]) & code_mask;
+            } else {
+#pragma unroll
+                for (int g = 0; g < kReduceGroups; ++g) {
+                    const uint64_t i = base + g * kWave + lane;
+                    k[g] = i < n ? uint64_t(sorted[i]) & code_mask : 0ull;
+                }
+            }
+            // Groups are processed from k[0]; the rest shift down after each.
+#pragma unroll 1
+            for (int g = 0; g < kReduceGroups && !done; ++g) {
+                const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
+                const uint64_t key = k[0];
+#pragma unroll
+                for (int j = 0; j + 1 < kReduceGroups; ++j) k[j] = k[j + 1];
+                const bool valid = gi + lane < n;
+                const uint64_t b = key >> sh;
+                uint64_t pb = __shfl_up(b, 1, kWave);
+                if (lane == 0) pb = prev_b;
+                prev_b = __shfl(b, kWave - 1, kWave);
+                const uint64_t V = __ballot(valid);
+                uint64_t heads = __ballot(valid && b != pb);  // first keys of block groups
+                // Segments [lo, next head): lanes before the first head continue the
+                // open group, or a group owned by an earlier wave when none is open.
+                uint32_t lo = 0;
+                while (true) {
+                    const uint32_t hi = heads ? uint32_t(__ffsll((long long)heads) - 1) : uint32_t(kWave);
+                    if (open && valid && lane >= lo && lane < hi)
+                        atomicOr((unsigned long long*)&tab[uint32_t(key >> 6) & low_mask], 1ull << (key & 63));
+                    const bool keys_end = hi == uint32_t(kWave) && V != ~0ull;
+                    if (hi == uint32_t(kWave) && !keys_end) {
+                        if (!open && gi + kWave >= c1) done = true;  // no group starts in the chunk
+                        break;
+                    }
+                    if (open) flush();
+                    if (keys_end || gi + hi >= c1) {
+                        done = true;
+                        break;
+                    }
+                    open = true;
+                    cur_b = __shfl(b, hi, kWave);
+                    heads &= heads - 1ull;
+                    lo = hi;
+                }
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        for (int32_t r = int32_t(lane); r <= run_base; r += kWave) lp_mask[q + uint32_t(r)] = wor[r];
+        if (lane == 0) counts[chunk] = written;
+    }
+    if (lane == 0) s_parents[w] = parents;
+    if (overflow && lane == 0) atomicOr(err, (unsigned long long)kErrInternal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int i = 0; i < kReduceWaves; ++i) t += s_parents[i];
+        if (t) atomicAdd((unsigned long long*)next_count, (unsigned long long)t);
+    }
+}
+
+// Region of chunk c (counts[c] leaves) -> lp_code / lp_mask at offsets[c].
+// One wave per chunk.  A total beyond the capacity (distinct leaves never
+// exceed it) is reported, not written.
+__global__ __launch_bounds__(kThreads) void k_leaf_compact(const uint64_t* __restrict__ reg_code,
+                                                           const uint64_t* __restrict__ reg_mask,
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint64_t* __restrict__ offsets,
+                                                           const uint64_t* __restrict__ n_ptr,
+                                                           uint64_t* __restrict__ lp_code,
+                                                           uint64_t* __restrict__ lp_mask, uint64_t capacity,
+                                                           unsigned long long* __restrict__ err)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t nchunks = (*n_ptr + kReduceChunk - 1) / kReduceChunk;
+    for (uint64_t chunk = uint64_t(blockIdx.x) * kReduceWaves + threadIdx.x / kWave; chunk < nchunks;
+         chunk += uint64_t(gridDim.x) * kReduceWaves) {
+        const uint32_t cnt = counts[chunk];
+        const uint64_t dst = offsets[chunk];
+        const uint64_t src = chunk * kReduceRegion;
+        if (dst + cnt > capacity) {
+            if (lane == 0) atomicOr(err, (unsigned long long)kErrInternal);
+            continue;
+        }
+        for (uint32_t j = lane; j < cnt; j += kWave) {
+            lp_code[dst + j] = reg_code[src + j];
+            lp_mask[dst + j] = reg_mask[src + j];
+        }
     }
 }
 
@@ -853,7 +890,7 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
 // ---------------------------------------------------------------------------
 constexpr int kFinalIpt = 16;
 constexpr int kFinalTile = kThreads * kFinalIpt;
-static_assert(kPackTile == kPackTileItems && kBuildTile == kBuildTileItems && kExtractTile == kExtractTileItems &&
+static_assert(kBuildTile == kBuildTileItems && kExtractTile == kExtractTileItems &&
                   kFinalTile == kFinalTileItems,
               "tile sizes shared with the scratch planner");
 
@@ -1038,27 +1075,33 @@ void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets
 }
 
 template <typename K>
-void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
-                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
-                      uint64_t code_mask)
+void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32_t group_bits, uint64_t* reg_code,
+                        uint64_t* reg_mask, uint64_t* lp_code, uint64_t* lp_mask, uint64_t lp_capacity,
+                        uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
+                        uint64_t code_mask)
 {
-    const int grid = grid_for((n_bound + kPackTile - 1) / kPackTile, 8);
+    const uint64_t chunks = (n_bound + kReduceChunk - 1) / kReduceChunk;
+    const int grid = grid_for((chunks + kReduceWaves - 1) / kReduceWaves, 8);
     {
-        TimedLaunch t(kKLeafCount, s);
-        hipLaunchKernelGGL(k_leaf_count<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_counts,
-                           &c->pre[leaf_level]);
+        TimedLaunch t(kKLeafReduce, s);
+        hipLaunchKernelGGL(k_leaf_reduce<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, group_bits,
+                           reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
+                           reinterpret_cast<unsigned long long*>(&c->error));
     }
-    launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kPackTile, &c->leaf_positions);
+    launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kReduceChunk, &c->leaf_positions);
     {
-        TimedLaunch t(kKLeafPack, s);
-        hipLaunchKernelGGL(k_leaf_pack<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, tile_offsets,
-                           lp_code, lp_mask);
+        TimedLaunch t(kKLeafCompact, s);
+        hipLaunchKernelGGL(k_leaf_compact, dim3(grid), dim3(kThreads), 0, s, reg_code, reg_mask, tile_counts,
+                           tile_offsets, &c->n, lp_code, lp_mask, lp_capacity,
+                           reinterpret_cast<unsigned long long*>(&c->error));
     }
 }
-template void launch_leaf_pack<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint64_t*, uint64_t*,
-                                         uint32_t*, uint64_t*, uint64_t, int, uint64_t);
-template void launch_leaf_pack<uint32_t>(hipStream_t, const uint32_t*, BuildCounters*, uint64_t*, uint64_t*,
-                                         uint32_t*, uint64_t*, uint64_t, int, uint64_t);
+template void launch_leaf_reduce<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint32_t, uint64_t*,
+                                           uint64_t*, uint64_t*, uint64_t*, uint64_t, uint32_t*, uint64_t*, uint64_t,
+                                           int, uint64_t);
+template void launch_leaf_reduce<uint32_t>(hipStream_t, const uint32_t*, BuildCounters*, uint32_t, uint64_t*,
+                                           uint64_t*, uint64_t*, uint64_t*, uint64_t, uint32_t*, uint64_t*, uint64_t,
+                                           int, uint64_t);
 
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound)

@@ -24,8 +24,8 @@ enum KernelId : int {
     kKSortScatterPairs,
     kKFill,
     kKScanCounts,
-    kKLeafCount,
-    kKLeafPack,
+    kKLeafReduce,
+    kKLeafCompact,
     kKTableClear,
     kKLeafInsert,
     kKRankOfSlot,
@@ -143,10 +143,13 @@ struct BuildCounters {
 
 constexpr uint64_t kErrTableOverflow = 1;
 constexpr uint64_t kErrKeyRange = 2;
+constexpr uint64_t kErrInternal = 4;  // a bound the kernels rely on did not hold
 
 // Tile sizes of the compaction stages; a stage over at most n items needs
 // tiles(n, kXxxTile) u32 counts and u64 offsets of scratch.
-constexpr uint64_t kPackTileItems = 4096;     // sorted keys per leaf-pack tile
+constexpr uint64_t kPackTileItems = 4096;     // grouped keys per leaf-reduce chunk
+constexpr uint32_t kLeafGroupBits = 9;        // low leaf-code bits the key sort leaves unsorted
+constexpr uint64_t kLeafRegionItems = kPackTileItems + (1u << kLeafGroupBits);  // leaves one chunk may emit
 constexpr uint64_t kBuildTileItems = 2048;    // child codes per level-build tile
 constexpr uint64_t kExtractTileItems = 512;   // table slots per extract tile
 constexpr uint64_t kFinalTileItems = 4096;    // nodes per final-layout tile
@@ -162,10 +165,16 @@ void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets
                         uint64_t per_tile, uint64_t* total_out);
 
 
+// Leaf ReduceByKey over keys grouped by code >> (6 + group_bits): per-chunk
+// leaves into the regions reg_code / reg_mask (kLeafRegionItems per chunk of
+// kPackTileItems keys), then compacted into lp_code / lp_mask (lp_capacity
+// entries); sets
+// c->leaf_positions and adds the leaf level's parent count to c->pre[leaf_level].
 template <typename K>
-void launch_leaf_pack(hipStream_t s, const K* sorted, BuildCounters* c, uint64_t* lp_code, uint64_t* lp_mask,
-                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
-                      uint64_t code_mask);
+void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32_t group_bits, uint64_t* reg_code,
+                        uint64_t* reg_mask, uint64_t* lp_code, uint64_t* lp_mask, uint64_t lp_capacity,
+                        uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
+                        uint64_t code_mask);
 
 // Clears an ordered (load_shift, overflow) or hashed table sized from *count_ptr
 // and records its log2 size.

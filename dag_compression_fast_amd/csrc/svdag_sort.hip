@@ -430,11 +430,6 @@ int device_cu_count()
     return cus;
 }
 
-#ifndef SVDAG_SCATTER_WIDE
-#define SVDAG_SCATTER_WIDE 0
-#endif
-constexpr bool kScatterWide = SVDAG_SCATTER_WIDE != 0;
-
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
@@ -454,11 +449,6 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
         TimedLaunch t(kKSortScatterPairs, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
                            src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
-    } else if constexpr (kScatterWide && sizeof(K) == 8 && IPT == kSortIptSmall) {
-        // The same 4096-key tile on 512 threads x 8 keys.
-        TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT / 2, 2 * kSortThreads>), dim3(grid), dim3(2 * kSortThreads),
-                           0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, false, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
