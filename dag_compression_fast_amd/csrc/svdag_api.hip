@@ -255,7 +255,7 @@ struct BuildLayout {
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
     uint64_t reg_code = 0, reg_mask = 0;  // leaf-reduce regions, over the later stages' buffers
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
-    uint64_t codes[2] = {}, slots[2] = {}, masks = 0, children = 0, tail_hash = 0;
+    uint64_t codes[2] = {}, slots[2] = {}, hashes[2] = {}, masks = 0, children = 0, tail_hash = 0;
     uint64_t tkeys[2] = {}, trep[2] = {}, trank[2] = {};
     uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
     uint32_t group_bits = 0;
@@ -308,6 +308,7 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     for (int i = 0; i < 2; ++i) {
         L.codes[i] = c.take(8 * L.level_bound);
         L.slots[i] = c.take(4 * L.level_bound);
+        L.hashes[i] = c.take(8 * L.level_bound);
     }
     L.masks = c.take(L.level_bound);
     L.children = c.take(32 * L.level_bound);
@@ -556,20 +557,25 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     for (int l = depth - 3; l > tail_top; --l) {
         const int step = depth - 3 - l;
         const int cur = step & 1;
+        // Children: their hashes in child order (a leaf's hash is its mask; the
+        // level below wrote its nodes' hashes) and their table slots, which the
+        // extract turns into ranks.
         LevelBuildArgs a{};
+        const uint32_t* child_rank_of_slot;
         if (l == depth - 3) {
             a.child_codes = lp_code;
             a.child_slots = lp_slot;
-            a.child_rank_of_slot = leaf_rank;
-            a.child_hashes = leaves_out;
+            a.child_hash_seq = lp_mask;
             a.child_count = &c->leaf_positions;
+            child_rank_of_slot = leaf_rank;
         } else {
             a.child_codes = reinterpret_cast<const uint64_t*>(A + L.codes[cur ^ 1]);
             a.child_slots = reinterpret_cast<const uint32_t*>(A + L.slots[cur ^ 1]);
-            a.child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur ^ 1]);
-            a.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[l + 1]);
+            a.child_hash_seq = reinterpret_cast<const uint64_t*>(A + L.hashes[cur ^ 1]);
             a.child_count = &c->pre[l + 1];
+            child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur ^ 1]);
         }
+        a.parent_hashes = reinterpret_cast<uint64_t*>(A + L.hashes[cur]);
         a.parent_codes = reinterpret_cast<uint64_t*>(A + L.codes[cur]);
         a.parent_slots = reinterpret_cast<uint32_t*>(A + L.slots[cur]);
         a.parent_masks = reinterpret_cast<uint8_t*>(A + L.masks);
@@ -593,6 +599,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.rank_of_slot = reinterpret_cast<uint32_t*>(A + L.trank[cur]);
         e.parent_masks = a.parent_masks;
         e.parent_children = a.parent_children;
+        e.child_rank_of_slot = child_rank_of_slot;
         e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
         e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
         e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);

@@ -85,9 +85,33 @@ constexpr uint32_t kReduceChunk = uint32_t(kPackTileItems);
 constexpr uint32_t kReduceTable = 1u << kLeafGroupBits;
 constexpr uint32_t kReduceRegion = uint32_t(kLeafRegionItems);
 static_assert(kReduceRegion == kReduceChunk + kReduceTable, "region bound");
-constexpr int kReduceGroups = 8;  // groups of 64 keys loaded together
+constexpr int kReduceGroups = 4;  // groups of 64 keys loaded together
 constexpr int kReduceWaves = kThreads / kWave;
 static_assert(kReduceTable == 8 * kWave, "one lane holds the 8 leaves of one parent");
+
+// s_waitcnt operand (gfx9 encoding) that waits for vmcnt 0 and leaves expcnt
+// and lgkmcnt at their maxima.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// Value of lane l (a wave-uniform l), as a scalar.
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
+{
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), int(l)));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), int(l)));
+    return uint64_t(hi) << 32 | lo;
+}
+
+// kReduceGroups groups of 64 raw keys from `base`; indices past the end read
+// the last key (n >= 1), and the caller ignores those lanes.  No use of the
+// values here, so the loads stay in flight until the keys are needed.
+template <typename K>
+__device__ __forceinline__ void load_groups(const K* __restrict__ sorted, uint64_t base, uint64_t n,
+                                            uint64_t (&k)[kReduceGroups])
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int g = 0; g < kReduceGroups; ++g) k[g] = uint64_t(sorted[min(base + g * kWave + lane, n - 1)]);
+}
 
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ sorted,
@@ -101,10 +125,12 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
     __shared__ uint64_t s_tab[kReduceWaves][kReduceTable];
     __shared__ uint64_t s_parents[kReduceWaves];
     const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x / kWave;
+    // The wave index as a scalar, so that the chunk loop and the group
+    // bookkeeping below compile to scalar control flow.
+    const uint32_t w = uint32_t(__builtin_amdgcn_readfirstlane(int(threadIdx.x / kWave)));
     uint64_t* const tab = s_tab[w];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) tab[lane * 8 + e] = 0ull;
+    for (int e = 0; e < 8; ++e) tab[e * kWave + lane] = 0ull;
     __builtin_amdgcn_wave_barrier();
     const uint64_t n = *n_ptr;
     const uint64_t nchunks = (n + kReduceChunk - 1) / kReduceChunk;
@@ -126,89 +152,129 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
         uint64_t cur_b = 0;
         bool done = false;
 
-        // Writes the open group's leaves in order and clears the table.  Lane l
-        // holds leaves 8l..8l+7, i.e. the children of one parent.
+        // Writes the open group's leaves in order and clears the table.  Row e
+        // of the table is read by lanes 0..63 (leaves 64e + lane), so a leaf's
+        // output position is the count of nonzero rows' ballots before it;
+        // the 8 leaves of one parent are 8 consecutive lanes of one row.
         auto flush = [&]() {
             __builtin_amdgcn_wave_barrier();
-            uint64_t m[8];
-            uint32_t cnt = 0;
+            uint64_t m[8], B[8];
+            uint32_t total = 0;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                m[e] = tab[lane * 8 + e];
-                cnt += m[e] != 0ull ? 1u : 0u;
+                m[e] = tab[e * kWave + lane];
+                B[e] = __ballot(m[e] != 0ull);
+                total += uint32_t(__popcll(B[e]));
+                uint64_t x = B[e];
+                x |= x >> 4;
+                x |= x >> 2;
+                x |= x >> 1;
+                parents += uint64_t(__popcll(x & 0x0101010101010101ull));
             }
 #pragma unroll
-            for (int e = 0; e < 8; ++e) tab[lane * 8 + e] = 0ull;
+            for (int e = 0; e < 8; ++e)
+                if (m[e]) tab[e * kWave + lane] = 0ull;
             __builtin_amdgcn_wave_barrier();
-            const uint32_t incl = wave_inclusive_scan(cnt);
-            const uint32_t total = __shfl(incl, kWave - 1, kWave);
             if (written + total > kReduceRegion) {
                 overflow = true;
             } else {
-                uint32_t pos = written + incl - cnt;
+                uint32_t pos = written;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     if (m[e]) {
-                        oc[pos] = (cur_b << r) | uint64_t(lane * 8 + e);
-                        om[pos] = m[e];
-                        ++pos;
+                        const uint32_t p = pos + uint32_t(__popcll(B[e] & lanemask_lt()));
+                        oc[p] = (cur_b << r) | uint64_t(e * kWave + lane);
+                        om[p] = m[e];
                     }
+                    pos += uint32_t(__popcll(B[e]));
                 }
                 written += total;
             }
-            parents += uint64_t(__popcll(__ballot(cnt != 0)));
             open = false;
         };
 
+        uint64_t k[kReduceGroups], nxt[kReduceGroups];
+        load_groups(sorted, c0, n, k);
+        // No load may be pending at the loop head (see the wait at its end).
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+#pragma unroll
+        for (int g = 0; g < kReduceGroups; ++g) k[g] &= code_mask;
         for (uint64_t base = c0; !done; base += uint64_t(kReduceGroups) * kWave) {
-            uint64_t k[kReduceGroups];
-            if (base + uint64_t(kReduceGroups) * kWave <= n) {
+            // The next batch of the chunk is in flight while this one is reduced.
+            const uint64_t next = base + uint64_t(kReduceGroups) * kWave;
+            const bool prefetch = next < c1;
+            if (prefetch) load_groups(sorted, next, n, nxt);
+            // Group starts of the whole batch first.  Keys are sorted by group, so
+            // a batch group whose first and last lanes agree has no start inside
+            // it (lane 0 may still start one); only the others need the lane
+            // shuffle.  Lanes past the end repeat the last key: they never start
+            // a group, and OR-ing their bit again is harmless.
+            uint64_t H[kReduceGroups];
 #pragma unroll
-                for (int g = 0; g < kReduceGroups; ++g) k[g] = uint64_t(sorted[base + g * kWave + lane]) & code_mask;
-            } else {
-#pragma unroll
-                for (int g = 0; g < kReduceGroups; ++g) {
-                    const uint64_t i = base + g * kWave + lane;
-                    k[g] = i < n ? uint64_t(sorted[i]) & code_mask : 0ull;
+            for (int g = 0; g < kReduceGroups; ++g) {
+                const uint64_t b = k[g] >> sh;
+                const uint64_t b0 = readlane64(b, 0), b63 = readlane64(b, kWave - 1);
+                if (b0 == b63) {
+                    H[g] = b0 != prev_b ? 1ull : 0ull;
+                } else {
+                    uint64_t pb = __shfl_up(b, 1, kWave);
+                    if (lane == 0) pb = prev_b;
+                    H[g] = __ballot(b != pb);
                 }
+                prev_b = b63;
             }
-            // Groups are processed from k[0]; the rest shift down after each.
-#pragma unroll 1
-            for (int g = 0; g < kReduceGroups && !done; ++g) {
-                const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
-                const uint64_t key = k[0];
 #pragma unroll
-                for (int j = 0; j + 1 < kReduceGroups; ++j) k[j] = k[j + 1];
-                const bool valid = gi + lane < n;
-                const uint64_t b = key >> sh;
-                uint64_t pb = __shfl_up(b, 1, kWave);
-                if (lane == 0) pb = prev_b;
-                prev_b = __shfl(b, kWave - 1, kWave);
-                const uint64_t V = __ballot(valid);
-                uint64_t heads = __ballot(valid && b != pb);  // first keys of block groups
-                // Segments [lo, next head): lanes before the first head continue the
-                // open group, or a group owned by an earlier wave when none is open.
+            for (int g = 0; g < kReduceGroups; ++g) {
+                if (done) break;
+                const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
+                const uint64_t key = k[g];
+                uint64_t* const word = &tab[uint32_t(key >> 6) & low_mask];
+                const uint64_t bit = 1ull << (key & 63);
+                uint64_t heads = H[g];
+                const bool keys_end = gi + kWave >= n;  // the last key is in this batch group
+                if (heads == 0ull) {
+                    // Every lane continues the open group, or one owned by an earlier wave.
+                    if (open) atomicOr((unsigned long long*)word, bit);
+                    if (keys_end) {
+                        if (open) flush();
+                        done = true;
+                    } else if (!open && gi + kWave >= c1) {
+                        done = true;  // no group starts in the chunk
+                    }
+                    continue;
+                }
+                // Segments [lo, next start): lanes before the first start continue
+                // the open group, or one owned by an earlier wave when none is open.
                 uint32_t lo = 0;
                 while (true) {
                     const uint32_t hi = heads ? uint32_t(__ffsll((long long)heads) - 1) : uint32_t(kWave);
-                    if (open && valid && lane >= lo && lane < hi)
-                        atomicOr((unsigned long long*)&tab[uint32_t(key >> 6) & low_mask], 1ull << (key & 63));
-                    const bool keys_end = hi == uint32_t(kWave) && V != ~0ull;
-                    if (hi == uint32_t(kWave) && !keys_end) {
-                        if (!open && gi + kWave >= c1) done = true;  // no group starts in the chunk
+                    if (open && lane >= lo && lane < hi) atomicOr((unsigned long long*)word, bit);
+                    if (hi == uint32_t(kWave)) {
+                        if (keys_end) {
+                            if (open) flush();
+                            done = true;
+                        }
                         break;
                     }
                     if (open) flush();
-                    if (keys_end || gi + hi >= c1) {
+                    if (gi + hi >= c1) {
                         done = true;
                         break;
                     }
                     open = true;
-                    cur_b = __shfl(b, hi, kWave);
+                    cur_b = readlane64(key >> sh, hi);
                     heads &= heads - 1ull;
                     lo = hi;
                 }
             }
+            if (done) break;
+            if (!prefetch) load_groups(sorted, next, n, nxt);
+            // Settle every load here (vmcnt 0, other counters left alone): the
+            // next batch's keys are then complete at the loop head, so the
+            // batch's first uses do not wait behind the prefetch issued there.
+            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+#pragma unroll
+            for (int g = 0; g < kReduceGroups; ++g) k[g] = nxt[g] & code_mask;
         }
         if (lane == 0) counts[chunk] = written;
     }
@@ -531,12 +597,14 @@ __device__ __forceinline__ uint64_t acc_emit(const ParentAcc& a, uint64_t p, con
     uint4* dst = reinterpret_cast<uint4*>(g.parent_children + 8 * p);
     dst[0] = make_uint4(a.child[0], a.child[1], a.child[2], a.child[3]);
     dst[1] = make_uint4(a.child[4], a.child[5], a.child[6], a.child[7]);
-    return hash_children(a.hash);
+    const uint64_t h = hash_children(a.hash);
+    if (g.parent_hashes) g.parent_hashes[p] = h;
+    return h;
 }
 
-// A thread owns kBuildIpt consecutive children; their rank and hash gathers are
-// issued together.  The tile's parent hashes are staged in LDS and inserted into
-// the level's table one thread per parent.
+// A thread owns kBuildIpt consecutive children; their loads (or rank and hash
+// gathers) are issued together.  The tile's parent hashes are staged in LDS and
+// inserted into the level's table one thread per parent.
 __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
@@ -548,6 +616,7 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
     const uint32_t* __restrict__ slots = g.child_slots;
     const uint32_t* __restrict__ rank_of_slot = g.child_rank_of_slot;
     const uint64_t* __restrict__ hashes = g.child_hashes;
+    const uint64_t* __restrict__ seq = g.child_hash_seq;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t first = tile * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt;
         uint64_t code[kBuildIpt];
@@ -559,12 +628,21 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
 
         // Indices are clamped into [0, n): loads past the end read a valid child
         // that is then ignored.
+        // rank[] holds the child pointers: slots with sequential hashes, else ranks.
         uint32_t rank[kBuildIpt];
         uint64_t hsh[kBuildIpt];
+        if (seq) {
 #pragma unroll
-        for (int j = 0; j < kBuildIpt; ++j) rank[j] = rank_of_slot[slots[min(first + j, n - 1)]];
+            for (int j = 0; j < kBuildIpt; ++j) {
+                rank[j] = slots[min(first + j, n - 1)];
+                hsh[j] = seq[min(first + j, n - 1)];
+            }
+        } else {
 #pragma unroll
-        for (int j = 0; j < kBuildIpt; ++j) hsh[j] = hashes[rank[j]];
+            for (int j = 0; j < kBuildIpt; ++j) rank[j] = rank_of_slot[slots[min(first + j, n - 1)]];
+#pragma unroll
+            for (int j = 0; j < kBuildIpt; ++j) hsh[j] = hashes[rank[j]];
+        }
 
         uint32_t q = local;
         bool open = false;
@@ -587,8 +665,12 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
             for (uint64_t i = first + kBuildIpt; i < n; ++i) {
                 const uint64_t cc = g.child_codes[i];
                 if ((cc >> 3) != acc.code) break;
-                const uint32_t r = rank_of_slot[slots[i]];
-                acc_add(acc, uint32_t(cc & 7), r, hashes[r]);
+                if (seq) {
+                    acc_add(acc, uint32_t(cc & 7), slots[i], seq[i]);
+                } else {
+                    const uint32_t r = rank_of_slot[slots[i]];
+                    acc_add(acc, uint32_t(cc & 7), r, hashes[r]);
+                }
             }
             s_hash[q] = acc_emit(acc, p0 + q, g);
         }
@@ -855,6 +937,19 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
             const uint4* src = reinterpret_cast<const uint4*>(g.parent_children + 8 * rep);
             ch0[j] = src[0];
             ch1[j] = src[1];
+        }
+        if (g.child_rank_of_slot) {
+            // Child slots of the level below -> its ranks (occupied slots only:
+            // the others hold parent 0's record, which may be unset).
+            const uint32_t* __restrict__ rk = g.child_rank_of_slot;
+            auto tr = [&](uint32_t v) { return v == kEmptyChild ? v : rk[v]; };
+#pragma unroll
+            for (int j = 0; j < kExtractIpt; ++j) {
+                if ((occ >> j) & 1u) {
+                    ch0[j] = make_uint4(tr(ch0[j].x), tr(ch0[j].y), tr(ch0[j].z), tr(ch0[j].w));
+                    ch1[j] = make_uint4(tr(ch1[j].x), tr(ch1[j].y), tr(ch1[j].z), tr(ch1[j].w));
+                }
+            }
         }
 #pragma unroll
         for (int j = 0; j < kExtractIpt; ++j) {

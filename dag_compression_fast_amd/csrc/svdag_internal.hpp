@@ -198,12 +198,18 @@ void launch_sketch_estimate(hipStream_t s, const uint32_t* regs, const uint64_t*
 void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint64_t* count_ptr,
                          uint32_t* rank_of_slot, uint64_t n_bound);
 
+// Children come in one of two forms.  With child_hash_seq, child i's hash is
+// child_hash_seq[i] and its table slot is recorded as the pointer (the level's
+// extract turns slots into ranks); otherwise its rank is
+// child_rank_of_slot[child_slots[i]] and its hash child_hashes[rank].
 struct LevelBuildArgs {
     const uint64_t* child_codes;
     const uint32_t* child_slots;
+    const uint64_t* child_hash_seq;
     const uint32_t* child_rank_of_slot;
     const uint64_t* child_hashes;
     const uint64_t* child_count;
+    uint64_t* parent_hashes;  // hash of parent p at p, or null
     uint64_t* parent_codes;
     uint32_t* parent_slots;
     uint8_t* parent_masks;
@@ -257,6 +263,7 @@ struct LevelExtractArgs {
     uint32_t* rank_of_slot;
     const uint8_t* parent_masks;
     const uint32_t* parent_children;
+    const uint32_t* child_rank_of_slot;  // non-null: parent_children hold child slots, written as ranks
     uint64_t* out_hashes;
     uint8_t* out_masks;
     uint32_t* out_children;

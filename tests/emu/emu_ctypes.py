@@ -6,6 +6,7 @@ ctypes layer to it, so host logic and kernel logic can be exercised without a
 GPU.  Nothing in the product imports this module.
 """
 import ctypes
+import fcntl
 import os
 import subprocess
 
@@ -14,7 +15,11 @@ EMU_LIB = os.environ.get('SVDAG_EMU_LIB', os.path.join(HERE, '_build', 'libsvdag
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    # Test workers (pytest -n) may call this at once: one make at a time.
+    os.makedirs(os.path.join(HERE, '_build'), exist_ok=True)
+    with open(os.path.join(HERE, '_build', '.lock'), 'w') as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
 def use_emulator():

@@ -54,6 +54,7 @@ inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 #define __builtin_amdgcn_s_sleep(x) emu::yield_sleep()
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
+#define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
 
 template <class T>
 T __shfl_up(T v, unsigned d, int = 64)
@@ -86,6 +87,10 @@ T __shfl(T v, int src, int = 64)
     std::memcpy(&out, &bits, sizeof(T));
     return out;
 }
+// Lane reads of wave-uniform calls (every lane of the wave takes part here).
+inline int emu_readlane(int v, int src) { return int(emu::exchange((unsigned long long)(unsigned)v, src & 63)); }
+#define __builtin_amdgcn_readfirstlane(x) emu_readlane(int(x), 0)
+#define __builtin_amdgcn_readlane(x, l) emu_readlane(int(x), int(l))
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline int __clzll(long long v) { return v ? __builtin_clzll((unsigned long long)v) : 64; }
 template <class T>
