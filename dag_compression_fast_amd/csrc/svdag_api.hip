@@ -143,6 +143,10 @@ struct svdag_context {
     int num_marks = 0;
     double wait_seconds = 120.0;
     EventRecorder recorder;
+    // Second stream (default priority) for work that may overlap the main
+    // chain; it forks from and joins back to `stream` through the two events.
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 namespace {
@@ -173,6 +177,31 @@ int set_error(int code, const char* fmt, ...)
     } while (0)
 
 // Records a named marker after the work enqueued so far.
+// Builds of at least this many keys order their unique leaves on the side stream.
+constexpr uint64_t kOverlapMinKeys = 1ull << 24;
+
+// The side stream and its events, created on first use; false if the runtime
+// refuses them (the work then stays on the main stream).
+bool ensure_side(svdag_context* ctx)
+{
+    if (ctx->side) return true;
+    hipStream_t s = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreateWithFlags(&a, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&b, hipEventDisableTiming) == hipSuccess) {
+        ctx->side = s;
+        ctx->fork_ev = a;
+        ctx->join_ev = b;
+        return true;
+    }
+    if (b) (void)hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipGetLastError();
+    return false;
+}
+
 void mark(svdag_context* ctx, const char* name)
 {
     if (ctx->num_marks >= svdag_context::kMaxMarks || !ctx->marks[ctx->num_marks]) return;
@@ -462,7 +491,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     void* storage = nullptr;
     TRY(device_alloc(ctx, &storage, S.total));
     char* O = static_cast<char*>(storage);
+    bool forked = false;  // work is in flight on the side stream
     auto fail = [&](int code) {
+        if (forked) (void)hipStreamSynchronize(ctx->side);
         device_release(ctx, storage);
         out->storage = nullptr;
         return code;
@@ -539,12 +570,34 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     launch_table_clear(st, leaf_table, leaf_size, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
     launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, lk0, ls0,
                        &c->unique_leaves, err);
+    // The unique leaves are ordered (sorted masks, slot -> rank) on the side
+    // stream for large builds: the level above the leaves needs their ranks
+    // only from its extract on, so its build overlaps this sort.  Small builds
+    // keep one stream (many concurrent contexts would share hardware queues).
     uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
-    if (launch_sort<uint64_t>(st, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
+    hipStream_t ls = st;
+    if (n >= kOverlapMinKeys && ensure_side(ctx) && hipEventRecord(ctx->fork_ev, st) == hipSuccess &&
+        hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0) == hipSuccess) {
+        ls = ctx->side;
+        forked = true;
+    }
+    if (launch_sort<uint64_t>(ls, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
                               &c->unique_leaves, L.leaf_bound, reinterpret_cast<uint32_t*>(A + L.leaf_sort_zero),
                               sort_counts, nullptr) < 0)
         return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
-    launch_rank_of_slot(st, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
+    launch_rank_of_slot(ls, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
+    bool joined = !forked;
+    if (forked && hipEventRecord(ctx->join_ev, ls) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "event record failed"));
+    // Before the first use of the leaf order.
+    auto join = [&]() -> int {
+        if (joined) return 0;
+        joined = true;
+        if (hipStreamWaitEvent(st, ctx->join_ev, 0) != hipSuccess)
+            return fail(set_error(SVDAG_ERR_HIP, "stream wait failed"));
+        mark(ctx, "leaf order");
+        return 0;
+    };
     mark(ctx, "leaf dedup");
 
     // Levels, bottom-up (DAGCompression.cu:1077-1092): hashed-table levels down
@@ -608,6 +661,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.slots_scratch = &c->table_slots;
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
+        TRY(join());
         launch_level_extract(st, e, L.level_slots);
         mark(ctx, "level extract");
         if (l - 1 > tail_top)
@@ -646,6 +700,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         }
         t.counters = c;
         t.check_collisions = ctx->check_collisions;
+        TRY(join());
         launch_level_tail(st, t);
         mark(ctx, "top levels");
     }
@@ -963,6 +1018,12 @@ int svdag_context_destroy(svdag_context* ctx)
     if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
     for (auto& e : ctx->marks)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipEventDestroy(ctx->fork_ev);
+        (void)hipEventDestroy(ctx->join_ev);
+        (void)hipStreamDestroy(ctx->side);
+    }
     if (active_recorder() == &ctx->recorder) active_recorder() = nullptr;
     delete ctx;
     return 0;

@@ -74,6 +74,7 @@ constexpr int kMaxPasses = 8;
 // A tile is 256 threads x ipt keys: 16 (4096 keys; key/value sorts) or 32
 // (8192 keys; keys-only sorts).
 constexpr int kSortThreads = 256;
+constexpr int kSortIptPairs = 8;   // u64 key/value sorts
 constexpr int kSortIptSmall = 16;
 constexpr int kSortIptLarge = 32;
 
@@ -97,7 +98,7 @@ inline uint64_t sort_tiles(uint64_t n, int ipt)
 // u32, zero before launch_sort) and the per-tile digit counts
 // (sort_count_words u32 for any tile size, any contents).
 inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) * kRadix; }
-inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound, kSortIptSmall); }
+inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound, kSortIptPairs); }
 
 // LSD radix sort of keys (and optionally u32 values) on the plan's bits; keys
 // equal on those bits keep their input order.

@@ -464,8 +464,9 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
 {
     if (plan.passes == 0 || n_bound == 0) return -1;
     const bool pairs = vals_in != nullptr;
-    // Key/value tiles stay at 4096 keys (their LDS would exceed 64 KB at 8192).
-    const int ipt = pairs ? kSortIptSmall : plan.ipt;
+    // Key/value sorts (the unique-leaf order: a few hundred thousand keys) use
+    // 2048-key tiles for u64 keys, so that more workgroups share each pass.
+    const int ipt = pairs ? (sizeof(K) == 8 ? kSortIptPairs : kSortIptSmall) : plan.ipt;
     uint32_t* hist = zero;
     const uint64_t stride = sort_tiles(n_bound, ipt);
     HistPlan hp{};
@@ -475,45 +476,42 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         hp.mask[p] = (1u << plan.bits[p]) - 1u;
     }
     const int cus = device_cu_count();
-    const K* src = keys_in;
-    const uint32_t* vsrc = vals_in;
-    int where = -1;
     // A multiple of the XCD count, as the tile walk assumes.
     const uint32_t grid = uint32_t(
         std::max<uint64_t>(kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
-    {
-        // Also writes the first pass's tile counts.
-        TimedLaunch t(kKHistogram, stream);
-        if (ipt == kSortIptLarge)
-            hipLaunchKernelGGL((k_histogram<K, kSortIptLarge>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
-                               n_ptr, hp, hist, key_or, counts, stride, prefix);
-        else
-            hipLaunchKernelGGL((k_histogram<K, kSortIptSmall>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in,
-                               n_ptr, hp, hist, key_or, counts, stride, prefix);
-    }
-    for (int p = 0; p < plan.passes; ++p) {
-        const bool last = p + 1 == plan.passes;
-        K* dst = (p % 2 == 0) ? keys_a : keys_b;
-        uint32_t* vdst = (p % 2 == 0) ? vals_a : vals_b;
-        where = p % 2;
-        if (last && final_keys) {
-            dst = final_keys;
-            where = 2;
+    auto run = [&](auto ipt_tag) -> int {
+        constexpr int I = decltype(ipt_tag)::value;
+        {
+            // Also writes the first pass's tile counts.
+            TimedLaunch t(kKHistogram, stream);
+            hipLaunchKernelGGL((k_histogram<K, I>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in, n_ptr, hp,
+                               hist, key_or, counts, stride, prefix);
         }
-        if (last && final_vals) vdst = final_vals;
-        const uint32_t dmask = (1u << plan.bits[p]) - 1u;
-        if (ipt == kSortIptLarge)
-            launch_pass<K, kSortIptLarge>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid, p == 0,
-                                          p == 0 ? prefix : ChunkPrefix{});
-        else
-            launch_pass<K, kSortIptSmall>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts,
-                                          hist + p * kRadix, stride, grid, p == 0,
-                                          p == 0 ? prefix : ChunkPrefix{});
-        src = dst;
-        vsrc = vdst;
-    }
-    return where;
+        const K* src = keys_in;
+        const uint32_t* vsrc = vals_in;
+        int where = -1;
+        for (int p = 0; p < plan.passes; ++p) {
+            const bool last = p + 1 == plan.passes;
+            K* dst = (p % 2 == 0) ? keys_a : keys_b;
+            uint32_t* vdst = (p % 2 == 0) ? vals_a : vals_b;
+            where = p % 2;
+            if (last && final_keys) {
+                dst = final_keys;
+                where = 2;
+            }
+            if (last && final_vals) vdst = final_vals;
+            const uint32_t dmask = (1u << plan.bits[p]) - 1u;
+            launch_pass<K, I>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts, hist + p * kRadix,
+                              stride, grid, p == 0, p == 0 ? prefix : ChunkPrefix{});
+            src = dst;
+            vsrc = vdst;
+        }
+        return where;
+    };
+    if (ipt == kSortIptLarge) return run(std::integral_constant<int, kSortIptLarge>{});
+    if constexpr (sizeof(K) == 8)
+        if (ipt == kSortIptPairs) return run(std::integral_constant<int, kSortIptPairs>{});
+    return run(std::integral_constant<int, kSortIptSmall>{});
 }
 
 template int launch_sort<uint64_t>(hipStream_t, const SortPlan&, const uint64_t*, const uint32_t*, uint64_t*,

@@ -338,5 +338,9 @@ hipError_t hipEventSynchronize(void*) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float* ms, void*, void*) { *ms = 0; return hipSuccess; }
 hipError_t hipEventDestroy(void* e) { std::free(e); return hipSuccess; }
 hipError_t hipStreamCreate(void** s) { *s = nullptr; return hipSuccess; }
+// Launches run synchronously in issue order, so streams need no state; a
+// created stream only needs a distinct non-null handle.
+hipError_t hipStreamCreateWithFlags(void** s, unsigned) { *s = std::malloc(8); return hipSuccess; }
+hipError_t hipStreamWaitEvent(void*, void*, unsigned) { return hipSuccess; }
 hipError_t hipStreamDestroy(void*) { return hipSuccess; }
 }

@@ -182,7 +182,10 @@ hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
+hipError_t hipStreamDestroy(hipStream_t s);
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags);
 }
-enum { hipErrorNotReady = 600, hipEventDisableTiming = 2 };
+enum { hipErrorNotReady = 600, hipEventDisableTiming = 2, hipStreamNonBlocking = 1 };
 
 #define hipLaunchKernelGGL(F, G, B, SH, ST, ...) emu::launch(G, B, [=]() { F(__VA_ARGS__); })
