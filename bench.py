@@ -70,8 +70,8 @@ def algorithmic_bytes(kernel, st, key_bytes, launches):
         "leaf_compact": 32 * l0,
         # mask read, slot write, unique list append
         "leaf_insert": 12 * l0 + 12 * lu,
-        # child code + slot + rank + hash reads; parent code, slot, mask, 8 children writes
-        "level_build": sum(24 * child[l] + 45 * pre[l] for l in range(levels)),
+        # child code + slot + hash reads; parent code, slot, mask, 8 children, hash writes
+        "level_build": sum(20 * child[l] + 53 * pre[l] for l in range(levels)),
         # rep + mask + children reads, rank + hash + mask + children writes per unique node
         "level_extract": sum(82 * uniq[l] for l in range(levels)),
         "final_write": sum(33 * uniq[l] for l in range(levels)) + 4 * sum(words) + 16 * lu,
