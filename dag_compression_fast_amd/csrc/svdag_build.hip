@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
 
 __host__ int grid_for(uint64_t tiles, int per_cu)
 {
@@ -56,12 +57,23 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const uint32_t* __
     __shared__ uint64_t s_scan[kScanThreads / kWave];
     const uint64_t ntiles = (*items + per_tile - 1) / per_tile;
     uint64_t running = 0;
-    for (uint64_t chunk = 0; chunk < ntiles; chunk += kScanThreads) {
-        const uint64_t i = chunk + threadIdx.x;
-        const uint64_t v = i < ntiles ? counts[i] : 0;
+    // kScanItems consecutive counts per thread per block scan.
+    for (uint64_t chunk = 0; chunk < ntiles; chunk += uint64_t(kScanThreads) * kScanItems) {
+        const uint64_t i0 = chunk + uint64_t(threadIdx.x) * kScanItems;
+        uint32_t v[kScanItems];
+        uint64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = i0 + k < ntiles ? counts[i0 + k] : 0u;
+            sum += v[k];
+        }
         uint64_t total;
-        const uint64_t ex = block_exclusive_scan64<kScanThreads>(v, s_scan, total);
-        if (i < ntiles) offsets[i] = running + ex;
+        uint64_t run = running + block_exclusive_scan64<kScanThreads>(sum, s_scan, total);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < ntiles) offsets[i0 + k] = run;
+            run += v[k];
+        }
         running += total;
     }
     if (total_out && threadIdx.x == 0) *total_out = running;

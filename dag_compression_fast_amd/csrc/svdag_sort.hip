@@ -215,6 +215,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
 // Block d: exclusive scan of row d of the tile counts, offset by the digit's
 // global start (sum of the pass histogram below d).  In place.
 constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
 
 __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ hist,
@@ -231,12 +232,23 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
     for (uint32_t i = threadIdx.x; i < d; i += kScanThreads) below += hist[i];
     uint64_t running = block_reduce_sum<kScanThreads>(below, s_red);
     uint32_t* row = counts + uint64_t(d) * stride;
-    for (uint64_t chunk = 0; chunk < ntiles; chunk += kScanThreads) {
-        const uint64_t i = chunk + threadIdx.x;
-        const uint32_t v = i < ntiles ? row[i] : 0u;
+    // kScanItems consecutive counts per thread per block scan.
+    for (uint64_t chunk = 0; chunk < ntiles; chunk += uint64_t(kScanThreads) * kScanItems) {
+        const uint64_t i0 = chunk + uint64_t(threadIdx.x) * kScanItems;
+        uint32_t v[kScanItems];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = i0 + k < ntiles ? row[i0 + k] : 0u;
+            sum += v[k];
+        }
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, s_scan, total);
-        if (i < ntiles) row[i] = uint32_t(running + ex);
+        uint64_t run = running + block_exclusive_scan<kScanThreads>(sum, s_scan, total);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < ntiles) row[i0 + k] = uint32_t(run);
+            run += v[k];
+        }
         running += total;
     }
 }
