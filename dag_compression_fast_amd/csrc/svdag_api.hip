@@ -31,7 +31,8 @@ const char* const kKernelNames[kNumKernelIds] = {
     "sort_histogram", "sort_count",  "sort_scan",     "sort_scatter_keys", "sort_scatter_pairs", "fill",
     "scan_counts",    "leaf_reduce", "leaf_compact",      "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
-    "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail"};
+    "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
+    "slot_list"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -310,7 +311,8 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.key_plan = make_sort_plan(int(6 + std::min(L.group_bits, code_bits - 1)), 3 * depth, radix_bits, sort_ipt);
     L.leaf_plan = make_sort_plan(0, 64);
     const uint64_t stage_tiles = std::max({tiles(n, kPackTileItems), tiles(L.leaf_bound, kBuildTileItems),
-                                           tiles(L.level_slots, kExtractTileItems), uint64_t(1)});
+                                           tiles(L.level_slots, kExtractTileItems),
+                                           tiles(L.leaf_slots, kListTileItems), uint64_t(1)});
     Carver c;
     L.zero_begin = c.off;
     L.counters = c.take(sizeof(BuildCounters));
@@ -568,8 +570,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         leaf_size = &c->leaf_estimate;
     }
     launch_table_clear(st, leaf_table, leaf_size, leaf_lg, ctx->load_shift, 0, L.leaf_slots);
-    launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, lk0, ls0,
-                       &c->unique_leaves, err);
+    launch_leaf_insert(st, lp_mask, &c->leaf_positions, leaf_table, leaf_lg, lp_slot, L.leaf_bound, err);
+    launch_slot_list(st, leaf_table, leaf_lg, L.leaf_slots, tile_counts, tile_offsets, &c->table_slots, lk0, ls0,
+                     &c->unique_leaves);
     // The unique leaves are ordered (sorted masks, slot -> rank) on the side
     // stream for large builds: the level above the leaves needs their ranks
     // only from its extract on, so its build overlaps this sort.  Small builds

@@ -368,8 +368,7 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
     return 0;
 }
 
-// Every leaf position's mask enters the table; the masks placed first are
-// appended to the unique list, one list reservation per workgroup round.
+// Every leaf position's mask (never 0) enters the table and records its slot.
 constexpr int kInsertIpt = 4;
 
 __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __restrict__ lp_mask,
@@ -377,49 +376,76 @@ __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __rest
                                                           uint64_t* __restrict__ tk,
                                                           const uint32_t* __restrict__ log2_ptr,
                                                           uint32_t* __restrict__ lp_slot,
-                                                          uint64_t* __restrict__ list_keys,
-                                                          uint32_t* __restrict__ list_slots,
-                                                          uint64_t* __restrict__ list_count,
                                                           unsigned long long* __restrict__ err)
 {
-    __shared__ uint32_t s_scan[kThreads / kWave];
-    __shared__ uint64_t s_base;
     const uint64_t n = *count_ptr;
     const uint32_t lg = *log2_ptr;
     constexpr uint64_t per_round = uint64_t(kThreads) * kInsertIpt;
-    const uint64_t rounds = (n + per_round - 1) / per_round;
-    for (uint64_t r = blockIdx.x; r < rounds; r += gridDim.x) {
-        uint64_t key[kInsertIpt];
-        uint32_t slot[kInsertIpt];
-        uint32_t won = 0;
+    for (uint64_t r0 = uint64_t(blockIdx.x) * per_round + threadIdx.x; r0 < n; r0 += uint64_t(gridDim.x) * per_round) {
 #pragma unroll
         for (int j = 0; j < kInsertIpt; ++j) {
-            const uint64_t q = r * per_round + uint64_t(j) * kThreads + threadIdx.x;
-            key[j] = 0;
-            slot[j] = 0;
+            const uint64_t q = r0 + uint64_t(j) * kThreads;
             if (q < n) {
-                key[j] = lp_mask[q];
-                bool w;
-                slot[j] = hashed_insert(tk, lg, key[j], w, err);
-                lp_slot[q] = slot[j];
-                won |= (w ? 1u : 0u) << j;
+                bool won;
+                lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], won, err);
             }
         }
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kThreads>(uint32_t(__popc(won)), s_scan, total);
-        if (threadIdx.x == 0 && total) s_base = atomicAdd((unsigned long long*)list_count, (unsigned long long)total);
-        __syncthreads();
-        if (won) {
-            uint64_t idx = s_base + ex;
+    }
+}
+
+// Unique list of a hashed table: its occupied slots (key, slot index), by
+// count / scan / write over tiles of kListTile slots.
+constexpr int kListIpt = 16;
+constexpr uint32_t kListTile = kThreads * kListIpt;
+
+__global__ __launch_bounds__(kThreads) void k_slot_count(const uint64_t* __restrict__ tk,
+                                                         const uint32_t* __restrict__ log2_ptr,
+                                                         uint32_t* __restrict__ counts, uint64_t* __restrict__ slots_out)
+{
+    __shared__ uint64_t s_red[kThreads / kWave];
+    const uint64_t total = 1ull << *log2_ptr;
+    const uint64_t ntiles = (total + kListTile - 1) / kListTile;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *slots_out = total;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t occ = 0;
 #pragma unroll
-            for (int j = 0; j < kInsertIpt; ++j)
-                if ((won >> j) & 1u) {
-                    list_keys[idx] = key[j];
-                    list_slots[idx] = slot[j];
-                    ++idx;
-                }
+        for (int j = 0; j < kListIpt; ++j) {
+            const uint64_t i = tile * kListTile + uint64_t(j) * kThreads + threadIdx.x;
+            occ += (i < total && tk[i] != 0) ? 1u : 0u;
         }
-        __syncthreads();  // s_base is rewritten by the next round
+        const uint64_t cnt = block_reduce_sum<kThreads>(occ, s_red);
+        if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_slot_list(const uint64_t* __restrict__ tk,
+                                                        const uint32_t* __restrict__ log2_ptr,
+                                                        const uint64_t* __restrict__ offsets,
+                                                        uint64_t* __restrict__ list_keys,
+                                                        uint32_t* __restrict__ list_slots)
+{
+    __shared__ uint32_t s_scan[kThreads / kWave];
+    const uint64_t total = 1ull << *log2_ptr;
+    const uint64_t ntiles = (total + kListTile - 1) / kListTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t key[kListIpt];
+        uint32_t occ = 0;
+#pragma unroll
+        for (int j = 0; j < kListIpt; ++j) {
+            const uint64_t i = tile * kListTile + uint64_t(j) * kThreads + threadIdx.x;
+            key[j] = i < total ? tk[i] : 0ull;
+            occ |= (key[j] != 0 ? 1u : 0u) << j;
+        }
+        uint32_t unused;
+        uint64_t o = offsets[tile] + block_exclusive_scan<kThreads>(uint32_t(__popc(occ)), s_scan, unused);
+#pragma unroll
+        for (int j = 0; j < kListIpt; ++j) {
+            if ((occ >> j) & 1u) {
+                list_keys[o] = key[j];
+                list_slots[o] = uint32_t(tile * kListTile + uint64_t(j) * kThreads + threadIdx.x);
+                ++o;
+            }
+        }
     }
 }
 
@@ -998,7 +1024,7 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
 constexpr int kFinalIpt = 16;
 constexpr int kFinalTile = kThreads * kFinalIpt;
 static_assert(kBuildTile == kBuildTileItems && kExtractTile == kExtractTileItems &&
-                  kFinalTile == kFinalTileItems,
+                  kFinalTile == kFinalTileItems && kListTile == kListTileItems,
               "tile sizes shared with the scratch planner");
 
 __device__ __forceinline__ int level_of(const FinalArgs& a, uint64_t g)
@@ -1219,13 +1245,29 @@ void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr
 }
 
 void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
-                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
-                        uint32_t* list_slots, uint64_t* list_count, unsigned long long* err)
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err)
 {
     TimedLaunch t(kKLeafInsert, s);
     hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for((n_bound + kThreads * kInsertIpt - 1) / (kThreads * kInsertIpt), 8)),
-                       dim3(kThreads), 0, s,
-                       lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, list_keys, list_slots, list_count, err);
+                       dim3(kThreads), 0, s, lp_mask, count_ptr, table_keys, log2_ptr, lp_slot, err);
+}
+
+void launch_slot_list(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t slot_bound,
+                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t* slots_scratch, uint64_t* list_keys,
+                      uint32_t* list_slots, uint64_t* list_count)
+{
+    const int grid = grid_for((slot_bound + kListTile - 1) / kListTile, 8);
+    {
+        TimedLaunch t(kKSlotList, s);
+        hipLaunchKernelGGL(k_slot_count, dim3(grid), dim3(kThreads), 0, s, table_keys, log2_ptr, tile_counts,
+                           slots_scratch);
+    }
+    launch_scan_counts(s, tile_counts, tile_offsets, slots_scratch, kListTile, list_count);
+    {
+        TimedLaunch t(kKSlotList, s);
+        hipLaunchKernelGGL(k_slot_list, dim3(grid), dim3(kThreads), 0, s, table_keys, log2_ptr, tile_offsets,
+                           list_keys, list_slots);
+    }
 }
 
 void launch_sketch(hipStream_t s, const uint64_t* values, const uint64_t* count_ptr, uint32_t* regs, uint64_t n_bound)

@@ -40,6 +40,7 @@ enum KernelId : int {
     kKMergeInsert,
     kKSketch,
     kKLevelTail,
+    kKSlotList,
     kNumKernelIds
 };
 
@@ -154,6 +155,7 @@ constexpr uint64_t kLeafRegionItems = kPackTileItems + (1u << kLeafGroupBits);  
 constexpr uint64_t kBuildTileItems = 2048;    // child codes per level-build tile
 constexpr uint64_t kExtractTileItems = 512;   // table slots per extract tile
 constexpr uint64_t kFinalTileItems = 4096;    // nodes per final-layout tile
+constexpr uint64_t kListTileItems = 4096;     // table slots per slot-list tile
 inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
 
 int device_cu_count();
@@ -182,11 +184,17 @@ void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound);
 
-// Inserts every leaf mask into the hashed table; the first inserter of a mask
-// appends (mask, slot) to the unique list at atomicAdd(list_count).
+// Inserts every leaf mask into the hashed table and records its slot.
 void launch_leaf_insert(hipStream_t s, const uint64_t* lp_mask, const uint64_t* count_ptr, uint64_t* table_keys,
-                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, uint64_t* list_keys,
-                        uint32_t* list_slots, uint64_t* list_count, unsigned long long* err);
+                        const uint32_t* log2_ptr, uint32_t* lp_slot, uint64_t n_bound, unsigned long long* err);
+
+// The occupied slots of a hashed table (2^*log2_ptr slots, at most slot_bound)
+// as a list (key, slot index); their number goes to *list_count.  tile_counts
+// and tile_offsets need tiles(slot_bound, 4096) entries; *slots_scratch
+// receives the slot count.
+void launch_slot_list(hipStream_t s, const uint64_t* table_keys, const uint32_t* log2_ptr, uint64_t slot_bound,
+                      uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t* slots_scratch, uint64_t* list_keys,
+                      uint32_t* list_slots, uint64_t* list_count);
 
 // Distinct-count sketch (HyperLogLog, kSketchRegs registers, zero before the
 // first launch) of values[0, *count_ptr), and the count a hashed table is sized
