@@ -101,10 +101,6 @@ constexpr int kReduceGroups = 4;  // groups of 64 keys loaded together
 constexpr int kReduceWaves = kThreads / kWave;
 static_assert(kReduceTable == 8 * kWave, "one lane holds the 8 leaves of one parent");
 
-// s_waitcnt operand (gfx9 encoding) that waits for vmcnt 0 and leaves expcnt
-// and lgkmcnt at their maxima.
-constexpr int kWaitVmcnt0 = 0x0F70;
-
 // Value of lane l (a wave-uniform l), as a scalar.
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
 {

@@ -12,6 +12,11 @@ namespace svdag {
 constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
 constexpr int kWave = 64;
 
+// s_waitcnt operand (gfx9 encoding) that waits for vmcnt 0 and leaves expcnt
+// and lgkmcnt at their maxima.  Placed where a loop consumes prefetched loads,
+// so that no load is pending at the loop head.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 __device__ __forceinline__ uint64_t atomic_load_relaxed(const uint64_t* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

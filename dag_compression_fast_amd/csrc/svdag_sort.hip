@@ -119,25 +119,30 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
     const uint32_t radix0 = plan.mask[0] + 1;
     const TileWalk walk = xcd_tiles(ntiles);
     K acc = 0;
+    // Keys of tile t; indices past the end read the last key (ignored below),
+    // so the loads need no guard.
+    auto load = [&](uint64_t t, K (&v)[IPT]) {
+        const uint64_t b = t * kTile;
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) v[i] = keys[min(b + uint64_t(i) * kSortThreads + threadIdx.x, n - 1)];
+    };
+    K k[IPT], nxt[IPT];
+    if (walk.tile < walk.end) load(walk.tile, k);
+    // No load may be pending at the loop head (see the wait at its end).
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
         for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) s_cnt[d] = 0;
         __syncthreads();
         const uint64_t base = tile * kTile;
         const uint32_t cnt = uint32_t(min(kTile, n - base));
-        K k[IPT];
-        if (cnt == kTile) {
-#pragma unroll
-            for (int i = 0; i < IPT; ++i) k[i] = keys[base + uint64_t(i) * kSortThreads + threadIdx.x];
-        } else {
-#pragma unroll
-            for (int i = 0; i < IPT; ++i) {
-                const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
-                k[i] = idx < cnt ? keys[base + idx] : K(0);
-            }
-        }
+        // The tile's chunks are looked up before the prefetch below, whose
+        // loads would otherwise be waited for with the lookup's.
+        const TileChunks tc = prefix.offsets ? tile_chunks(prefix, base, cnt) : TileChunks{0, 0};
+        // The next tile's keys are in flight while this one is counted.
+        const bool more = tile + walk.step < walk.end;
+        if (more) load(tile + walk.step, nxt);
         if (prefix.offsets) {
             // Chunk-local keys: range check, then the chunk index above them.
-            const TileChunks tc = tile_chunks(prefix, base, cnt);
             bool bad = false;
 #pragma unroll
             for (int i = 0; i < IPT; ++i) {
@@ -165,6 +170,11 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
             s_hist[0][d] += c;
         }
         __syncthreads();
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) k[i] = nxt[i];
+        }
     }
     for (int p = 0; p < plan.passes; ++p)
         for (uint32_t d = threadIdx.x; d <= plan.mask[p]; d += kSortThreads) {
@@ -193,22 +203,36 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
     const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix = digit_mask + 1;
     const TileWalk walk = xcd_tiles(ntiles);
+    // Keys of tile t; indices past the end read the last key (counted as
+    // invalid below), so the loads need no guard.
+    auto load = [&](uint64_t t, K (&k)[IPT]) {
+        const uint64_t base = t * kTile;
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) k[i] = keys[min(base + uint64_t(i) * kSortThreads + threadIdx.x, n - 1)];
+    };
+    K cur[IPT], nxt[IPT];
+    if (walk.tile < walk.end) load(walk.tile, cur);
+    // No load may be pending at the loop head (see the wait at its end).
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
         for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) s_cnt[d] = 0;
         __syncthreads();
+        // The next tile's keys are in flight while this one is counted.
+        const bool more = tile + walk.step < walk.end;
+        if (more) load(tile + walk.step, nxt);
         const uint64_t base = tile * kTile;
-        uint32_t dg[IPT];
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            const uint64_t idx = base + uint64_t(i) * kSortThreads + threadIdx.x;
-            dg[i] = idx < n ? uint32_t(keys[idx] >> shift) & digit_mask : ~0u;
-        }
 #pragma unroll
         for (int i = 0; i < IPT; ++i)
-            if (dg[i] != ~0u) atomicAdd(&s_cnt[dg[i]], 1u);
+            if (base + uint64_t(i) * kSortThreads + threadIdx.x < n)
+                atomicAdd(&s_cnt[uint32_t(cur[i] >> shift) & digit_mask], 1u);
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
         __syncthreads();
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) cur[i] = nxt[i];
+        }
     }
 }
 
