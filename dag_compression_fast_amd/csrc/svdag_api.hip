@@ -607,6 +607,12 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     // to tail_top + 1, then levels tail_top..0 (at most 4096 nodes each) in one
     // workgroup.
     const int tail_top = std::min(depth - 3, kTailTopLevel);
+    // Host bounds of level l (at most 8^l nodes) size the launches of the small
+    // levels; the kernels walk the device-side counts.
+    auto nodes_bound = [&](int l) { return std::min<uint64_t>(L.level_bound, pow8_capped(l, L.level_bound)); };
+    auto slots_bound = [&](int l) {
+        return std::min<uint64_t>(L.level_slots, (1ull << table_lg(nodes_bound(l), ctx->load_shift)) + kTableOverflow);
+    };
     if (depth - 3 > tail_top)
         launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[0]), &c->pre[depth - 3],
                            &c->table_log2[depth - 3], ctx->load_shift, kTableOverflow, L.level_slots);
@@ -643,11 +649,11 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         a.offsets = tile_offsets;
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
-        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_bound);
+        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : nodes_bound(l + 1));
         mark(ctx, "level build");
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
-                               &c->collisions, L.level_bound);
+                               &c->collisions, nodes_bound(l));
         LevelExtractArgs e{};
         e.table_keys = a.table_keys;
         e.table_rep = a.table_rep;
@@ -665,11 +671,11 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
         TRY(join());
-        launch_level_extract(st, e, L.level_slots);
+        launch_level_extract(st, e, slots_bound(l));
         mark(ctx, "level extract");
         if (l - 1 > tail_top)
             launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[cur ^ 1]), &c->pre[l - 1],
-                               &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, L.level_slots);
+                               &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, slots_bound(l - 1));
     }
     {
         TailArgs t{};
