@@ -55,6 +55,7 @@ inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 #define __builtin_amdgcn_s_sleep(x) emu::yield_sleep()
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
 #define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
+#define __builtin_amdgcn_sched_barrier(x) ((void)(x))
 
 template <class T>
 T __shfl_up(T v, unsigned d, int = 64)
