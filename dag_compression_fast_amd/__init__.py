@@ -14,6 +14,7 @@ the hand-written HIP kernels of ``libsvdag.so`` through its C ABI.  Errors raise
 device memory is held in ``DeviceBuffer`` objects.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -289,6 +290,58 @@ def create_final_dag(dag, ctx=None):
 def free_all(ctx=None):
     """DAGCompression::FreeAll: release the context's scratch arena."""
     (ctx or default_context()).free_all()
+
+
+# --- the DAG file and point queries ------------------------------------------
+def write_dag_file(path, words, levels, aabb_min=(0.0, 0.0, 0.0), aabb_max=(1.0, 1.0, 1.0)):
+    """Writes the flat layout `words` (host uint32 array) as the reference's DAG
+    file (FFileWriter framing, src/Serializer.h:10-101; fields in the order
+    src/DAG_Compression_Fast.cpp:362-375 reads them)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    lo = (ctypes.c_double * 3)(*aabb_min)
+    hi = (ctypes.c_double * 3)(*aabb_max)
+    _lib.check(_lib.lib().svdag_write_dag_file(None, os.fsencode(path), lo, hi, int(levels),
+                                               w.ctypes.data if w.size else None, w.size, SVDAG_HOST))
+
+
+def read_dag_file(path):
+    """Reads a DAG file: dict(words, levels, aabb_min, aabb_max)."""
+    lo = (ctypes.c_double * 3)()
+    hi = (ctypes.c_double * 3)()
+    levels = ctypes.c_uint32()
+    p = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    _lib.check(_lib.lib().svdag_read_dag_file(os.fsencode(path), lo, hi, ctypes.byref(levels), ctypes.byref(p),
+                                              ctypes.byref(n)))
+    try:
+        words = (np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint32)), shape=(n.value,)).copy()
+                 if n.value else np.zeros(0, dtype=np.uint32))
+    finally:
+        _lib.lib().svdag_gen_free(p)
+    return {"words": words, "levels": int(levels.value), "aabb_min": tuple(lo), "aabb_max": tuple(hi)}
+
+
+def query_voxels(words, depth, codes, ctx=None):
+    """Occupancy of Morton codes in a flat layout (host uint32 words of a
+    depth-`depth` DAG), walked on the device as the reference's tracer reads
+    it (src/DAGTracer.cu:85-176): uint8 per code, 1 set, 0 not set, 2 the walk
+    left the words."""
+    ctx = ctx or default_context()
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    c = np.ascontiguousarray(codes, dtype=np.uint64)
+    if c.size == 0:
+        return np.zeros(0, dtype=np.uint8)
+    dw = DeviceBuffer.from_numpy(w) if w.size else None
+    dc = DeviceBuffer.from_numpy(c)
+    do = DeviceBuffer(c.size)
+    try:
+        _lib.check(_lib.lib().svdag_query_voxels(ctx.handle, dw.ptr if dw else None, w.size, int(depth), dc.ptr,
+                                                 c.size, do.ptr))
+        return do.to_numpy(np.uint8, c.size)
+    finally:
+        for b in (dw, dc, do):
+            if b is not None:
+                b.free()
 
 
 # --- synthetic inputs --------------------------------------------------------

@@ -117,6 +117,11 @@ def bind_library(L):
         "svdag_gen_chunk": ([u64, u64, i32, ctypes.c_double, ctypes.c_double, u64, u64, vp, u64,
                              ctypes.POINTER(u64)], i32),
         "svdag_gen_free": ([vp], None),
+        "svdag_write_dag_file": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.c_uint32, vp, u64, i32], i32),
+        "svdag_read_dag_file": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
+        "svdag_query_voxels": ([vp, vp, u64, i32, vp, u64, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -32,7 +32,7 @@ const char* const kKernelNames[kNumKernelIds] = {
     "scan_counts",    "leaf_reduce", "leaf_compact",      "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
-    "slot_list"};
+    "slot_list",      "query_voxels"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -1336,6 +1336,84 @@ int svdag_free_all(svdag_context* ctx)
 }
 
 const char* svdag_last_error(void) { return g_error.c_str(); }
+
+// The reference's DAG file (FFileWriter / FFileReader framing, src/Serializer.h:10-101,
+// read in that order at src/DAG_Compression_Fast.cpp:362-375): six float64
+// (AABB min x y z, max x y z), uint32 level count, then the words as an array
+// (uint64 count, uint32 data), native little-endian, no padding.
+int svdag_write_dag_file(svdag_context* ctx, const char* path, const double* aabb_min, const double* aabb_max,
+                         uint32_t levels, const uint32_t* words, uint64_t n_words, int32_t words_location)
+{
+    if (!path || !aabb_min || !aabb_max || (n_words && !words))
+        return set_error(SVDAG_ERR_ARGUMENT, "null path, bounds or words");
+    if (words_location != SVDAG_HOST && words_location != SVDAG_DEVICE)
+        return set_error(SVDAG_ERR_ARGUMENT, "bad words location %d", words_location);
+    std::vector<uint32_t> staged;
+    const uint32_t* src = words;
+    if (words_location == SVDAG_DEVICE && n_words) {
+        TRY(check_ctx(ctx));
+        staged.resize(n_words);
+        HIP_TRY(hipMemcpyAsync(staged.data(), words, 4 * n_words, hipMemcpyDeviceToHost, ctx->stream));
+        TRY(finish(ctx, "write_dag_file"));
+        src = staged.data();
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(SVDAG_ERR_ARGUMENT, "cannot open %s for writing", path);
+    bool ok = std::fwrite(aabb_min, sizeof(double), 3, f) == 3 && std::fwrite(aabb_max, sizeof(double), 3, f) == 3 &&
+              std::fwrite(&levels, sizeof(uint32_t), 1, f) == 1 && std::fwrite(&n_words, sizeof(uint64_t), 1, f) == 1 &&
+              (n_words == 0 || std::fwrite(src, sizeof(uint32_t), n_words, f) == n_words);
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) return set_error(SVDAG_ERR_ARGUMENT, "write to %s failed", path);
+    return 0;
+}
+
+int svdag_read_dag_file(const char* path, double* aabb_min, double* aabb_max, uint32_t* levels, uint32_t** words,
+                        uint64_t* n_words)
+{
+    if (!path || !aabb_min || !aabb_max || !levels || !words || !n_words)
+        return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    *words = nullptr;
+    *n_words = 0;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(SVDAG_ERR_ARGUMENT, "cannot open %s", path);
+    uint64_t n = 0;
+    bool ok = std::fread(aabb_min, sizeof(double), 3, f) == 3 && std::fread(aabb_max, sizeof(double), 3, f) == 3 &&
+              std::fread(levels, sizeof(uint32_t), 1, f) == 1 && std::fread(&n, sizeof(uint64_t), 1, f) == 1;
+    // The word count must fit what is left of the file.
+    long here = ok ? std::ftell(f) : -1;
+    if (ok && here >= 0 && std::fseek(f, 0, SEEK_END) == 0) {
+        const long end = std::ftell(f);
+        ok = end >= here && n <= uint64_t(end - here) / sizeof(uint32_t) && std::fseek(f, here, SEEK_SET) == 0;
+    } else {
+        ok = false;
+    }
+    uint32_t* buf = nullptr;
+    if (ok && n) {
+        buf = static_cast<uint32_t*>(std::malloc(sizeof(uint32_t) * n));
+        ok = buf && std::fread(buf, sizeof(uint32_t), n, f) == n;
+    }
+    std::fclose(f);
+    if (!ok) {
+        std::free(buf);
+        return set_error(SVDAG_ERR_ARGUMENT, "%s is not a DAG file (short or inconsistent)", path);
+    }
+    *words = buf;
+    *n_words = n;
+    return 0;
+}
+
+int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words, int32_t depth,
+                       const uint64_t* d_codes, uint64_t n, uint8_t* d_out)
+{
+    TRY(check_ctx(ctx));
+    if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
+    if (n && (!d_codes || !d_out)) return set_error(SVDAG_ERR_ARGUMENT, "null codes or output");
+    if (n_words && !d_words) return set_error(SVDAG_ERR_ARGUMENT, "null words");
+    if (n == 0) return 0;
+    launch_query_voxels(ctx->stream, d_words, n_words, depth, d_codes, n, d_out);
+    HIP_TRY(hipGetLastError());
+    return finish(ctx, "query_voxels");
+}
 
 int svdag_profile_enable(svdag_context* ctx, uint64_t kernel_mask)
 {

@@ -1333,6 +1333,62 @@ void launch_level_check(hipStream_t s, const uint32_t* parent_slots, const uint3
                        parent_slots, table_rep, parent_masks, parent_children, count_ptr, collisions);
 }
 
+// ---------------------------------------------------------------------------
+// Point query against the flat layout, walking it as the reference's tracer
+// does (src/DAGTracer.cu:85-176): a node is its child mask word followed by
+// one absolute word index per set slot; after depth - 2 nodes the index
+// names a leaf, two words of 4x4x4 occupancy (low word first).  The slot of
+// level l is bits [3(depth-1-l), 3(depth-l)) of the code.
+// out[i]: 1 set, 0 not set (or the code is outside 2^(3*depth)), 2 the walk
+// left the words (not a valid layout).
+// ---------------------------------------------------------------------------
+__global__ void k_query_voxels(const uint32_t* __restrict__ words, uint64_t n_words, int depth,
+                               const uint64_t* __restrict__ codes, uint64_t n, uint8_t* __restrict__ out)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t c = codes[i];
+        uint8_t r = 3;  // walk in progress
+        if (n_words == 0 || (c >> (3 * depth)) != 0) r = 0;
+        uint64_t idx = 0;
+        for (int l = 0; r == 3 && l < depth - 2; ++l) {
+            if (idx >= n_words) {
+                r = 2;
+                break;
+            }
+            const uint32_t mask = words[idx];
+            const uint32_t slot = uint32_t(c >> (3 * (depth - 1 - l))) & 7u;
+            if (mask > 0xFFu) {
+                r = 2;
+            } else if (!((mask >> slot) & 1u)) {
+                r = 0;
+            } else {
+                const uint64_t at = idx + 1 + uint32_t(__popc(mask & ((1u << slot) - 1u)));
+                if (at >= n_words) r = 2;
+                else idx = words[at];
+            }
+        }
+        if (r == 3) {
+            if (idx + 1 >= n_words) {
+                r = 2;
+            } else {
+                const uint32_t bit = uint32_t(c & 63u);
+                r = uint8_t((words[idx + (bit >> 5)] >> (bit & 31u)) & 1u);
+            }
+        }
+        out[i] = r;
+    }
+}
+
+void launch_query_voxels(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth, const uint64_t* codes,
+                         uint64_t n, uint8_t* out)
+{
+    if (n == 0) return;
+    TimedLaunch t(kKQueryVoxels, s);
+    hipLaunchKernelGGL(k_query_voxels, dim3(grid_for((n + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, words,
+                       n_words, depth, codes, n, out);
+}
+
 void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev)
 {
     if (a.total_nodes == 0 && a.num_leaves == 0) return;

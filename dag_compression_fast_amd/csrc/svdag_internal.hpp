@@ -41,6 +41,7 @@ enum KernelId : int {
     kKSketch,
     kKLevelTail,
     kKSlotList,
+    kKQueryVoxels,
     kNumKernelIds
 };
 
@@ -309,6 +310,11 @@ struct FinalArgs {
 };
 // total_nodes_dev: device copy of a.total_nodes (the scan reads it there).
 void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev);
+
+// out[i] = occupancy of voxel codes[i] in the flat layout (1 set, 0 not set,
+// 2 the walk left the words).
+void launch_query_voxels(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth, const uint64_t* codes,
+                         uint64_t n, uint8_t* out);
 
 // ---- merge ----------------------------------------------------------------
 // Remaps and inserts the nodes of many sub-DAG levels into one shared table.

@@ -155,6 +155,29 @@ int svdag_free_all(svdag_context* ctx);
 /* Message of the last failing call on this host thread. */
 const char* svdag_last_error(void);
 
+/* --- the DAG file and point queries --------------------------------------- */
+
+/* The reference's on-disk DAG (FFileWriter / FFileReader framing,
+ * src/Serializer.h:10-101, read at src/DAG_Compression_Fast.cpp:361-375): six
+ * float64 (AABB min x, y, z, max x, y, z), uint32 level count, uint64 word
+ * count, the uint32 words of the flat layout.  Native little-endian, no padding.
+ * words_location: SVDAG_HOST, or SVDAG_DEVICE (copied out on ctx's stream;
+ * ctx may be NULL for host words). */
+int svdag_write_dag_file(svdag_context* ctx, const char* path, const double* aabb_min,
+                         const double* aabb_max, uint32_t levels, const uint32_t* words,
+                         uint64_t n_words, int32_t words_location);
+/* Reads such a file; *words is malloc'd host memory (svdag_gen_free).  A file
+ * shorter than its word count says is an error. */
+int svdag_read_dag_file(const char* path, double* aabb_min, double* aabb_max, uint32_t* levels,
+                        uint32_t** words, uint64_t* n_words);
+
+/* Occupancy of voxels in a flat layout of a depth-`depth` DAG, walked as the
+ * reference's tracer reads it (src/DAGTracer.cu:85-176).  d_words, d_codes and
+ * d_out are device pointers; d_out[i] = 1 if Morton code d_codes[i] is set, 0 if
+ * not (or the code is >= 2^(3*depth)), 2 if the walk leaves the words. */
+int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words,
+                       int32_t depth, const uint64_t* d_codes, uint64_t n, uint8_t* d_out);
+
 /* --- per-kernel timing ---------------------------------------------------- */
 
 /* Counterpart of the reference's per-operation timer FCudaScopePerfRecorder
