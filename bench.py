@@ -122,6 +122,60 @@ def final_words(L, _lib, ctx, dag):
     return fetch_words(L, _lib, ctx, *device_words(L, _lib, ctx, dag))
 
 
+def bench_voxelize(svd, spec, args, ctx):
+    """Device fragment producer on the triangle workload's own triangles:
+    triangles (resident in HBM) -> unsorted Morton keys, then the DAG build of
+    those keys.  Wall time per call, mean of `reps` after one warm-up."""
+    kind, seed, count, mu, sigma, _ = spec["gen"]
+    if kind != "triangles":
+        return None
+    depth = spec["depth"]
+    tris = svd.gen_triangles(seed, depth, count, mu, sigma)
+    d_tris = svd.DeviceBuffer.from_numpy(tris)
+    L = svd._lib.lib()
+    reps = 5
+
+    def voxelize():
+        p, n = ctypes.c_void_p(), ctypes.c_uint64()
+        svd._lib.check(L.svdag_voxelize_triangles(ctx.handle, d_tris.ptr, count, depth, ctypes.byref(p),
+                                                  ctypes.byref(n)))
+        return p, int(n.value)
+
+    p, n = voxelize()
+    L.svdag_free_keys(ctx.handle, p)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p, n = voxelize()
+        L.svdag_free_keys(ctx.handle, p)
+    t_vox = (time.perf_counter() - t0) / reps
+    # Triangles -> keys -> DAG -> flat layout, keys never leaving the device.
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p, n = voxelize()
+        dag = svd._lib.SvdagDag()
+        svd._lib.check(L.svdag_create_dag(ctx.handle, p, n, depth, ctypes.byref(dag)))
+        words, nw = ctypes.c_void_p(), ctypes.c_uint64()
+        svd._lib.check(L.svdag_create_final_dag(ctx.handle, ctypes.byref(dag), ctypes.byref(words),
+                                                ctypes.byref(nw)))
+        L.svdag_free_words(ctx.handle, words)
+        L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+        L.svdag_free_keys(ctx.handle, p)
+    t_e2e = (time.perf_counter() - t0) / reps
+    if args.kernel_table:
+        ctx.profile_enable(None)
+        ctx.profile_reset()
+        p, n = voxelize()
+        L.svdag_free_keys(ctx.handle, p)
+        table = ctx.profile_read()
+        ctx.profile_enable([])
+        for k, (ms, cnt) in sorted(table.items(), key=lambda kv: -kv[1][0]):
+            log(f"  voxelize {k:20s} {ms:8.3f} ms  {cnt:4d} launches")
+    d_tris.free()
+    log(f"voxelize: {count} triangles -> {n} keys in {t_vox * 1e3:.2f} ms; with the build {t_e2e * 1e3:.2f} ms")
+    return {"triangles": count, "keys": n, "voxelize_ms": round(t_vox * 1e3, 3),
+            "keys_per_s": round(n / t_vox, 1), "triangles_to_words_ms": round(t_e2e * 1e3, 3)}
+
+
 def bench_chunks(svd, _lib, args):
     """Plain (CreateDAG per chunk + MergeDAGs + CreateFinalDAG, the reference's
     flow) against batched (one create_dag_batched) and served (continuous
@@ -248,6 +302,7 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
     ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
     ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
+    ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
@@ -359,6 +414,8 @@ def main():
         if not same:
             log("GPU result differs from the oracle")
 
+    vox_stats = None if args.no_voxelize else bench_voxelize(svd, spec, args, ctx)
+
     chunk_stats = None
     if not args.no_chunks:
         chunk_stats = bench_chunks(svd, _lib, args)
@@ -372,6 +429,7 @@ def main():
         "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "check": check, "chunks": chunk_stats,
+        "voxelize": vox_stats,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -321,6 +321,59 @@ def read_dag_file(path):
     return {"words": words, "levels": int(levels.value), "aabb_min": tuple(lo), "aabb_max": tuple(hi)}
 
 
+def gen_triangles(seed, depth, count, mu, sigma):
+    """The triangle soup of gen_triangle_surfaces: float64 array (count, 3, 3)."""
+    out = np.zeros((max(0, int(count)), 3, 3), dtype=np.float64)
+    _lib.check(_lib.lib().svdag_gen_triangles(seed, depth, int(count), mu, sigma,
+                                              out.ctypes.data if out.size else None))
+    return out
+
+
+def gen_voxelize(triangles, depth, threads=8):
+    """Host voxeliser (same rule as voxelize_triangles): sorted unique u64 keys."""
+    t = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
+    p = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    rc = _lib.lib().svdag_gen_voxelize(t.ctypes.data if t.size else None, t.shape[0], int(depth), int(threads),
+                                       ctypes.byref(p), ctypes.byref(n))
+    return _take_keys(rc, p, n)
+
+
+def voxelize_triangles(triangles, depth, ctx=None):
+    """Device voxeliser: triangles (n, 3, 3) float64 in a 2^depth grid ->
+    DeviceBuffer of unsorted u64 Morton keys with duplicates, and their count.
+    The buffer feeds create_dag directly; free it with .free()."""
+    ctx = ctx or default_context()
+    t = np.ascontiguousarray(triangles, dtype=np.float64).reshape(-1, 9)
+    if t.shape[0] == 0:
+        return _KeyBuffer(ctypes.c_void_p(), 0, ctx), 0
+    dt = DeviceBuffer.from_numpy(t)
+    p = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    try:
+        _lib.check(_lib.lib().svdag_voxelize_triangles(ctx.handle, dt.ptr, t.shape[0], int(depth), ctypes.byref(p),
+                                                       ctypes.byref(n)))
+    finally:
+        dt.free()
+    return _KeyBuffer(p, int(n.value), ctx), int(n.value)
+
+
+class _KeyBuffer(DeviceBuffer):
+    """Keys owned by the library (svdag_free_keys)."""
+
+    def __init__(self, ptr, count, ctx):
+        self.ptr = ptr
+        self.nbytes = 8 * count
+        self.count = count
+        self.dtype = np.dtype(np.uint64)
+        self._ctx = ctx
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _lib.lib().svdag_free_keys(self._ctx.handle, self.ptr)
+        self.ptr = None
+
+
 def query_voxels(words, depth, codes, ctx=None):
     """Occupancy of Morton codes in a flat layout (host uint32 words of a
     depth-`depth` DAG), walked on the device as the reference's tracer reads

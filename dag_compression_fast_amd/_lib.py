@@ -122,6 +122,10 @@ def bind_library(L):
         "svdag_read_dag_file": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_query_voxels": ([vp, vp, u64, i32, vp, u64, vp], i32),
+        "svdag_voxelize_triangles": ([vp, vp, u64, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
+        "svdag_free_keys": ([vp, vp], i32),
+        "svdag_gen_triangles": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, vp], i32),
+        "svdag_gen_voxelize": ([vp, u64, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libsvdag.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["svdag_sort.hip", "svdag_build.hip", "svdag_api.hip"]
+HIP_SOURCES = ["svdag_sort.hip", "svdag_build.hip", "svdag_api.hip", "svdag_voxelize.hip"]
 CXX_SOURCES = ["svdag_gen.cpp"]
 HEADERS = ["svdag_device.hpp", "svdag_internal.hpp"]
 
@@ -58,7 +58,7 @@ def build(verbose=False, jobs=None):
         objs.append(obj)
     for src in CXX_SOURCES:
         obj = os.path.join(OUT, src.replace(".cpp", ".o"))
-        cmds.append(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-Wall", "-pthread",
+        cmds.append(["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-Wall", "-pthread",
                      "-I", INCLUDE, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs or min(8, len(cmds))) as ex:

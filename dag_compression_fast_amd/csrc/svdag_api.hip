@@ -32,7 +32,7 @@ const char* const kKernelNames[kNumKernelIds] = {
     "scan_counts",    "leaf_reduce", "leaf_compact",      "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
-    "slot_list",      "query_voxels"};
+    "slot_list",      "query_voxels",    "vox_setup",   "vox_count",         "vox_write"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -1399,6 +1399,79 @@ int svdag_read_dag_file(const char* path, double* aabb_min, double* aabb_max, ui
     }
     *words = buf;
     *n_words = n;
+    return 0;
+}
+
+int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t n_tris, int32_t depth,
+                             uint64_t** d_keys, uint64_t* n_keys)
+{
+    TRY(check_ctx(ctx));
+    if (!d_keys || !n_keys || (n_tris && !d_tris)) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    if (depth < 1 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [1, 21]", depth);
+    *d_keys = nullptr;
+    *n_keys = 0;
+    if (n_tris == 0) return 0;
+    hipStream_t st = ctx->stream;
+    // Triangle stage in the arena: constants, column counts and offsets, and
+    // three counters (triangles, columns, keys) plus an error word.
+    Carver cv;
+    const uint64_t o_setup = cv.take(sizeof(TriSetup) * n_tris);
+    const uint64_t o_cols = cv.take(4 * n_tris);
+    const uint64_t o_coff = cv.take(8 * n_tris);
+    const uint64_t o_cnt = cv.take(8 * 4);
+    TRY(ensure_arena(ctx, cv.off));
+    char* A = ctx->arena;
+    TriSetup* setup = reinterpret_cast<TriSetup*>(A + o_setup);
+    uint32_t* cols = reinterpret_cast<uint32_t*>(A + o_cols);
+    uint64_t* coff = reinterpret_cast<uint64_t*>(A + o_coff);
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(A + o_cnt);  // [n_tris, columns, keys, error]
+    launch_fill_u64(st, cnt, 4, 0);
+    launch_fill_u64(st, cnt, 1, n_tris);
+    launch_tri_setup(st, d_tris, n_tris, depth, setup, cols, reinterpret_cast<unsigned long long*>(cnt + 3));
+    launch_scan_counts(st, cols, coff, cnt, 1, cnt + 1);
+    uint64_t h[4] = {};
+    HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    TRY(finish(ctx, "voxelize (triangle setup)"));
+    if (h[3]) return set_error(SVDAG_ERR_CAPACITY, "a triangle spans 2^32 or more voxel columns");
+    const uint64_t columns = h[1];
+    if (columns == 0) return 0;
+    // Tile counts and offsets, then the keys (the caller's, svdag_free_keys).
+    const uint64_t ntiles = tiles(columns, kVoxTileItems);
+    void* tile_mem = nullptr;
+    TRY(device_alloc(ctx, &tile_mem, align_up(4 * ntiles) + align_up(8 * ntiles) + 8 * (ntiles + 1)));
+    uint32_t* tcounts = static_cast<uint32_t*>(tile_mem);
+    uint64_t* toffs = reinterpret_cast<uint64_t*>(static_cast<char*>(tile_mem) + align_up(4 * ntiles));
+    uint64_t* ttri = reinterpret_cast<uint64_t*>(static_cast<char*>(tile_mem) + align_up(4 * ntiles) +
+                                                 align_up(8 * ntiles));
+    launch_vox_count(st, setup, coff, cnt, cnt + 1, columns, depth, ttri, tcounts);
+    launch_scan_counts(st, tcounts, toffs, cnt + 1, kVoxTileItems, cnt + 2);
+    if (hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess) {
+        device_release(ctx, tile_mem);
+        return set_error(SVDAG_ERR_HIP, "count read-back failed");
+    }
+    int rc = finish(ctx, "voxelize (count)");
+    const uint64_t keys = h[2];
+    void* out = nullptr;
+    if (rc == 0 && keys) rc = device_alloc(ctx, &out, 8 * keys);
+    if (rc == 0 && keys) {
+        launch_vox_write(st, setup, coff, ttri, cnt, cnt + 1, columns, depth, toffs, static_cast<uint64_t*>(out));
+        rc = hipGetLastError() == hipSuccess ? finish(ctx, "voxelize (write)")
+                                             : set_error(SVDAG_ERR_HIP, "voxelizer launch failed");
+    }
+    device_release(ctx, tile_mem);
+    if (rc != 0) {
+        device_release(ctx, out);
+        return rc;
+    }
+    *d_keys = static_cast<uint64_t*>(out);
+    *n_keys = keys;
+    return 0;
+}
+
+int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys)
+{
+    TRY(check_ctx(ctx));
+    device_release(ctx, d_keys);
     return 0;
 }
 

@@ -12,6 +12,9 @@
 //                            tests dilated by ½·max(|n_e.x|, |n_e.y|);
 //   * value noise            trilinear lattice noise thresholded at ½;
 //   * chunks                 a seeded mix of the above for one sub-volume.
+// All floating-point arithmetic here is evaluated as written (the library builds
+// this file with -ffp-contract=off): the device voxeliser repeats the triangle
+// arithmetic operation for operation and must emit the same voxels.
 // Duplicates are appended by resampling the unique set, then the list is
 // Fisher-Yates shuffled.  Every random draw comes from xoshiro256** seeded by
 // splitmix64 of (seed, item index), so output is independent of thread count.
@@ -266,6 +269,19 @@ std::vector<uint64_t> gen_spheres(uint64_t seed, int32_t depth, int32_t count, d
     return all;
 }
 
+// Triangle i of the triangle-surface workload: v0 uniform in the volume, the
+// two other corners at lognormal distances in uniform directions.
+void make_triangle(uint64_t seed, uint64_t i, int64_t side, double mu, double sigma, V3 (&v)[3])
+{
+    Rng r(seed, 1 + i);
+    const V3 v0 = {r.uniform(0, double(side)), r.uniform(0, double(side)), r.uniform(0, double(side))};
+    const double l1 = lognormal(r, mu, sigma), l2 = lognormal(r, mu, sigma);
+    const V3 d1 = random_direction(r), d2 = random_direction(r);
+    v[0] = v0;
+    v[1] = {v0.x + l1 * d1.x, v0.y + l1 * d1.y, v0.z + l1 * d1.z};
+    v[2] = {v0.x + l2 * d2.x, v0.y + l2 * d2.y, v0.z + l2 * d2.z};
+}
+
 std::vector<uint64_t> gen_triangles(uint64_t seed, int32_t depth, int32_t count, double mu, double sigma, int threads)
 {
     const int64_t side = side_of(depth);
@@ -274,13 +290,9 @@ std::vector<uint64_t> gen_triangles(uint64_t seed, int32_t depth, int32_t count,
     parallel_for(threads, uint64_t(std::max(1, threads)), [&](uint64_t tb, uint64_t te, int) {
         for (uint64_t t = tb; t < te; ++t)
             for (uint64_t i = t; i < uint64_t(count); i += uint64_t(std::max(1, threads))) {
-                Rng r(seed, 1 + i);
-                const V3 v0 = {r.uniform(0, double(side)), r.uniform(0, double(side)), r.uniform(0, double(side))};
-                const double l1 = lognormal(r, mu, sigma), l2 = lognormal(r, mu, sigma);
-                const V3 d1 = random_direction(r), d2 = random_direction(r);
-                const V3 v1 = {v0.x + l1 * d1.x, v0.y + l1 * d1.y, v0.z + l1 * d1.z};
-                const V3 v2 = {v0.x + l2 * d2.x, v0.y + l2 * d2.y, v0.z + l2 * d2.z};
-                triangle_surface(v0, v1, v2, side, [&](int64_t x, int64_t y, int64_t z) { local[t].push_back(morton(uint32_t(x), uint32_t(y), uint32_t(z))); });
+                V3 v[3];
+                make_triangle(seed, i, side, mu, sigma, v);
+                triangle_surface(v[0], v[1], v[2], side, [&](int64_t x, int64_t y, int64_t z) { local[t].push_back(morton(uint32_t(x), uint32_t(y), uint32_t(z))); });
             }
     });
     std::vector<uint64_t> all;
@@ -401,6 +413,41 @@ int svdag_gen_triangle_surfaces(uint64_t seed, int32_t depth, int32_t count, dou
     auto v = gen_triangles(seed, depth, count, mu, sigma, threads);
     add_duplicates_and_shuffle(v, dup_fraction, seed);
     return hand_out(v, keys, n);
+}
+
+int svdag_gen_triangles(uint64_t seed, int32_t depth, int32_t count, double mu, double sigma, double* out)
+{
+    if (depth < 1 || depth > 21 || count < 0 || (count && !out)) return SVDAG_ERR_ARGUMENT;
+    const int64_t side = side_of(depth);
+    for (int32_t i = 0; i < count; ++i) {
+        V3 v[3];
+        make_triangle(seed, uint64_t(i), side, mu, sigma, v);
+        for (int c = 0; c < 3; ++c) {
+            out[9 * i + 3 * c + 0] = v[c].x;
+            out[9 * i + 3 * c + 1] = v[c].y;
+            out[9 * i + 3 * c + 2] = v[c].z;
+        }
+    }
+    return 0;
+}
+
+int svdag_gen_voxelize(const double* tris, uint64_t n_tris, int32_t depth, int32_t threads, uint64_t** keys,
+                       uint64_t* n)
+{
+    if (depth < 1 || depth > 21 || !keys || !n || (n_tris && !tris)) return SVDAG_ERR_ARGUMENT;
+    const int64_t side = side_of(depth);
+    std::vector<std::vector<uint64_t>> local(std::max(1, threads));
+    parallel_for(threads, n_tris, [&](uint64_t b, uint64_t e, int t) {
+        for (uint64_t i = b; i < e; ++i) {
+            const double* p = tris + 9 * i;
+            triangle_surface({p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, side,
+                             [&](int64_t x, int64_t y, int64_t z) { local[t].push_back(morton(uint32_t(x), uint32_t(y), uint32_t(z))); });
+        }
+    });
+    std::vector<uint64_t> all;
+    for (auto& l : local) all.insert(all.end(), l.begin(), l.end());
+    sort_unique(all, threads);
+    return hand_out(all, keys, n);
 }
 
 int svdag_gen_value_noise(uint64_t seed, int32_t depth, int32_t spacing, uint32_t sub_depth, uint64_t sub_index,

@@ -42,6 +42,9 @@ enum KernelId : int {
     kKLevelTail,
     kKSlotList,
     kKQueryVoxels,
+    kKVoxSetup,
+    kKVoxCount,
+    kKVoxWrite,
     kNumKernelIds
 };
 
@@ -310,6 +313,36 @@ struct FinalArgs {
 };
 // total_nodes_dev: device copy of a.total_nodes (the scan reads it there).
 void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev);
+
+// ---- device voxeliser ------------------------------------------------------
+// Per-triangle constants of the thin surface rule: normal n, n . v0, the
+// projected edge functions (nx, ny, d) per axis plane and edge, the dominant
+// axis a with the column axes u, w, the planes that are not edge-on, and the
+// column rectangle (u0, w0, nw columns along w).
+struct TriSetup {
+    double n[3];
+    double nd;
+    double ex[3][3], ey[3][3], ed[3][3];
+    int64_t u0, w0;
+    uint32_t nw;
+    int32_t a, u, w, planes;
+};
+constexpr uint64_t kVoxTileItems = 2048;  // columns per voxeliser tile
+
+// cols[t] = columns of triangle t (tris: 9 doubles per triangle); a triangle
+// with 2^32 columns or more sets kErrInternal in *err.
+void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* cols,
+                      unsigned long long* err);
+// Voxels per tile of kVoxTileItems global columns; col_offsets = exclusive scan
+// of cols; *total_cols_dev columns in all, at most total_cols_bound.  tile_tri
+// (tiles + 1 entries) receives each tile's first triangle; the write pass
+// reads it.
+void launch_vox_count(hipStream_t s, const TriSetup* setup, const uint64_t* col_offsets, const uint64_t* n_tris_dev,
+                      const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth, uint64_t* tile_tri,
+                      uint32_t* tile_counts);
+void launch_vox_write(hipStream_t s, const TriSetup* setup, const uint64_t* col_offsets, const uint64_t* tile_tri,
+                      const uint64_t* n_tris_dev, const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth,
+                      const uint64_t* tile_offsets, uint64_t* keys);
 
 // out[i] = occupancy of voxel codes[i] in the flat layout (1 set, 0 not set,
 // 2 the walk left the words).

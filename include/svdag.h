@@ -171,6 +171,17 @@ int svdag_write_dag_file(svdag_context* ctx, const char* path, const double* aab
 int svdag_read_dag_file(const char* path, double* aabb_min, double* aabb_max, uint32_t* levels,
                         uint32_t** words, uint64_t* n_words);
 
+/* Device fragment producer: voxelises a triangle soup into unsorted Morton
+ * keys with duplicates (the reference's input contract; it rasterises with
+ * OpenGL, src/Voxelizer.cpp:254-369).  d_tris: device, 9 float64 per triangle
+ * (x y z of its three corners) in a 2^depth grid; the thin surface rule of the
+ * triangle-surface generator, evaluated identically (no fused multiply-adds),
+ * so the voxel set equals svdag_gen_triangle_surfaces' for the same triangles.
+ * *d_keys is a device allocation (svdag_free_keys); 1 <= depth <= 21. */
+int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t n_tris,
+                             int32_t depth, uint64_t** d_keys, uint64_t* n_keys);
+int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys);
+
 /* Occupancy of voxels in a flat layout of a depth-`depth` DAG, walked as the
  * reference's tracer reads it (src/DAGTracer.cu:85-176).  d_words, d_codes and
  * d_out are device pointers; d_out[i] = 1 if Morton code d_codes[i] is set, 0 if
@@ -204,6 +215,15 @@ int svdag_gen_sphere_shells(uint64_t seed, int32_t depth, int32_t count, double 
 int svdag_gen_triangle_surfaces(uint64_t seed, int32_t depth, int32_t count, double mu,
                                 double sigma, double dup_fraction, int32_t threads,
                                 uint64_t** keys, uint64_t* n);
+/* The triangles of svdag_gen_triangle_surfaces(seed, depth, count, mu, sigma,
+ * ...): out receives 9 float64 per triangle (corners v0, v1, v2). */
+int svdag_gen_triangles(uint64_t seed, int32_t depth, int32_t count, double mu, double sigma,
+                        double* out);
+/* Host voxeliser of the triangle-surface rule (the same rule as
+ * svdag_voxelize_triangles): sorted unique Morton keys of host triangles
+ * (9 float64 each); *keys is malloc'd (svdag_gen_free). */
+int svdag_gen_voxelize(const double* tris, uint64_t n_tris, int32_t depth, int32_t threads,
+                       uint64_t** keys, uint64_t* n);
 int svdag_gen_value_noise(uint64_t seed, int32_t depth, int32_t spacing, uint32_t sub_depth,
                           uint64_t sub_index, int32_t threads, uint64_t** keys, uint64_t* n);
 int svdag_gen_chunk_plan(uint64_t seed, uint64_t num_chunks, double median, double sigma,
