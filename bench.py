@@ -176,6 +176,51 @@ def bench_voxelize(svd, spec, args, ctx):
             "keys_per_s": round(n / t_vox, 1), "triangles_to_words_ms": round(t_e2e * 1e3, 3)}
 
 
+def bench_trace(svd, words, depth, args, ctx):
+    """Primary rays through the step's flat layout (resident in HBM): one
+    1920x1080 frame from a camera outside the volume looking at its centre,
+    position colours.  Wall time per frame, mean of `reps` after one warm-up."""
+    side = float(1 << depth)
+    eye = np.array([-0.45, -0.35, 1.3]) * side
+    fwd = np.full(3, 0.5 * side) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, [0.0, 0.0, 1.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    width, height = 1920, 1080
+    p = svd.trace_params(eye, right, up, fwd, 60.0, width, height, (0.0, 0.0, 0.0), (side, side, side), depth,
+                         debug_level=0xFFFFFFFF)
+    tp = svd._lib.TraceParams.from_dict(p)
+    L = svd._lib.lib()
+    dw = svd.DeviceBuffer.from_numpy(words)
+    dpaths = svd.DeviceBuffer(12 * width * height)
+    dcolors = svd.DeviceBuffer(4 * width * height)
+    reps = 10
+
+    def frame():
+        svd._lib.check(L.svdag_trace_paths(ctx.handle, dw.ptr, words.size, depth, ctypes.byref(tp), dpaths.ptr,
+                                           dcolors.ptr))
+
+    try:
+        frame()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            frame()
+        dt = (time.perf_counter() - t0) / reps
+        hits = int(dpaths.to_numpy(np.uint32, 3 * width * height).reshape(-1, 3).any(axis=1).sum())
+        ctx.profile_enable(["trace_paths"])
+        ctx.profile_reset()
+        frame()
+        kernel_ms = ctx.profile_read()["trace_paths"][0]
+        ctx.profile_enable([])
+    finally:
+        for b in (dw, dpaths, dcolors):
+            b.free()
+    log(f"trace: {width}x{height} in {dt * 1e3:.2f} ms ({hits} rays hit), kernel {kernel_ms:.3f} ms")
+    return {"width": width, "height": height, "frame_ms": round(dt * 1e3, 3), "kernel_ms": round(kernel_ms, 3),
+            "rays_per_s": round(width * height / dt, 1), "hits": hits}
+
+
 def bench_chunks(svd, _lib, args):
     """Plain (CreateDAG per chunk + MergeDAGs + CreateFinalDAG, the reference's
     flow) against batched (one create_dag_batched) and served (continuous
@@ -303,6 +348,7 @@ def main():
     ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
     ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
     ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
+    ap.add_argument("--no-trace", action="store_true", help="skip the ray tracing timing")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
@@ -415,6 +461,7 @@ def main():
             log("GPU result differs from the oracle")
 
     vox_stats = None if args.no_voxelize else bench_voxelize(svd, spec, args, ctx)
+    trace_stats = None if args.no_trace else bench_trace(svd, result_words, depth, args, ctx)
 
     chunk_stats = None
     if not args.no_chunks:
@@ -429,7 +476,7 @@ def main():
         "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "check": check, "chunks": chunk_stats,
-        "voxelize": vox_stats,
+        "voxelize": vox_stats, "trace": trace_stats,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

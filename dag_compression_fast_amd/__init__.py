@@ -397,6 +397,53 @@ def query_voxels(words, depth, codes, ctx=None):
                 b.free()
 
 
+def trace_params(position, right, up, direction, fov_degrees, width, height, aabb_min, aabb_max, depth,
+                 debug_level=0):
+    """The reference's FDAGTracer::GetTraceParams (src/DAGTracer.h:33-82):
+    camera at `position` with rotation rows right, up, direction and full
+    vertical field of view `fov_degrees`, in the 2^depth grid of the box
+    [aabb_min, aabb_max].  Returns a dict for trace_paths."""
+    def d3(v):
+        return (ctypes.c_double * 3)(*[float(x) for x in v])
+    out = _lib.TraceParams()
+    _lib.check(_lib.lib().svdag_trace_params_from_view(d3(position), d3(right), d3(up), d3(direction),
+                                                       float(fov_degrees), int(width), int(height), d3(aabb_min),
+                                                       d3(aabb_max), int(depth), int(debug_level) & 0xFFFFFFFF,
+                                                       ctypes.byref(out)))
+    return out.as_dict()
+
+
+def trace_paths(words, depth, params, ctx=None):
+    """Primary rays through a flat layout on the device (the reference's
+    trace_paths, src/DAGTracer.cu:177-428).  words: host uint32 words or a
+    DeviceBuffer; params: dict from trace_params.  Returns (paths, colors):
+    uint32 (height, width, 3) grid position of the first voxel hit ((0, 0, 0)
+    for none, 0xFFFFFFFF when the walk leaves the words) and uint32
+    (height, width) colours."""
+    ctx = ctx or default_context()
+    p = _lib.TraceParams.from_dict(params)
+    own = not isinstance(words, DeviceBuffer)
+    if own:
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        dw = DeviceBuffer.from_numpy(w) if w.size else None
+        n_words = w.size
+    else:
+        dw, n_words = words, words.nbytes // 4
+    pixels = int(p.width) * int(p.height)
+    dpaths = DeviceBuffer(max(1, 12 * pixels))
+    dcolors = DeviceBuffer(max(1, 4 * pixels))
+    try:
+        _lib.check(_lib.lib().svdag_trace_paths(ctx.handle, dw.ptr if dw else None, n_words, int(depth),
+                                                ctypes.byref(p), dpaths.ptr, dcolors.ptr))
+        paths = dpaths.to_numpy(np.uint32, 3 * pixels).reshape(p.height, p.width, 3)
+        colors = dcolors.to_numpy(np.uint32, pixels).reshape(p.height, p.width)
+        return paths, colors
+    finally:
+        for b in (dpaths, dcolors) + ((dw,) if own else ()):
+            if b is not None:
+                b.free()
+
+
 # --- synthetic inputs --------------------------------------------------------
 def morton_encode(x, y, z):
     return int(_lib.lib().svdag_morton_encode(x, y, z))

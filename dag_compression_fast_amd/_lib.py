@@ -55,6 +55,35 @@ class SvdagDag(ctypes.Structure):
     ]
 
 
+class TraceParams(ctypes.Structure):
+    """svdag_trace_params."""
+    _fields_ = [
+        ("debug_level", ctypes.c_uint32),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("camera_position", ctypes.c_double * 3),
+        ("ray_min", ctypes.c_double * 3),
+        ("ray_ddx", ctypes.c_double * 3),
+        ("ray_ddy", ctypes.c_double * 3),
+    ]
+
+    def as_dict(self):
+        d = {"debug_level": self.debug_level, "width": self.width, "height": self.height}
+        for f in ("camera_position", "ray_min", "ray_ddx", "ray_ddy"):
+            d[f] = tuple(getattr(self, f))
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        p = cls()
+        p.debug_level = int(d.get("debug_level", 0))
+        p.width, p.height = int(d["width"]), int(d["height"])
+        for f in ("camera_position", "ray_min", "ray_ddx", "ray_ddy"):
+            getattr(p, f)[:] = [float(v) for v in d[f]]
+        return p
+
+
 class SvdagError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(f"svdag error {STATUS_NAMES.get(code, code)}: {message}")
@@ -85,6 +114,7 @@ def bind_library(L):
     """Declares the C signatures of include/svdag.h on a loaded library."""
     vp, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int64
     dagp = ctypes.POINTER(SvdagDag)
+    dp = ctypes.POINTER(ctypes.c_double)
     sig = {
         "svdag_context_create": ([ctypes.POINTER(vp), vp], i32),
         "svdag_context_destroy": ([vp], i32),
@@ -124,6 +154,9 @@ def bind_library(L):
         "svdag_query_voxels": ([vp, vp, u64, i32, vp, u64, vp], i32),
         "svdag_voxelize_triangles": ([vp, vp, u64, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_free_keys": ([vp, vp], i32),
+        "svdag_trace_params_from_view": ([dp, dp, dp, dp, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, dp, dp,
+                                          i32, ctypes.c_uint32, ctypes.POINTER(TraceParams)], i32),
+        "svdag_trace_paths": ([vp, vp, u64, i32, ctypes.POINTER(TraceParams), vp, vp], i32),
         "svdag_gen_triangles": ([u64, i32, i32, ctypes.c_double, ctypes.c_double, vp], i32),
         "svdag_gen_voxelize": ([vp, u64, i32, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
     }

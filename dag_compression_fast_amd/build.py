@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libsvdag.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["svdag_sort.hip", "svdag_build.hip", "svdag_api.hip", "svdag_voxelize.hip"]
+HIP_SOURCES = ["svdag_sort.hip", "svdag_build.hip", "svdag_api.hip", "svdag_voxelize.hip", "svdag_trace.hip"]
 CXX_SOURCES = ["svdag_gen.cpp"]
 HEADERS = ["svdag_device.hpp", "svdag_internal.hpp"]
 

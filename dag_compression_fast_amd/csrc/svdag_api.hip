@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -32,7 +33,8 @@ const char* const kKernelNames[kNumKernelIds] = {
     "scan_counts",    "leaf_reduce", "leaf_compact",      "table_clear",        "leaf_insert",
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
-    "slot_list",      "query_voxels",    "vox_setup",   "vox_count",         "vox_write"};
+    "slot_list",      "query_voxels",    "vox_setup",   "vox_count",         "vox_write",
+    "trace_paths"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -1486,6 +1488,57 @@ int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_w
     launch_query_voxels(ctx->stream, d_words, n_words, depth, d_codes, n, d_out);
     HIP_TRY(hipGetLastError());
     return finish(ctx, "query_voxels");
+}
+
+int svdag_trace_params_from_view(const double* position, const double* right, const double* up,
+                                 const double* direction, double fov_degrees, uint32_t width, uint32_t height,
+                                 const double* aabb_min, const double* aabb_max, int32_t depth, uint32_t debug_level,
+                                 svdag_trace_params* out)
+{
+#pragma clang fp contract(off)
+    if (!position || !right || !up || !direction || !aabb_min || !aabb_max || !out)
+        return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    if (width == 0 || height == 0) return set_error(SVDAG_ERR_ARGUMENT, "empty image %ux%u", width, height);
+    if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
+    // src/DAGTracer.h:43-80, operation for operation.
+    const double fov = fov_degrees / 2.0 * (3.14159265358979323846 / 180.);
+    const double aspect = double(width) / double(height);
+    const double s = std::sin(fov), c = std::cos(fov);
+    svdag_trace_params p{};
+    p.debug_level = debug_level;
+    p.width = width;
+    p.height = height;
+    for (int a = 0; a < 3; ++a) {
+        const double x = right[a] * s * aspect, y = up[a] * s, z = direction[a] * c;
+        const double bottom_left = position[a] + z - y - x;
+        const double bottom_right = position[a] + z - y + x;
+        const double top_left = position[a] + z + y - x;
+        const double translation = -aabb_min[a];
+        const double scale = double(1 << depth) / (aabb_max[a] - aabb_min[a]);
+        const double f_bl = (bottom_left + translation) * scale;
+        p.camera_position[a] = (position[a] + translation) * scale;
+        p.ray_min[a] = f_bl;
+        p.ray_ddx[a] = ((bottom_right + translation) * scale - f_bl) * (1.0 / width);
+        p.ray_ddy[a] = ((top_left + translation) * scale - f_bl) * (1.0 / height);
+    }
+    *out = p;
+    return 0;
+}
+
+int svdag_trace_paths(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words, int32_t depth,
+                      const svdag_trace_params* params, uint32_t* d_paths, uint32_t* d_colors)
+{
+    TRY(check_ctx(ctx));
+    if (!params) return set_error(SVDAG_ERR_ARGUMENT, "null trace parameters");
+    if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
+    if (n_words && !d_words) return set_error(SVDAG_ERR_ARGUMENT, "null words");
+    if (params->width > 65535u * 16u || params->height > 65535u * 16u)
+        return set_error(SVDAG_ERR_ARGUMENT, "image %ux%u too large", params->width, params->height);
+    if (!d_paths && !d_colors) return 0;
+    if (params->width == 0 || params->height == 0) return 0;
+    launch_trace_paths(ctx->stream, d_words, n_words, depth, *params, d_paths, d_colors);
+    HIP_TRY(hipGetLastError());
+    return finish(ctx, "trace_paths");
 }
 
 int svdag_profile_enable(svdag_context* ctx, uint64_t kernel_mask)

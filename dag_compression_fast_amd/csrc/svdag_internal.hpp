@@ -45,6 +45,7 @@ enum KernelId : int {
     kKVoxSetup,
     kKVoxCount,
     kKVoxWrite,
+    kKTracePaths,
     kNumKernelIds
 };
 
@@ -346,6 +347,8 @@ void launch_vox_write(hipStream_t s, const TriSetup* setup, const uint64_t* col_
 
 // out[i] = occupancy of voxel codes[i] in the flat layout (1 set, 0 not set,
 // 2 the walk left the words).
+void launch_trace_paths(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth,
+                        const svdag_trace_params& p, uint32_t* paths, uint32_t* colors);
 void launch_query_voxels(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth, const uint64_t* codes,
                          uint64_t n, uint8_t* out);
 

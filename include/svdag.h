@@ -189,6 +189,42 @@ int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys);
 int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words,
                        int32_t depth, const uint64_t* d_codes, uint64_t n, uint8_t* d_out);
 
+/* Pinhole camera of the tracer, in grid units (the reference's
+ * FDAGTracer::TracePathsParams, src/DAGTracer.h:17-29): pixel (x, y) looks
+ * along ray_min + x * ray_ddx + y * ray_ddy - camera_position. */
+typedef struct svdag_trace_params {
+    uint32_t debug_level;  /* 0xFFFFFFFF: colour = low bytes of the hit position;
+                            * else MurmurHash32 of the word index of the node at
+                            * level depth - 3 - debug_level on the path */
+    uint32_t width;
+    uint32_t height;
+    uint32_t reserved;     /* 0 */
+    double camera_position[3];
+    double ray_min[3];
+    double ray_ddx[3];
+    double ray_ddy[3];
+} svdag_trace_params;
+
+/* The reference's FDAGTracer::GetTraceParams (src/DAGTracer.h:33-82): camera at
+ * `position` with the rows right, up, direction of its rotation, full vertical
+ * field of view `fov_degrees`, mapped into the 2^depth grid of the box
+ * [aabb_min, aabb_max]. */
+int svdag_trace_params_from_view(const double* position, const double* right, const double* up,
+                                 const double* direction, double fov_degrees, uint32_t width,
+                                 uint32_t height, const double* aabb_min, const double* aabb_max,
+                                 int32_t depth, uint32_t debug_level, svdag_trace_params* out);
+
+/* Primary rays through a flat layout (the reference's trace_paths kernel,
+ * src/DAGTracer.cu:177-428, launched by FDAGTracer::ResolvePaths, :430-444).
+ * Per pixel (row-major, y * width + x): d_paths[3 * pixel + 0..2] = grid x, y,
+ * z of the first voxel hit, (0, 0, 0) when the ray hits nothing; d_colors[pixel]
+ * = the reference's colour (rgb888 of the position's low bytes, or the debug
+ * hash).  A walk that leaves the words gives (0xFFFFFFFF x 3) and colour 0.
+ * Either output may be NULL; all pointers are device pointers; 3 <= depth <= 21. */
+int svdag_trace_paths(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words,
+                      int32_t depth, const svdag_trace_params* params, uint32_t* d_paths,
+                      uint32_t* d_colors);
+
 /* --- per-kernel timing ---------------------------------------------------- */
 
 /* Counterpart of the reference's per-operation timer FCudaScopePerfRecorder

@@ -35,6 +35,19 @@ class OraDag(ctypes.Structure):
     ]
 
 
+class OraTraceParams(ctypes.Structure):
+    _fields_ = [
+        ("debug_level", ctypes.c_uint32),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("camera_position", ctypes.c_double * 3),
+        ("ray_min", ctypes.c_double * 3),
+        ("ray_ddx", ctypes.c_double * 3),
+        ("ray_ddy", ctypes.c_double * 3),
+    ]
+
+
 def build():
     """Compile the oracle with its Makefile (g++ only)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -61,6 +74,9 @@ def lib():
                                      ctypes.POINTER(OraDag)]
         L.ora_create_final_dag.argtypes = [ctypes.POINTER(OraDag), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint32)),
                                            u64p]
+        L.ora_trace_paths.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                                      ctypes.POINTER(OraTraceParams), ctypes.c_int32, ctypes.c_void_p,
+                                      ctypes.c_void_p]
         L.ora_free_dag.argtypes = [ctypes.POINTER(OraDag)]
         L.ora_free.argtypes = [ctypes.c_void_p]
         L.ora_last_error.restype = ctypes.c_char_p
@@ -181,3 +197,21 @@ def hash_children(child_hashes):
     data = np.ascontiguousarray(child_hashes, dtype=np.uint64)
     assert data.shape == (8,)
     return int(lib().ora_hash_children(data.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+
+
+def trace_paths(words, depth, params, threads=8):
+    """Oracle trace_paths: params is a dict with the fields of OraTraceParams.
+    Returns (paths uint32 (height, width, 3), colors uint32 (height, width))."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    p = OraTraceParams()
+    p.debug_level = int(params.get("debug_level", 0))
+    p.width, p.height = int(params["width"]), int(params["height"])
+    for f in ("camera_position", "ray_min", "ray_ddx", "ray_ddy"):
+        getattr(p, f)[:] = [float(v) for v in params[f]]
+    paths = np.zeros((p.height, p.width, 3), dtype=np.uint32)
+    colors = np.zeros((p.height, p.width), dtype=np.uint32)
+    rc = lib().ora_trace_paths(w.ctypes.data if w.size else None, w.size, depth, ctypes.byref(p), threads,
+                               paths.ctypes.data, colors.ctypes.data)
+    if rc != 0:
+        raise OracleError("ora_trace_paths: bad arguments")
+    return paths, colors

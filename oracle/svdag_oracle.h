@@ -72,6 +72,24 @@ int ora_merge_dags(const ora_dag* dags, uint64_t count, int32_t threads,
  * *words is malloc'd; free with ora_free. */
 int ora_create_final_dag(const ora_dag* dag, uint32_t** words, uint64_t* n_words);
 
+/* Camera of the tracer in grid units, field for field the layout of
+ * svdag_trace_params (src/DAGTracer.h:17-29). */
+typedef struct ora_trace_params {
+    uint32_t debug_level, width, height, reserved;
+    double camera_position[3], ray_min[3], ray_ddx[3], ray_ddy[3];
+} ora_trace_params;
+
+/* trace_paths (src/DAGTracer.cu:177-428) over a flat layout: per pixel
+ * (row-major) the grid position of the first voxel hit ((0,0,0) for none,
+ * 0xFFFFFFFF x 3 when the walk leaves the words) and the reference's colour.
+ * Parity pins: every hit is a voxel of the decoded set, and it is the voxel
+ * the ray enters first up to float32 rounding, by brute force over all voxels
+ * (tests/test_trace.py).  Either output may be NULL.  Returns non-zero on bad
+ * arguments. */
+int ora_trace_paths(const uint32_t* words, uint64_t n_words, int32_t depth,
+                    const ora_trace_params* p, int32_t threads, uint32_t* paths,
+                    uint32_t* colors);
+
 void ora_free_dag(ora_dag* dag);
 void ora_free(void* p);
 const char* ora_last_error(void);
