@@ -1414,53 +1414,67 @@ int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t 
     *n_keys = 0;
     if (n_tris == 0) return 0;
     hipStream_t st = ctx->stream;
-    // Triangle stage in the arena: constants, column counts and offsets, and
-    // three counters (triangles, columns, keys) plus an error word.
+    // Triangle stage in the arena: constants, row counts and offsets, and the
+    // counters [triangles, rows, columns, keys, rectangle columns, error].
     Carver cv;
     const uint64_t o_setup = cv.take(sizeof(TriSetup) * n_tris);
-    const uint64_t o_cols = cv.take(4 * n_tris);
-    const uint64_t o_coff = cv.take(8 * n_tris);
-    const uint64_t o_cnt = cv.take(8 * 4);
+    const uint64_t o_rows = cv.take(4 * n_tris);
+    const uint64_t o_roff = cv.take(8 * n_tris);
+    const uint64_t o_cnt = cv.take(8 * 6);
     TRY(ensure_arena(ctx, cv.off));
     char* A = ctx->arena;
     TriSetup* setup = reinterpret_cast<TriSetup*>(A + o_setup);
-    uint32_t* cols = reinterpret_cast<uint32_t*>(A + o_cols);
-    uint64_t* coff = reinterpret_cast<uint64_t*>(A + o_coff);
-    uint64_t* cnt = reinterpret_cast<uint64_t*>(A + o_cnt);  // [n_tris, columns, keys, error]
-    launch_fill_u64(st, cnt, 4, 0);
+    uint32_t* rows = reinterpret_cast<uint32_t*>(A + o_rows);
+    uint64_t* roff = reinterpret_cast<uint64_t*>(A + o_roff);
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(A + o_cnt);
+    launch_fill_u64(st, cnt, 6, 0);
     launch_fill_u64(st, cnt, 1, n_tris);
-    launch_tri_setup(st, d_tris, n_tris, depth, setup, cols, reinterpret_cast<unsigned long long*>(cnt + 3));
-    launch_scan_counts(st, cols, coff, cnt, 1, cnt + 1);
-    uint64_t h[4] = {};
+    launch_tri_setup(st, d_tris, n_tris, depth, setup, rows, cnt + 4);
+    launch_scan_counts(st, rows, roff, cnt, 1, cnt + 1);
+    uint64_t h[6] = {};
     HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
     TRY(finish(ctx, "voxelize (triangle setup)"));
-    if (h[3]) return set_error(SVDAG_ERR_CAPACITY, "a triangle spans 2^32 or more voxel columns");
-    const uint64_t columns = h[1];
-    if (columns == 0) return 0;
-    // Tile counts and offsets, then the keys (the caller's, svdag_free_keys).
-    const uint64_t ntiles = tiles(columns, kVoxTileItems);
-    void* tile_mem = nullptr;
-    TRY(device_alloc(ctx, &tile_mem, align_up(4 * ntiles) + align_up(8 * ntiles) + 8 * (ntiles + 1)));
-    uint32_t* tcounts = static_cast<uint32_t*>(tile_mem);
-    uint64_t* toffs = reinterpret_cast<uint64_t*>(static_cast<char*>(tile_mem) + align_up(4 * ntiles));
-    uint64_t* ttri = reinterpret_cast<uint64_t*>(static_cast<char*>(tile_mem) + align_up(4 * ntiles) +
-                                                 align_up(8 * ntiles));
-    launch_vox_count(st, setup, coff, cnt, cnt + 1, columns, depth, ttri, tcounts);
-    launch_scan_counts(st, tcounts, toffs, cnt + 1, kVoxTileItems, cnt + 2);
+    const uint64_t n_rows = h[1], rect_columns = h[4];
+    if (n_rows == 0 || rect_columns == 0) return 0;
+    // Row table and tile arrays (sized for the rectangles' columns, which bound
+    // the kept ones), then the keys (the caller's, svdag_free_keys).
+    const uint64_t ntiles = tiles(rect_columns, kVoxTileItems);
+    Carver rows_cv;
+    const uint64_t o_tri = rows_cv.take(4 * n_rows), o_u = rows_cv.take(4 * n_rows), o_w0 = rows_cv.take(4 * n_rows);
+    const uint64_t o_cols = rows_cv.take(4 * n_rows), o_coff = rows_cv.take(8 * n_rows);
+    const uint64_t o_tcnt = rows_cv.take(4 * ntiles), o_toff = rows_cv.take(8 * ntiles), o_trow = rows_cv.take(8 * (ntiles + 1));
+    const uint64_t nrt = tiles(n_rows, kVoxTileItems);
+    const uint64_t o_rtc = rows_cv.take(4 * nrt), o_rto = rows_cv.take(8 * nrt);
+    void* mem = nullptr;
+    TRY(device_alloc(ctx, &mem, rows_cv.off));
+    char* M = static_cast<char*>(mem);
+    const VoxRows R{reinterpret_cast<uint32_t*>(M + o_tri), reinterpret_cast<uint32_t*>(M + o_u),
+                    reinterpret_cast<uint32_t*>(M + o_w0), reinterpret_cast<uint32_t*>(M + o_cols),
+                    reinterpret_cast<uint64_t*>(M + o_coff)};
+    uint32_t* tcounts = reinterpret_cast<uint32_t*>(M + o_tcnt);
+    uint64_t* toffs = reinterpret_cast<uint64_t*>(M + o_toff);
+    uint64_t* trow = reinterpret_cast<uint64_t*>(M + o_trow);
+    launch_row_setup(st, setup, roff, cnt, cnt + 1, n_rows, R, reinterpret_cast<uint32_t*>(M + o_rtc),
+                     reinterpret_cast<uint64_t*>(M + o_rto), cnt + 2, reinterpret_cast<unsigned long long*>(cnt + 5));
+    launch_vox_count(st, setup, R, cnt + 1, cnt + 2, rect_columns, depth, trow, tcounts);
+    launch_scan_counts(st, tcounts, toffs, cnt + 2, kVoxTileItems, cnt + 3);
     if (hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess) {
-        device_release(ctx, tile_mem);
+        device_release(ctx, mem);
         return set_error(SVDAG_ERR_HIP, "count read-back failed");
     }
     int rc = finish(ctx, "voxelize (count)");
-    const uint64_t keys = h[2];
+    const uint64_t keys = h[3];
     void* out = nullptr;
+    if (rc == 0 && h[5]) rc = set_error(SVDAG_ERR_CAPACITY, "a block of %llu rows spans 2^32 or more voxel columns",
+                                        (unsigned long long)kVoxTileItems);
+    if (rc == 0 && h[2] > rect_columns) rc = set_error(SVDAG_ERR_INTERNAL, "row ranges exceed the rectangles");
     if (rc == 0 && keys) rc = device_alloc(ctx, &out, 8 * keys);
     if (rc == 0 && keys) {
-        launch_vox_write(st, setup, coff, ttri, cnt, cnt + 1, columns, depth, toffs, static_cast<uint64_t*>(out));
+        launch_vox_write(st, setup, R, trow, cnt + 1, cnt + 2, rect_columns, depth, toffs, static_cast<uint64_t*>(out));
         rc = hipGetLastError() == hipSuccess ? finish(ctx, "voxelize (write)")
                                              : set_error(SVDAG_ERR_HIP, "voxelizer launch failed");
     }
-    device_release(ctx, tile_mem);
+    device_release(ctx, mem);
     if (rc != 0) {
         device_release(ctx, out);
         return rc;

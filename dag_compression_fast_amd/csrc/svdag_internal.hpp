@@ -330,25 +330,43 @@ struct TriSetup {
 };
 constexpr uint64_t kVoxTileItems = 2048;  // columns per voxeliser tile
 
-// cols[t] = columns of triangle t (tris: 9 doubles per triangle); a triangle
-// with 2^32 columns or more sets kErrInternal in *err.
-void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* cols,
-                      unsigned long long* err);
-// Voxels per tile of kVoxTileItems global columns; col_offsets = exclusive scan
-// of cols; *total_cols_dev columns in all, at most total_cols_bound.  tile_tri
-// (tiles + 1 entries) receives each tile's first triangle; the write pass
-// reads it.
-void launch_vox_count(hipStream_t s, const TriSetup* setup, const uint64_t* col_offsets, const uint64_t* n_tris_dev,
-                      const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth, uint64_t* tile_tri,
+// Per-row table of the triangles' column rectangles (a row runs along w).
+struct VoxRows {
+    uint32_t* tri;      // triangle of the row
+    uint32_t* u;        // its u coordinate
+    uint32_t* w0;       // first column kept
+    uint32_t* cols;     // columns kept (a conservative superset of those the rule can emit)
+    uint64_t* col_off;  // exclusive scan of cols
+};
+
+// rows[t] = rows of triangle t's rectangle (tris: 9 doubles per triangle);
+// *rect_columns += the rectangles' columns (a bound on the kept columns).
+void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* rows,
+                      uint64_t* rect_columns);
+// rows.tri / u / w0 / cols / col_off for the *n_rows_dev rows (n_rows on the
+// host), the column total into *total_cols_dev; row_off = exclusive scan of
+// the per-triangle row counts; tile_cols / tile_off: one entry per
+// kVoxTileItems rows; a tile of rows with 2^32 or more columns sets
+// kErrInternal in *err.
+void launch_row_setup(hipStream_t s, const TriSetup* setup, const uint64_t* row_off, const uint64_t* n_tris_dev,
+                      const uint64_t* n_rows_dev, uint64_t n_rows, const VoxRows& rows, uint32_t* tile_cols,
+                      uint64_t* tile_off, uint64_t* total_cols_dev, unsigned long long* err);
+// Voxels per tile of kVoxTileItems global columns (*total_cols_dev in all, at
+// most total_cols_bound).  tile_row (tiles + 1 entries) receives each tile's
+// first row; the write pass reads it.
+void launch_vox_count(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* n_rows_dev,
+                      const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth, uint64_t* tile_row,
                       uint32_t* tile_counts);
-void launch_vox_write(hipStream_t s, const TriSetup* setup, const uint64_t* col_offsets, const uint64_t* tile_tri,
-                      const uint64_t* n_tris_dev, const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth,
+void launch_vox_write(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* tile_row,
+                      const uint64_t* n_rows_dev, const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth,
                       const uint64_t* tile_offsets, uint64_t* keys);
+
+// Per pixel: the first voxel hit and its colour (the reference's trace_paths).
+void launch_trace_paths(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth,
+                        const svdag_trace_params& p, uint32_t* paths, uint32_t* colors);
 
 // out[i] = occupancy of voxel codes[i] in the flat layout (1 set, 0 not set,
 // 2 the walk left the words).
-void launch_trace_paths(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth,
-                        const svdag_trace_params& p, uint32_t* paths, uint32_t* colors);
 void launch_query_voxels(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth, const uint64_t* codes,
                          uint64_t n, uint8_t* out);
 

@@ -34,6 +34,25 @@ def edge_cases(side):
     ], dtype=np.float64)
 
 
+def near_axis_soup(side, count, seed):
+    """Triangles with an edge parallel to a grid axis up to 1e-13..1e-6: the
+    projected edge normals then have a tiny component, where the per-row column
+    ranges lean on their rounding margin."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        v0 = rng.uniform(1.0, side - 2.0, 3)
+        ax = rng.integers(0, 3)
+        v1 = v0.copy()
+        v1[ax] = rng.uniform(1.0, side - 2.0)
+        v1 += rng.choice([-1.0, 1.0], 3) * 10.0 ** rng.uniform(-13, -6, 3)
+        v2 = v0 + rng.uniform(-0.3, 0.3, 3) * side
+        if i % 3 == 0:  # also nearly parallel to the first edge
+            v2 = v0 + (v1 - v0) * rng.uniform(0.2, 0.9) + 10.0 ** rng.uniform(-12, -2, 3)
+        out.append([v0, v1, np.clip(v2, 0.0, side - 0.5)])
+    return np.array(out, dtype=np.float64)
+
+
 def unique_keys(buf, n):
     keys = buf.to_numpy(np.uint64, n)
     buf.free()
@@ -51,6 +70,7 @@ def test_voxelize_on_the_cpu_emulator():
         assert np.array_equal(host.gen_voxelize(tris, depth),
                               np.unique(host.gen_triangle_surfaces(seed, depth, count, mu, sigma, 0.0)))
     rng_cases.append((edge_cases(32), 5))
+    rng_cases.append((near_axis_soup(128, 150, 3), 7))
     for tris, depth in rng_cases:
         hostset[id(tris)] = host.gen_voxelize(tris, depth)
     import emu_ctypes
@@ -90,6 +110,6 @@ def test_voxelize_full_workload_on_gpu(gpu):
         buf.free()
     b = gpu.create_final_dag(gpu.create_dag(want, depth))
     assert np.array_equal(a, b)
-    for tris_e, d in ((edge_cases(64), 6), (edge_cases(2048), 11)):
+    for tris_e, d in ((edge_cases(64), 6), (edge_cases(2048), 11), (near_axis_soup(2048, 400, 4), 11)):
         buf, n = gpu.voxelize_triangles(tris_e, d)
         assert np.array_equal(unique_keys(buf, n), gpu.gen_voxelize(tris_e, d))
