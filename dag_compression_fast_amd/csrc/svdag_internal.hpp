@@ -106,8 +106,9 @@ inline uint64_t sort_tiles(uint64_t n, int ipt)
 inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) * kRadix; }
 inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound, kSortIptPairs); }
 
-// LSD radix sort of keys (and optionally u32 values) on the plan's bits; keys
-// equal on those bits keep their input order.
+// LSD radix sort of keys (and optionally u32 values) on the plan's bits.  With
+// values, keys equal on those bits keep their input order; without, their
+// order is unspecified (the first pass ranks unstably).
 // n is read from device memory (*n_ptr, < 2^30); n_bound is a host upper bound
 // sizing the grid and the count buffer.  Passes ping-pong between (keys_a,
 // keys_b); the first pass reads keys_in.  Returns which buffer holds the result

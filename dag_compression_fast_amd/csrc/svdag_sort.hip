@@ -7,10 +7,14 @@
 //   k_sort_scan     one workgroup per digit scans that digit's row of tile
 //                   counts and adds the digit's global start (from the
 //                   all-pass histogram, read once up front)
-//   k_sort_scatter  tile t ranks its keys stably (8 ballots per key give the
-//                   lanes sharing a digit; rank = popcount below the lane plus a
-//                   per-wave LDS counter), stages them in LDS in digit order and
-//                   writes runs to offset[digit][t].
+//   k_sort_scatter  tile t ranks its keys, stages them in LDS in digit order
+//                   and writes runs to offset[digit][t].  Stable ranking: the
+//                   lanes of a wave sharing a digit OR their bits into a per-wave
+//                   LDS word (rank = popcount below the lane plus a per-wave
+//                   digit counter).  The first pass of a key-only sort needs no
+//                   stability (its input order is arbitrary and later passes
+//                   are stable) and takes its rank from one LDS atomicAdd on the
+//                   tile's digit counter.
 // Tiles are assigned statically (blockIdx.x + k * gridDim.x).
 #include "svdag_device.hpp"
 #include "svdag_internal.hpp"
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
     }
 }
 
-template <typename K, bool PAIRS, int IPT, int THREADS>
+template <typename K, bool PAIRS, bool STABLE, int IPT, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                                const uint32_t* __restrict__ vals_in,
                                                                uint32_t* __restrict__ vals_out,
@@ -289,16 +293,18 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
     constexpr int W = THREADS / kWave;
     constexpr uint32_t kTile = uint32_t(THREADS) * IPT;
     constexpr int DPT = kRadix / THREADS;  // digit slots per thread in the tile scan
-    // The staging area holds the tile in digit order; during ranking it holds
-    // the per-wave match table s_match[W][kRadix] instead.
-    static_assert(W * kRadix * sizeof(uint64_t) <= kTile * sizeof(K), "match table fits the staging area");
     // Per-wave digit counts and tile-local digit starts never exceed the tile
-    // size, so 16 bits hold them (keeps the LDS under 40 KB: 4 workgroups per CU).
+    // size, so 16 bits hold them.
     static_assert(kTile <= 65535, "16-bit tile-local counters");
     __shared__ K s_keys[kTile];
     __shared__ uint32_t s_vals[PAIRS ? kTile : 1];
-    __shared__ uint16_t s_wcnt[W][kRadix];
-    uint64_t* const s_match = reinterpret_cast<uint64_t*>(s_keys) + (threadIdx.x / kWave) * kRadix;
+    // Stable ranking: per-wave digit counts and the per-wave match table (zeroed
+    // once; each key's ranking clears the word it set, so it is zero again at
+    // the end of every tile).  Unstable ranking: the tile's digit counts.
+    __shared__ uint16_t s_wcnt[STABLE ? W : 1][kRadix];
+    __shared__ uint64_t s_match_all[STABLE ? W * kRadix : 1];
+    __shared__ uint32_t s_dcnt[STABLE ? 1 : kRadix];
+    uint64_t* const s_match = s_match_all + (STABLE ? (threadIdx.x / kWave) * kRadix : 0);
     __shared__ uint16_t s_lstart[kRadix];
     __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_scan[W];
@@ -345,11 +351,17 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         }
     };
     load_tile(walk.tile);
+    if constexpr (STABLE)
+        for (uint32_t i = tid; i < W * kRadix; i += THREADS) s_match_all[i] = 0;
 
+    // Each tile: clear the counters, rank, digit starts, stage, store.  The
+    // barrier after the clear also orders the previous tile's stores (reads of
+    // s_keys and s_base) before this tile's writes of them.
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
-        for (uint32_t i = tid; i < W * kRadix; i += THREADS) {
-            (&s_wcnt[0][0])[i] = 0;
-            reinterpret_cast<uint64_t*>(s_keys)[i] = 0;
+        if constexpr (STABLE) {
+            for (uint32_t i = tid; i < W * kRadix; i += THREADS) (&s_wcnt[0][0])[i] = 0;
+        } else {
+            for (uint32_t i = tid; i < kRadix; i += THREADS) s_dcnt[i] = 0;
         }
         __syncthreads();
         const uint64_t tile_base = tile * kTile;
@@ -359,11 +371,20 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         // the common case) drops the per-key bounds checks and their registers.
         auto rank_and_stage = [&](auto full_tag) {
         constexpr bool FULL = decltype(full_tag)::value;
-        // Per key: (rank among the wave's keys of its digit) << 16 | digit.
+        // Per key: (rank among the wave's / tile's keys of its digit) << 16 | digit.
         uint32_t dr[IPT];
-        uint16_t* wc = s_wcnt[w];
+        uint16_t* wc = s_wcnt[STABLE ? w : 0];
+        if constexpr (!STABLE) {
 #pragma unroll
-        for (int i = 0; i < IPT; ++i) {
+            for (int i = 0; i < IPT; ++i) {
+                const bool valid = FULL || wave_base + i * kWave + lane < tile_n;
+                const uint32_t d = uint32_t(key[i] >> shift) & digit_mask;
+                const uint32_t r = valid ? atomicAdd(&s_dcnt[d], 1u) : 0u;
+                dr[i] = r << 16 | d;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < (STABLE ? IPT : 0); ++i) {
             const bool valid = FULL || wave_base + i * kWave + lane < tile_n;
             const uint32_t d = uint32_t(key[i] >> shift) & digit_mask;
             // Lanes holding the same digit: every lane ORs its bit into the wave's
@@ -396,11 +417,15 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
             const uint32_t d = tid * DPT + k;
             uint32_t total = 0;
             if (d < radix) {
+                if constexpr (STABLE) {
 #pragma unroll
-                for (int i = 0; i < W; ++i) {
-                    const uint32_t c = s_wcnt[i][d];
-                    s_wcnt[i][d] = uint16_t(total);
-                    total += c;
+                    for (int i = 0; i < W; ++i) {
+                        const uint32_t c = s_wcnt[i][d];
+                        s_wcnt[i][d] = uint16_t(total);
+                        total += c;
+                    }
+                } else {
+                    total = s_dcnt[d];
                 }
             }
             dtot[k] = total;
@@ -423,7 +448,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         for (int i = 0; i < IPT; ++i) {
             if (FULL || wave_base + i * kWave + lane < tile_n) {
                 const uint32_t d = dr[i] & 0xFFFFu;
-                const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + (dr[i] >> 16);
+                const uint32_t pos = s_lstart[d] + (STABLE ? uint32_t(s_wcnt[w][d]) : 0u) + (dr[i] >> 16);
                 s_keys[pos] = key[i];
                 if constexpr (PAIRS) s_vals[pos] = val[i];
             }
@@ -447,7 +472,6 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         } else {
             for (uint32_t j = tid; j < tile_n; j += THREADS) store(j);
         }
-        __syncthreads();
     }
 }
 
@@ -466,11 +490,14 @@ int device_cu_count()
     return cus;
 }
 
+constexpr unsigned kUnstableExtraLds = 20 * 1024;
+
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
-                        uint64_t stride, uint32_t grid, bool counted, const ChunkPrefix& prefix)
+                        uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix)
 {
+    const bool counted = first;  // the histogram kernel counted the first pass's tiles
     if (!counted) {
         TimedLaunch t(kKSortCount, stream);
         hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
@@ -483,12 +510,20 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     }
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
-                           src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, true, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0,
+                           stream, src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
+    } else if (first) {
+        // Key-only first pass: the order within a digit is free.  Its kernel
+        // needs fewer registers and would fit three workgroups per CU; the extra
+        // LDS holds it to two, as the stable pass, which measured faster (more
+        // tiles in flight spread the scattered writes over more open lines).
+        TimedLaunch t(kKSortScatterKeys, stream);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, false, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads),
+                           kUnstableExtraLds, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0, stream,
-                           src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0,
+                           stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     }
 }
 
