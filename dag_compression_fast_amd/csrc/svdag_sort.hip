@@ -498,6 +498,16 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
                         uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix)
 {
     const bool counted = first;  // the histogram kernel counted the first pass's tiles
+    // Key sorts (tens of millions of keys) launch about one workgroup per tile
+    // (up to 64 per CU): the dispatcher then balances the tiles, which measured
+    // 4% faster than 8 per CU walking several tiles each.  Key/value sorts are
+    // small and sized from a loose bound, where the empty workgroups would cost
+    // more than they balance.
+    const uint32_t scatter_grid =
+        vsrc ? grid
+             : uint32_t(std::max<uint64_t>(
+                   kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(device_cu_count()) * 64) /
+                              kXcds * kXcds));
     if (!counted) {
         TimedLaunch t(kKSortCount, stream);
         hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
@@ -510,7 +520,7 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     }
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, true, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0,
+        hipLaunchKernelGGL((k_sort_scatter<K, true, true, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads), 0,
                            stream, src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
     } else if (first) {
         // Key-only first pass: the order within a digit is free.  Its kernel
@@ -518,11 +528,11 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
         // LDS holds it to two, as the stable pass, which measured faster (more
         // tiles in flight spread the scattered writes over more open lines).
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, false, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads),
+        hipLaunchKernelGGL((k_sort_scatter<K, false, false, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads),
                            kUnstableExtraLds, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, true, IPT, kSortThreads>), dim3(grid), dim3(kSortThreads), 0,
+        hipLaunchKernelGGL((k_sort_scatter<K, false, true, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads), 0,
                            stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
     }
 }
