@@ -274,6 +274,34 @@ def bench_chunks(svd, _lib, args):
 
     t_plain, w_plain = timed(plain, args.chunk_reps)
     log(f"chunks plain: {t_plain * 1e3:.1f} ms")
+    tuned = None
+    if not args.no_autotune:
+        # Digit width, sort tile and table load factor for the batched build,
+        # picked by timing it (every candidate must give the same DAG).
+        from dag_compression_fast_amd import autotune
+        state = {}
+
+        def tune_build(c):
+            _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+            dag = batched()
+            if "words" in state:
+                L.svdag_free_words(ctx.handle, state.pop("words")[0])
+            state["words"] = device_words(L, _lib, ctx, dag)
+            L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+
+        def tune_fetch():
+            return fetch_words(L, _lib, ctx, *state.pop("words"))
+
+        t0 = time.perf_counter()
+        res = autotune.autotune(ctx, tune_build, tune_fetch, reps=3)
+        if "words" in state:
+            L.svdag_free_words(ctx.handle, state.pop("words")[0])
+        tuned = {"best": res["best"], "best_ms": round(res["best_ms"], 3), "candidates": len(res["table"]),
+                 "default_ms": round(dict((tuple(sorted(st.items())), ms) for st, ms in res["table"])
+                                     [tuple(sorted(autotune.DEFAULTS.items()))], 3),
+                 "tune_s": round(time.perf_counter() - t0, 2)}
+        log(f"autotune: {res['best']} {res['best_ms']:.2f} ms (default {tuned['default_ms']:.2f} ms) "
+            f"over {len(res['table'])} settings in {tuned['tune_s']:.1f} s")
     t_batch, w_batch = timed(batched, args.chunk_reps)
     log(f"chunks batched: {t_batch * 1e3:.1f} ms")
     if args.kernel_table:
@@ -315,7 +343,7 @@ def bench_chunks(svd, _lib, args):
     return {"workload": CHUNKS["desc"], "n_keys": total, "plain_voxels_per_s": round(total / t_plain, 1),
             "plain_ms": round(t_plain * 1e3, 2), "batched_voxels_per_s": round(total / t_batch, 1),
             "batched_ms": round(t_batch * 1e3, 2), "batched_vs_plain": round(t_plain / t_batch, 2),
-            "served": served, "serve_slots": args.serve_slots,
+            "served": served, "serve_slots": args.serve_slots, "autotune": tuned,
             "check": "plain, batched and served final DAGs identical" if same else "MISMATCH"}
 
 
@@ -349,6 +377,7 @@ def main():
     ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
     ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
     ap.add_argument("--no-trace", action="store_true", help="skip the ray tracing timing")
+    ap.add_argument("--no-autotune", action="store_true", help="batched mode with the default options")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
