@@ -5,8 +5,8 @@
 // bits wide.  No workgroup ever waits on another one: every pass is
 //   k_sort_count    tile t (a fixed range of 4096 keys) -> per-digit counts
 //   k_sort_scan     one workgroup per digit scans that digit's row of tile
-//                   counts and adds the digit's global start (from the
-//                   all-pass histogram, read once up front)
+//                   counts and adds the digit's global start (from the pass's
+//                   digit totals, summed by the counting kernel)
 //   k_sort_scatter  tile t ranks its keys, stages them in LDS in digit order
 //                   and writes runs to offset[digit][t].  Stable ranking: the
 //                   lanes of a wave sharing a digit OR their bits into a per-wave
@@ -104,8 +104,8 @@ struct HistPlan {
 };
 
 // One read of the keys gives the first pass's per-tile digit counts
-// (counts[d * stride + tile], walked XCD by XCD like the count kernel), the
-// digit totals of every pass, and the OR of the keys (range check).
+// (counts[d * stride + tile], walked XCD by XCD like the count kernel) and
+// digit totals, and the OR of the keys (range check).
 template <typename K, int IPT>
 __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict__ keys,
                                                             const uint64_t* __restrict__ n_ptr, HistPlan plan,
@@ -115,9 +115,9 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
                                                             ChunkPrefix prefix)
 {
     constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
-    __shared__ uint32_t s_hist[kMaxPasses][kRadix];
+    __shared__ uint32_t s_hist[kRadix];
     __shared__ uint32_t s_cnt[kRadix];
-    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortThreads) (&s_hist[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < kRadix; i += kSortThreads) s_hist[i] = 0;
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix0 = plan.mask[0] + 1;
@@ -163,15 +163,13 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
             if (uint32_t(i) * kSortThreads + threadIdx.x < cnt) {
                 acc |= k[i];
                 atomicAdd(&s_cnt[uint32_t(k[i] >> plan.shift[0]) & plan.mask[0]], 1u);
-                for (int p = 1; p < plan.passes; ++p)
-                    atomicAdd(&s_hist[p][uint32_t(k[i] >> plan.shift[p]) & plan.mask[p]], 1u);
             }
         }
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) {
             const uint32_t c = s_cnt[d];
             counts[uint64_t(d) * stride + tile] = c;
-            s_hist[0][d] += c;
+            s_hist[d] += c;
         }
         __syncthreads();
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
@@ -180,11 +178,10 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
             for (int i = 0; i < IPT; ++i) k[i] = nxt[i];
         }
     }
-    for (int p = 0; p < plan.passes; ++p)
-        for (uint32_t d = threadIdx.x; d <= plan.mask[p]; d += kSortThreads) {
-            const uint32_t v = s_hist[p][d];
-            if (v) atomicAdd(&hist[p * kRadix + d], v);
-        }
+    for (uint32_t d = threadIdx.x; d <= plan.mask[0]; d += kSortThreads) {
+        const uint32_t v = s_hist[d];
+        if (v) atomicAdd(&hist[d], v);
+    }
     if (key_or) {
         uint64_t a = uint64_t(acc);
 #pragma unroll
@@ -193,16 +190,19 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
     }
 }
 
-// Per-tile digit counts, digit-major: counts[d * stride + tile].  Tiles are
-// walked XCD by XCD, so the counts of neighbouring tiles are written from one L2.
+// Per-tile digit counts, digit-major: counts[d * stride + tile], and the
+// pass's digit totals added into hist[d].  Tiles are walked XCD by XCD, so the
+// counts of neighbouring tiles are written from one L2.
 template <typename K, int IPT>
 __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict__ keys,
                                                              const uint64_t* __restrict__ n_ptr, uint32_t shift,
                                                              uint32_t digit_mask, uint32_t* __restrict__ counts,
-                                                             uint64_t stride)
+                                                             uint64_t stride, uint32_t* __restrict__ hist)
 {
     constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
     __shared__ uint32_t s_cnt[kRadix];
+    __shared__ uint32_t s_tot[kRadix];
+    for (int i = threadIdx.x; i < kRadix; i += kSortThreads) s_tot[i] = 0;
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix = digit_mask + 1;
@@ -230,13 +230,21 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
             if (base + uint64_t(i) * kSortThreads + threadIdx.x < n)
                 atomicAdd(&s_cnt[uint32_t(cur[i] >> shift) & digit_mask], 1u);
         __syncthreads();
-        for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) counts[uint64_t(d) * stride + tile] = s_cnt[d];
+        for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
+            const uint32_t c = s_cnt[d];
+            counts[uint64_t(d) * stride + tile] = c;
+            s_tot[d] += c;
+        }
         __syncthreads();
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         if (more) {
 #pragma unroll
             for (int i = 0; i < IPT; ++i) cur[i] = nxt[i];
         }
+    }
+    for (uint32_t d = threadIdx.x; d < radix; d += kSortThreads) {
+        const uint32_t v = s_tot[d];
+        if (v) atomicAdd(&hist[d], v);
     }
 }
 
@@ -494,7 +502,7 @@ constexpr unsigned kUnstableExtraLds = 20 * 1024;
 
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
-                        const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, const uint32_t* hist,
+                        const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, uint32_t* hist,
                         uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix)
 {
     const bool counted = first;  // the histogram kernel counted the first pass's tiles
@@ -511,7 +519,7 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     if (!counted) {
         TimedLaunch t(kKSortCount, stream);
         hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
-                           counts, stride);
+                           counts, stride, hist);
     }
     {
         TimedLaunch t(kKSortScan, stream);
