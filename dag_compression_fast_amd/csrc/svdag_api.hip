@@ -586,16 +586,27 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         ls = ctx->side;
         forked = true;
     }
-    if (launch_sort<uint64_t>(ls, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
-                              &c->unique_leaves, L.leaf_bound, reinterpret_cast<uint32_t*>(A + L.leaf_sort_zero),
-                              sort_counts, nullptr) < 0)
-        return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
-    launch_rank_of_slot(ls, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
+    // The sort's launches are issued after the first level build's: the side
+    // stream already waits for the leaf dedup (fork_ev), and issuing the
+    // build first keeps the main stream busy while the host issues the sort's
+    // two dozen small launches.
+    bool ordered = false;
+    auto order_leaves = [&]() -> int {
+        if (ordered) return 0;
+        ordered = true;
+        if (launch_sort<uint64_t>(ls, L.leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted_slots,
+                                  &c->unique_leaves, L.leaf_bound, reinterpret_cast<uint32_t*>(A + L.leaf_sort_zero),
+                                  sort_counts, nullptr) < 0)
+            return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
+        launch_rank_of_slot(ls, sorted_slots, &c->unique_leaves, leaf_rank, L.leaf_bound);
+        if (forked && hipEventRecord(ctx->join_ev, ls) != hipSuccess)
+            return fail(set_error(SVDAG_ERR_HIP, "event record failed"));
+        return 0;
+    };
     bool joined = !forked;
-    if (forked && hipEventRecord(ctx->join_ev, ls) != hipSuccess)
-        return fail(set_error(SVDAG_ERR_HIP, "event record failed"));
     // Before the first use of the leaf order.
     auto join = [&]() -> int {
+        TRY(order_leaves());
         if (joined) return 0;
         joined = true;
         if (hipStreamWaitEvent(st, ctx->join_ev, 0) != hipSuccess)
@@ -653,6 +664,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         a.counters = c;
         launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : nodes_bound(l + 1));
         mark(ctx, "level build");
+        TRY(order_leaves());
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
                                &c->collisions, nodes_bound(l));
