@@ -289,15 +289,24 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
     }
 }
 
-template <typename K, bool PAIRS, bool STABLE, int IPT, int THREADS>
+// Ranking of a scatter pass: unstable (the first pass of a key-only sort),
+// stable, or adaptive: unstable for a tile whose keys all agree on the bits
+// the earlier passes sorted (their order within a digit is then free; the
+// tiles follow the input order), stable otherwise.
+enum RankMode { kRankUnstable = 0, kRankStable = 1, kRankAdaptive = 2 };
+
+template <typename K, bool PAIRS, int MODE, int IPT, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                                const uint32_t* __restrict__ vals_in,
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint64_t* __restrict__ n_ptr, uint32_t shift,
                                                                uint32_t digit_mask,
                                                                const uint32_t* __restrict__ offsets, uint64_t stride,
-                                                               ChunkPrefix prefix)
+                                                               ChunkPrefix prefix, uint32_t sorted_shift,
+                                                               uint32_t sorted_mask)
 {
+    constexpr bool MATCH = MODE != kRankUnstable;  // needs the stable ranking's tables
+    constexpr bool DCNT = MODE != kRankStable;     // needs the unstable ranking's counters
     constexpr int W = THREADS / kWave;
     constexpr uint32_t kTile = uint32_t(THREADS) * IPT;
     constexpr int DPT = kRadix / THREADS;  // digit slots per thread in the tile scan
@@ -309,10 +318,11 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
     // Stable ranking: per-wave digit counts and the per-wave match table (zeroed
     // once; each key's ranking clears the word it set, so it is zero again at
     // the end of every tile).  Unstable ranking: the tile's digit counts.
-    __shared__ uint16_t s_wcnt[STABLE ? W : 1][kRadix];
-    __shared__ uint64_t s_match_all[STABLE ? W * kRadix : 1];
-    __shared__ uint32_t s_dcnt[STABLE ? 1 : kRadix];
-    uint64_t* const s_match = s_match_all + (STABLE ? (threadIdx.x / kWave) * kRadix : 0);
+    __shared__ uint16_t s_wcnt[MATCH ? W : 1][kRadix];
+    __shared__ uint64_t s_match_all[MATCH ? W * kRadix : 1];
+    __shared__ uint32_t s_dcnt[DCNT ? kRadix : 1];
+    __shared__ uint32_t s_span[2 * W];  // per wave: min and max of the sorted bits
+    uint64_t* const s_match = s_match_all + (MATCH ? (threadIdx.x / kWave) * kRadix : 0);
     __shared__ uint16_t s_lstart[kRadix];
     __shared__ uint32_t s_base[kRadix];
     __shared__ uint32_t s_scan[W];
@@ -359,21 +369,50 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         }
     };
     load_tile(walk.tile);
-    if constexpr (STABLE)
+    if constexpr (MATCH)
         for (uint32_t i = tid; i < W * kRadix; i += THREADS) s_match_all[i] = 0;
 
     // Each tile: clear the counters, rank, digit starts, stage, store.  The
     // barrier after the clear also orders the previous tile's stores (reads of
     // s_keys and s_base) before this tile's writes of them.
     for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
-        if constexpr (STABLE) {
-            for (uint32_t i = tid; i < W * kRadix; i += THREADS) (&s_wcnt[0][0])[i] = 0;
-        } else {
-            for (uint32_t i = tid; i < kRadix; i += THREADS) s_dcnt[i] = 0;
-        }
-        __syncthreads();
         const uint64_t tile_base = tile * kTile;
         const uint32_t tile_n = uint32_t(min(uint64_t(kTile), n - tile_base));
+        if constexpr (MODE == kRankAdaptive) {
+            // The span of the earlier passes' bits over the tile's keys.
+            uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+            for (int i = 0; i < IPT; ++i)
+                if (wave_base + i * kWave + lane < tile_n) {
+                    const uint32_t v = uint32_t(key[i] >> sorted_shift) & sorted_mask;
+                    lo = min(lo, v);
+                    hi = max(hi, v);
+                }
+#pragma unroll
+            for (int o = kWave / 2; o >= 1; o >>= 1) {
+                lo = min(lo, uint32_t(__shfl_xor(int(lo), o, kWave)));
+                hi = max(hi, uint32_t(__shfl_xor(int(hi), o, kWave)));
+            }
+            if (lane == 0) {
+                s_span[w] = lo;
+                s_span[W + w] = hi;
+            }
+        }
+        if constexpr (MATCH)
+            for (uint32_t i = tid; i < W * kRadix; i += THREADS) (&s_wcnt[0][0])[i] = 0;
+        if constexpr (DCNT)
+            for (uint32_t i = tid; i < kRadix; i += THREADS) s_dcnt[i] = 0;
+        __syncthreads();
+        bool stable = MODE == kRankStable;
+        if constexpr (MODE == kRankAdaptive) {
+            uint32_t lo = s_span[0], hi = s_span[W];
+#pragma unroll
+            for (int i = 1; i < W; ++i) {
+                lo = min(lo, s_span[i]);
+                hi = max(hi, s_span[W + i]);
+            }
+            stable = lo != hi;
+        }
 
         // Ranking, digit starts and staging; FULL (every key of the tile present,
         // the common case) drops the per-key bounds checks and their registers.
@@ -381,8 +420,8 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         constexpr bool FULL = decltype(full_tag)::value;
         // Per key: (rank among the wave's / tile's keys of its digit) << 16 | digit.
         uint32_t dr[IPT];
-        uint16_t* wc = s_wcnt[STABLE ? w : 0];
-        if constexpr (!STABLE) {
+        uint16_t* wc = s_wcnt[MATCH ? w : 0];
+        if (DCNT && !stable) {
 #pragma unroll
             for (int i = 0; i < IPT; ++i) {
                 const bool valid = FULL || wave_base + i * kWave + lane < tile_n;
@@ -391,8 +430,9 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
                 dr[i] = r << 16 | d;
             }
         }
+        if (MATCH && stable) {
 #pragma unroll
-        for (int i = 0; i < (STABLE ? IPT : 0); ++i) {
+        for (int i = 0; i < (MATCH ? IPT : 0); ++i) {
             const bool valid = FULL || wave_base + i * kWave + lane < tile_n;
             const uint32_t d = uint32_t(key[i] >> shift) & digit_mask;
             // Lanes holding the same digit: every lane ORs its bit into the wave's
@@ -414,6 +454,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
             if (valid && (match >> lane) == 1ull) wc[d] = uint16_t(c + uint32_t(__popcll(match)));
             __builtin_amdgcn_wave_barrier();
         }
+        }
         __syncthreads();
 
         // Per digit: wave counts -> exclusive per-wave starts; tile-local digit
@@ -425,14 +466,14 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
             const uint32_t d = tid * DPT + k;
             uint32_t total = 0;
             if (d < radix) {
-                if constexpr (STABLE) {
+                if (MATCH && stable) {
 #pragma unroll
                     for (int i = 0; i < W; ++i) {
                         const uint32_t c = s_wcnt[i][d];
                         s_wcnt[i][d] = uint16_t(total);
                         total += c;
                     }
-                } else {
+                } else if constexpr (DCNT) {
                     total = s_dcnt[d];
                 }
             }
@@ -456,7 +497,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
         for (int i = 0; i < IPT; ++i) {
             if (FULL || wave_base + i * kWave + lane < tile_n) {
                 const uint32_t d = dr[i] & 0xFFFFu;
-                const uint32_t pos = s_lstart[d] + (STABLE ? uint32_t(s_wcnt[w][d]) : 0u) + (dr[i] >> 16);
+                const uint32_t pos = s_lstart[d] + (MATCH && stable ? uint32_t(s_wcnt[w][d]) : 0u) + (dr[i] >> 16);
                 s_keys[pos] = key[i];
                 if constexpr (PAIRS) s_vals[pos] = val[i];
             }
@@ -503,9 +544,13 @@ constexpr unsigned kUnstableExtraLds = 20 * 1024;
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, uint32_t* hist,
-                        uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix)
+                        uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix, uint32_t sorted_lo)
 {
     const bool counted = first;  // the histogram kernel counted the first pass's tiles
+    // Bits the earlier passes sorted: [sorted_lo, shift).  The adaptive ranking
+    // compares them as one 32-bit word, so wider spans rank stably.
+    const bool span_fits = shift > sorted_lo && shift - sorted_lo <= 31;
+    const uint32_t sorted_mask = span_fits ? (1u << (shift - sorted_lo)) - 1u : 0u;
     // Key sorts (tens of millions of keys) launch about one workgroup per tile
     // (up to 64 per CU): the dispatcher then balances the tiles, which measured
     // 4% faster than 8 per CU walking several tiles each.  Key/value sorts are
@@ -528,20 +573,31 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     }
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, true, true, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads), 0,
-                           stream, src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, true, kRankStable, IPT, kSortThreads>), dim3(scatter_grid),
+                           dim3(kSortThreads), 0, stream, src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride,
+                           prefix, sorted_lo, sorted_mask);
     } else if (first) {
         // Key-only first pass: the order within a digit is free.  Its kernel
         // needs fewer registers and would fit three workgroups per CU; the extra
         // LDS holds it to two, as the stable pass, which measured faster (more
         // tiles in flight spread the scattered writes over more open lines).
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, false, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads),
-                           kUnstableExtraLds, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, kRankUnstable, IPT, kSortThreads>), dim3(scatter_grid),
+                           dim3(kSortThreads), kUnstableExtraLds, stream, src, dst, nullptr, nullptr, n_ptr, shift,
+                           dmask, counts, stride, prefix, sorted_lo, sorted_mask);
+    } else if (span_fits) {
+        // Later key-only passes: tiles uniform in the earlier passes' bits (the
+        // common case: the input is sorted on them, in runs longer than a tile)
+        // rank unstably, the others stably.
+        TimedLaunch t(kKSortScatterKeys, stream);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, kRankAdaptive, IPT, kSortThreads>), dim3(scatter_grid),
+                           dim3(kSortThreads), 0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts,
+                           stride, prefix, sorted_lo, sorted_mask);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
-        hipLaunchKernelGGL((k_sort_scatter<K, false, true, IPT, kSortThreads>), dim3(scatter_grid), dim3(kSortThreads), 0,
-                           stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts, stride, prefix);
+        hipLaunchKernelGGL((k_sort_scatter<K, false, kRankStable, IPT, kSortThreads>), dim3(scatter_grid),
+                           dim3(kSortThreads), 0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts,
+                           stride, prefix, sorted_lo, sorted_mask);
     }
 }
 
@@ -591,7 +647,7 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
             if (last && final_vals) vdst = final_vals;
             const uint32_t dmask = (1u << plan.bits[p]) - 1u;
             launch_pass<K, I>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts, hist + p * kRadix,
-                              stride, grid, p == 0, p == 0 ? prefix : ChunkPrefix{});
+                              stride, grid, p == 0, p == 0 ? prefix : ChunkPrefix{}, plan.shift[0]);
             src = dst;
             vsrc = vdst;
         }

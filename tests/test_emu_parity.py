@@ -58,6 +58,25 @@ def test_emu_clustered_and_runs(emu, oracle):
                                              [20000, 9000, 3]), 9)
 
 
+def test_emu_long_runs_in_sorted_bits(emu, oracle):
+    """Later sort passes rank a tile unstably when all its keys agree on the
+    bits the earlier passes sorted.  Long runs of few values in those bits make
+    most tiles uniform, with mixed tiles at the run boundaries."""
+    rng = np.random.default_rng(5)
+    n = 60000
+    # depth 10: bits 15..30 in two passes (15..23, 23..30)
+    d1 = rng.choice(np.array([3, 4, 200], dtype=np.uint64), size=n, p=[0.5, 0.2, 0.3])
+    d2 = rng.integers(0, 1 << 7, size=n, dtype=np.uint64)
+    low = rng.integers(0, 1 << 15, size=n, dtype=np.uint64)
+    keys = (d2 << np.uint64(23)) | (d1 << np.uint64(15)) | low
+    build_and_compare(emu, oracle, np.concatenate([keys, keys[:n // 4]]), 10)
+    # depth 12: bits 15..36 in three passes of 7; the third sees the first two
+    a = rng.choice(np.array([5, 6, 0x3FFF], dtype=np.uint64), size=n, p=[0.4, 0.4, 0.2])
+    hi = rng.integers(0, 1 << 7, size=n, dtype=np.uint64)
+    keys = (hi << np.uint64(29)) | (a << np.uint64(15)) | rng.integers(0, 1 << 15, size=n, dtype=np.uint64)
+    build_and_compare(emu, oracle, keys, 12)
+
+
 def test_emu_u32(emu, oracle):
     rng = np.random.default_rng(4)
     keys = rng.integers(0, 8 ** 9, size=20000, dtype=np.uint64)
