@@ -26,6 +26,7 @@ OPT_WAIT_SECONDS = 5
 OPT_RADIX_BITS = 6
 OPT_SORT_TILE = 7
 OPT_TABLE_ESTIMATE = 8
+OPT_VOXEL_STAGE_BYTES = 9
 
 STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "HIP", 3: "MEMORY", 4: "INTERNAL", 5: "CAPACITY"}
 

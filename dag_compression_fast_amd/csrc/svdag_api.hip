@@ -138,6 +138,7 @@ struct svdag_context {
     int radix_bits = 9;
     int sort_ipt = kSortIptSmall;
     bool table_estimate = true;
+    uint64_t vox_stage_max = kVoxStageMaxBytes;
     int host_threads = 1;
     // Stage markers of the work in flight (bounded completion wait).
     static constexpr int kMaxMarks = 64;
@@ -1069,6 +1070,10 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         ctx->radix_bits = int(value);
         return 0;
     case SVDAG_OPT_TABLE_ESTIMATE: ctx->table_estimate = value != 0; return 0;
+    case SVDAG_OPT_VOXEL_STAGE_BYTES:
+        if (value < 0) return set_error(SVDAG_ERR_ARGUMENT, "stage bytes must be >= 0");
+        ctx->vox_stage_max = uint64_t(value);
+        return 0;
     case SVDAG_OPT_SORT_TILE:
         if (value != 4096 && value != 8192) return set_error(SVDAG_ERR_ARGUMENT, "sort tile must be 4096 or 8192 keys");
         ctx->sort_ipt = value == 8192 ? kSortIptLarge : kSortIptSmall;
@@ -1468,24 +1473,47 @@ int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t 
     uint64_t* trow = reinterpret_cast<uint64_t*>(M + o_trow);
     launch_row_setup(st, setup, roff, cnt, cnt + 1, n_rows, R, reinterpret_cast<uint32_t*>(M + o_rtc),
                      reinterpret_cast<uint64_t*>(M + o_rto), cnt + 2, reinterpret_cast<unsigned long long*>(cnt + 5));
-    launch_vox_count(st, setup, R, cnt + 1, cnt + 2, rect_columns, depth, trow, tcounts);
-    launch_scan_counts(st, tcounts, toffs, cnt + 2, kVoxTileItems, cnt + 3);
-    if (hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess) {
+    uint64_t h2[6] = {};
+    if (hipMemcpyAsync(h2, cnt, sizeof(h2), hipMemcpyDeviceToHost, st) != hipSuccess) {
         device_release(ctx, mem);
-        return set_error(SVDAG_ERR_HIP, "count read-back failed");
+        return set_error(SVDAG_ERR_HIP, "column read-back failed");
     }
-    int rc = finish(ctx, "voxelize (count)");
+    int rc = finish(ctx, "voxelize (rows)");
+    if (rc == 0 && h2[5]) rc = set_error(SVDAG_ERR_CAPACITY, "a block of %llu rows spans 2^32 or more voxel columns",
+                                         (unsigned long long)kVoxTileItems);
+    if (rc == 0 && h2[2] > rect_columns) rc = set_error(SVDAG_ERR_INTERNAL, "row ranges exceed the rectangles");
+    const uint64_t columns = h2[2];
+    if (rc != 0 || columns == 0) {
+        device_release(ctx, mem);
+        return rc;
+    }
+    // One evaluation of the columns with a staging area of 16 bytes per
+    // column; past the context's limit (or if it cannot be allocated), two
+    // (count, then write in place).
+    const uint64_t stage_bytes = tiles(columns, kVoxTileItems) * 2 * kVoxTileItems * sizeof(uint64_t);
+    void* stage = nullptr;
+    if (stage_bytes <= ctx->vox_stage_max && device_alloc(ctx, &stage, stage_bytes) != 0) stage = nullptr;
+    if (stage)
+        launch_vox_emit(st, setup, R, cnt + 1, cnt + 2, columns, depth, trow, tcounts, static_cast<uint64_t*>(stage));
+    else
+        launch_vox_count(st, setup, R, cnt + 1, cnt + 2, columns, depth, trow, tcounts);
+    launch_scan_counts(st, tcounts, toffs, cnt + 2, kVoxTileItems, cnt + 3);
+    if (hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = set_error(SVDAG_ERR_HIP, "count read-back failed");
+    if (rc == 0) rc = finish(ctx, "voxelize (count)");
     const uint64_t keys = h[3];
     void* out = nullptr;
-    if (rc == 0 && h[5]) rc = set_error(SVDAG_ERR_CAPACITY, "a block of %llu rows spans 2^32 or more voxel columns",
-                                        (unsigned long long)kVoxTileItems);
-    if (rc == 0 && h[2] > rect_columns) rc = set_error(SVDAG_ERR_INTERNAL, "row ranges exceed the rectangles");
     if (rc == 0 && keys) rc = device_alloc(ctx, &out, 8 * keys);
     if (rc == 0 && keys) {
-        launch_vox_write(st, setup, R, trow, cnt + 1, cnt + 2, rect_columns, depth, toffs, static_cast<uint64_t*>(out));
+        if (stage)
+            launch_vox_compact(st, static_cast<const uint64_t*>(stage), tcounts, toffs, cnt + 2, columns,
+                               static_cast<uint64_t*>(out));
+        else
+            launch_vox_write(st, setup, R, trow, cnt + 1, cnt + 2, columns, depth, toffs, static_cast<uint64_t*>(out));
         rc = hipGetLastError() == hipSuccess ? finish(ctx, "voxelize (write)")
                                              : set_error(SVDAG_ERR_HIP, "voxelizer launch failed");
     }
+    device_release(ctx, stage);
     device_release(ctx, mem);
     if (rc != 0) {
         device_release(ctx, out);

@@ -330,6 +330,7 @@ struct TriSetup {
     int32_t a, u, w, planes;
 };
 constexpr uint64_t kVoxTileItems = 2048;  // columns per voxeliser tile
+constexpr uint64_t kVoxStageMaxBytes = 4ull << 30;  // default largest one-pass staging area
 
 // Per-row table of the triangles' column rectangles (a row runs along w).
 struct VoxRows {
@@ -361,6 +362,15 @@ void launch_vox_count(hipStream_t s, const TriSetup* setup, const VoxRows& rows,
 void launch_vox_write(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* tile_row,
                       const uint64_t* n_rows_dev, const uint64_t* total_cols_dev, uint64_t total_cols_bound, int depth,
                       const uint64_t* tile_offsets, uint64_t* keys);
+// One evaluation of the columns: tile t's keys go to stage[t * 2 * kVoxTileItems
+// ...] (2 * kVoxTileItems keys per tile of staging), their count to
+// tile_counts[t]; total_cols = the columns (known on the host).  Then each
+// tile's keys move to tile_offsets[t] (exclusive scan of the counts).
+void launch_vox_emit(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* n_rows_dev,
+                     const uint64_t* total_cols_dev, uint64_t total_cols, int depth, uint64_t* tile_row,
+                     uint32_t* tile_counts, uint64_t* stage);
+void launch_vox_compact(hipStream_t s, const uint64_t* stage, const uint32_t* tile_counts, const uint64_t* tile_offsets,
+                        const uint64_t* total_cols_dev, uint64_t total_cols, uint64_t* keys);
 
 // Per pixel: the first voxel hit and its colour (the reference's trace_paths).
 void launch_trace_paths(hipStream_t s, const uint32_t* words, uint64_t n_words, int depth,

@@ -14,8 +14,9 @@
 //
 // Layout per wave: 8x8 pixels (neighbouring rays walk the same nodes), four
 // waves per workgroup (16x16 pixels).  The walk's stack (one entry per level:
-// node word index, child mask, children still to visit) lives in LDS,
-// level-major so a wave's 64 entries of one level are 64 consecutive words.
+// node word index, child mask, children still to visit) lives in LDS as a
+// 32-bit index array and a 16-bit mask array (31.5 KB: five workgroups per
+// CU), level-major so a wave's 64 entries of one level are consecutive.
 #include "svdag_device.hpp"
 #include "svdag_internal.hpp"
 
@@ -110,24 +111,20 @@ __device__ __forceinline__ uint32_t rgb888(float r, float g, float b)
     return uint32_t(R * 255.0f) | uint32_t(G * 255.0f) << 8 | uint32_t(B * 255.0f) << 16;
 }
 
-// Stack entry: word index (bits 0-31), child mask (32-39), children left to
-// visit (40-47).
-__device__ __forceinline__ uint64_t pack_entry(uint32_t index, uint32_t child_mask, uint32_t visit)
-{
-    return uint64_t(index) | uint64_t(child_mask) << 32 | uint64_t(visit) << 40;
-}
-
 __global__ __launch_bounds__(kTraceThreads) void k_trace_paths(const uint32_t* __restrict__ words, uint64_t n_words,
                                                                int depth, svdag_trace_params p,
                                                                uint32_t* __restrict__ paths,
                                                                uint32_t* __restrict__ colors)
 {
-    __shared__ uint64_t s_stack[kTraceMaxDepth][kTraceThreads];
+    __shared__ uint32_t s_index[kTraceMaxDepth][kTraceThreads];
+    __shared__ uint16_t s_masks[kTraceMaxDepth][kTraceThreads];  // child mask | children to visit << 8
     const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const uint32_t px = blockIdx.x * kTraceTile + (w & 1u) * 8u + (lane & 7u);
     const uint32_t py = blockIdx.y * kTraceTile + (w >> 1) * 8u + (lane >> 3);
     if (px >= p.width || py >= p.height) return;
-    uint64_t* const stack = &s_stack[0][threadIdx.x];  // entry l at stack[l * kTraceThreads]
+    // Entry l of this thread's stack: stack_index[l * kTraceThreads], stack_masks[...].
+    uint32_t* const stack_index = &s_index[0][threadIdx.x];
+    uint16_t* const stack_masks = &s_masks[0][threadIdx.x];
 
     // Ray (src/DAGTracer.cu:308-319).
     Ray r;
@@ -161,10 +158,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_paths(const uint32_t* _
         int up = level;
         while (up > 0 && !visit) {
             --up;
-            const uint64_t e = stack[up * kTraceThreads];
-            index = uint32_t(e);
-            child_mask = uint32_t(e >> 32) & 0xFFu;
-            visit = uint32_t(e >> 40) & 0xFFu;
+            index = stack_index[up * kTraceThreads];
+            const uint32_t m = stack_masks[up * kTraceThreads];
+            child_mask = m & 0xFFu;
+            visit = m >> 8;
         }
         if (up == 0 && !visit) {
             path[0] = path[1] = path[2] = 0;
@@ -179,7 +176,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_paths(const uint32_t* _
         path[0] = path[0] << 1 | (child >> 2 & 1u);
         path[1] = path[1] << 1 | (child >> 1 & 1u);
         path[2] = path[2] << 1 | (child & 1u);
-        stack[level * kTraceThreads] = pack_entry(index, child_mask, visit);
+        stack_index[level * kTraceThreads] = index;
+        stack_masks[level * kTraceThreads] = uint16_t(child_mask | visit << 8);
         ++level;
         if (uint32_t(depth - 2 - level) == p.debug_level) debug_index = index;
         if (level == depth) break;  // a voxel

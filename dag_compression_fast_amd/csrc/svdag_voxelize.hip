@@ -19,9 +19,13 @@
 //                   exact per-column test below decides; about a fifth of the
 //                   rectangle for the triangle workload)
 //   (scan)          column offsets per row
-//   k_vox_count     tile of 2048 global columns: voxels emitted
+//   k_vox_emit      tile of 2048 global columns: its keys (at most two per
+//                   column) into the tile's 4096-key staging region, and
+//                   their count
 //   (scan)          key offsets per tile
-//   k_vox_write     the same columns again, writing the keys
+//   k_vox_compact   each tile's staged keys to its offset
+// When the staging area would be too large, k_vox_count counts the keys per
+// tile and k_vox_write evaluates the columns again to write them in place.
 #pragma clang fp contract(off)
 
 #include "svdag_device.hpp"
@@ -355,7 +359,73 @@ __global__ __launch_bounds__(kVoxThreads) void k_vox_write(const TriSetup* __res
     }
 }
 
+// As k_vox_write, but the keys go to the tile's staging region
+// stage[tile * 2 * kVoxTile ...] and the tile's count to counts[tile].
+__global__ __launch_bounds__(kVoxThreads) void k_vox_emit(const TriSetup* __restrict__ setup, RowTable R,
+                                                          const uint64_t* __restrict__ tile_row,
+                                                          const uint64_t* __restrict__ n_rows_ptr,
+                                                          const uint64_t* __restrict__ total_ptr, int64_t side,
+                                                          uint32_t* __restrict__ counts, uint64_t* __restrict__ stage)
+{
+    __shared__ uint32_t s_scan[kVoxThreads / kWave];
+    const uint64_t n_rows = *n_rows_ptr, total = *total_ptr;
+    const uint64_t ntiles = (total + kVoxTile - 1) / kVoxTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t k[kVoxKeys];
+        const int mine = thread_columns(setup, R, tile_row, n_rows, total, tile, side, k);
+        uint32_t tile_keys;
+        const uint64_t o = tile * 2 * kVoxTile + block_exclusive_scan<kVoxThreads>(uint32_t(mine), s_scan, tile_keys);
+#pragma unroll
+        for (int j = 0; j < kVoxKeys; ++j)
+            if (j < mine) stage[o + j] = k[j];
+        if (threadIdx.x == 0) counts[tile] = tile_keys;
+    }
+}
+
+// keys[offsets[tile] + j] = stage[tile * 2 * kVoxTile + j] for j < counts[tile].
+__global__ __launch_bounds__(kVoxThreads) void k_vox_compact(const uint64_t* __restrict__ stage,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint64_t* __restrict__ offsets,
+                                                             const uint64_t* __restrict__ total_ptr,
+                                                             uint64_t* __restrict__ keys)
+{
+    const uint64_t ntiles = (*total_ptr + kVoxTile - 1) / kVoxTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t c = counts[tile];
+        const uint64_t* src = stage + tile * 2 * kVoxTile;
+        uint64_t* dst = keys + offsets[tile];
+        for (uint32_t j = threadIdx.x; j < c; j += kVoxThreads) dst[j] = src[j];
+    }
+}
+
 }  // namespace
+
+static int vox_grid(uint64_t columns_bound)
+{
+    const uint64_t tiles = (columns_bound + kVoxTile - 1) / kVoxTile;
+    return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, uint64_t(device_cu_count()) * 8)));
+}
+
+void launch_vox_emit(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* n_rows_dev,
+                     const uint64_t* total_cols_dev, uint64_t total_cols, int depth, uint64_t* tile_row,
+                     uint32_t* tile_counts, uint64_t* stage)
+{
+    TimedLaunch t(kKVoxCount, s);
+    const uint64_t tiles = (total_cols + kVoxTile - 1) / kVoxTile;
+    hipLaunchKernelGGL(k_tile_rows, dim3(uint32_t((tiles + kVoxThreads) / kVoxThreads)), dim3(kVoxThreads), 0, s,
+                       rows.col_off, n_rows_dev, total_cols_dev, tile_row);
+    const RowTable R{rows.tri, rows.u, rows.w0, rows.col_off};
+    hipLaunchKernelGGL(k_vox_emit, dim3(vox_grid(total_cols)), dim3(kVoxThreads), 0, s, setup, R, tile_row,
+                       n_rows_dev, total_cols_dev, int64_t(1) << depth, tile_counts, stage);
+}
+
+void launch_vox_compact(hipStream_t s, const uint64_t* stage, const uint32_t* tile_counts, const uint64_t* tile_offsets,
+                        const uint64_t* total_cols_dev, uint64_t total_cols, uint64_t* keys)
+{
+    TimedLaunch t(kKVoxWrite, s);
+    hipLaunchKernelGGL(k_vox_compact, dim3(vox_grid(total_cols)), dim3(kVoxThreads), 0, s, stage, tile_counts,
+                       tile_offsets, total_cols_dev, keys);
+}
 
 void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* rows,
                       uint64_t* rect_columns)
@@ -382,12 +452,6 @@ void launch_row_setup(hipStream_t s, const TriSetup* setup, const uint64_t* row_
     TimedLaunch t(kKVoxSetup, s);
     hipLaunchKernelGGL(k_row_offsets, dim3(grid), dim3(kVoxThreads), 0, s, rows.cols, tile_off, n_rows_dev,
                        rows.col_off);
-}
-
-static int vox_grid(uint64_t columns_bound)
-{
-    const uint64_t tiles = (columns_bound + kVoxTile - 1) / kVoxTile;
-    return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, uint64_t(device_cu_count()) * 8)));
 }
 
 void launch_vox_count(hipStream_t s, const TriSetup* setup, const VoxRows& rows, const uint64_t* n_rows_dev,

@@ -84,10 +84,14 @@ typedef enum svdag_option {
                                          the call fails and names the stage that stalled */
     SVDAG_OPT_RADIX_BITS = 6,         /* widest digit of the key sort, 4..9 (default 9)      */
     SVDAG_OPT_SORT_TILE = 7,          /* keys per key-sort tile: 4096 or 8192 (default 4096) */
-    SVDAG_OPT_TABLE_ESTIMATE = 8      /* 1: size the leaf dedup table from a distinct-count
+    SVDAG_OPT_TABLE_ESTIMATE = 8,     /* 1: size the leaf dedup table from a distinct-count
                                          estimate of the leaf masks (default); 0: from the leaf
                                          positions.  An estimate that proves too small makes the
                                          call rebuild with the bound-sized table. */
+    SVDAG_OPT_VOXEL_STAGE_BYTES = 9   /* largest staging area of svdag_voxelize_triangles
+                                         (16 bytes per voxel column, default 4 GiB): within it
+                                         the columns are evaluated once, past it twice (count,
+                                         then write); 0 always takes two passes */
 } svdag_option;
 
 /* --- context ------------------------------------------------------------ */

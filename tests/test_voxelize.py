@@ -53,6 +53,11 @@ def near_axis_soup(side, count, seed):
     return np.array(out, dtype=np.float64)
 
 
+def _lib_opt():
+    from dag_compression_fast_amd import _lib
+    return _lib.OPT_VOXEL_STAGE_BYTES
+
+
 def unique_keys(buf, n):
     keys = buf.to_numpy(np.uint64, n)
     buf.free()
@@ -83,6 +88,21 @@ def test_voxelize_on_the_cpu_emulator():
             assert got.size == 0 or int(got.max()) < 8 ** depth
         buf, n = m.voxelize_triangles(np.zeros((0, 3, 3)), 5)
         assert n == 0
+        # Two passes over the columns (no staging area): the same voxels, and
+        # the same key sequence as the one-pass path.
+        tris, depth = rng_cases[0]
+        buf, n = m.voxelize_triangles(tris, depth)
+        one_pass = buf.to_numpy(np.uint64, n)
+        buf.free()
+        ctx = m.default_context()
+        ctx.set_option(_lib_opt(), 0)
+        try:
+            buf, n2 = m.voxelize_triangles(tris, depth)
+            two_pass = buf.to_numpy(np.uint64, n2)
+            buf.free()
+        finally:
+            ctx.set_option(_lib_opt(), 4 << 30)
+        assert np.array_equal(one_pass, two_pass)
     finally:
         from dag_compression_fast_amd import _lib
         import dag_compression_fast_amd as svd
@@ -113,3 +133,17 @@ def test_voxelize_full_workload_on_gpu(gpu):
     for tris_e, d in ((edge_cases(64), 6), (edge_cases(2048), 11), (near_axis_soup(2048, 400, 4), 11)):
         buf, n = gpu.voxelize_triangles(tris_e, d)
         assert np.array_equal(unique_keys(buf, n), gpu.gen_voxelize(tris_e, d))
+    # The two-pass path (no staging area) gives the same key sequence.
+    tris_e = near_axis_soup(2048, 400, 4)
+    buf, n = gpu.voxelize_triangles(tris_e, 11)
+    one_pass = buf.to_numpy(np.uint64, n)
+    buf.free()
+    ctx = gpu.default_context()
+    ctx.set_option(_lib_opt(), 0)
+    try:
+        buf, n2 = gpu.voxelize_triangles(tris_e, 11)
+        two_pass = buf.to_numpy(np.uint64, n2)
+        buf.free()
+    finally:
+        ctx.set_option(_lib_opt(), 4 << 30)
+    assert np.array_equal(one_pass, two_pass)
