@@ -118,7 +118,8 @@ __device__ __forceinline__ void load_groups(const K* __restrict__ sorted, uint64
 {
     const uint32_t lane = lane_id();
 #pragma unroll
-    for (int g = 0; g < kReduceGroups; ++g) k[g] = uint64_t(sorted[min(base + g * kWave + lane, n - 1)]);
+    for (int g = 0; g < kReduceGroups; ++g)  // read once: non-temporal
+        k[g] = uint64_t(__builtin_nontemporal_load(sorted + min(base + g * kWave + lane, n - 1)));
 }
 
 template <typename K>

@@ -56,6 +56,7 @@ inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
 #define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
 #define __builtin_amdgcn_sched_barrier(x) ((void)(x))
+#define __builtin_nontemporal_load(p) (*(p))
 
 template <class T>
 T __shfl_up(T v, unsigned d, int = 64)
