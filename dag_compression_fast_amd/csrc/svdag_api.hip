@@ -1446,11 +1446,12 @@ int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t 
     uint64_t* cnt = reinterpret_cast<uint64_t*>(A + o_cnt);
     launch_fill_u64(st, cnt, 6, 0);
     launch_fill_u64(st, cnt, 1, n_tris);
-    launch_tri_setup(st, d_tris, n_tris, depth, setup, rows, cnt + 4);
+    launch_tri_setup(st, d_tris, n_tris, depth, setup, rows, cnt + 4, reinterpret_cast<unsigned long long*>(cnt + 5));
     launch_scan_counts(st, rows, roff, cnt, 1, cnt + 1);
     uint64_t h[6] = {};
     HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
     TRY(finish(ctx, "voxelize (triangle setup)"));
+    if (h[5] & kErrKeyRange) return set_error(SVDAG_ERR_ARGUMENT, "a triangle has a non-finite coordinate");
     const uint64_t n_rows = h[1], rect_columns = h[4];
     if (n_rows == 0 || rect_columns == 0) return 0;
     // Row table and tile arrays (sized for the rectangles' columns, which bound

@@ -342,9 +342,10 @@ struct VoxRows {
 };
 
 // rows[t] = rows of triangle t's rectangle (tris: 9 doubles per triangle);
-// *rect_columns += the rectangles' columns (a bound on the kept columns).
+// *rect_columns += the rectangles' columns (a bound on the kept columns); a
+// non-finite coordinate sets kErrKeyRange in *err.
 void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* rows,
-                      uint64_t* rect_columns);
+                      uint64_t* rect_columns, unsigned long long* err);
 // rows.tri / u / w0 / cols / col_off for the *n_rows_dev rows (n_rows on the
 // host), the column total into *total_cols_dev; row_off = exclusive scan of
 // the per-triangle row counts; tile_cols / tile_off: one entry per

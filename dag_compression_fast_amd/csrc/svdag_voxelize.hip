@@ -59,20 +59,28 @@ __device__ __forceinline__ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z)
     return split_by_3(x) << 2 | split_by_3(y) << 1 | split_by_3(z);
 }
 
+// A triangle with a non-finite coordinate gets no rows and sets kErrKeyRange
+// in *err.
 __global__ __launch_bounds__(kVoxThreads) void k_tri_setup(const double* __restrict__ tris, uint64_t n_tris,
                                                            int64_t side, TriSetup* __restrict__ setup,
                                                            uint32_t* __restrict__ rows,
-                                                           unsigned long long* __restrict__ rect_columns)
+                                                           unsigned long long* __restrict__ rect_columns,
+                                                           unsigned long long* __restrict__ err)
 {
     __shared__ uint64_t s_red[kVoxThreads / kWave];
     const uint64_t t = uint64_t(blockIdx.x) * kVoxThreads + threadIdx.x;
     uint64_t rect = 0;  // columns of the triangle's rectangle (bounds the row ranges)
     if (t < n_tris) {
         double vs[3][3];
+        bool finite = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) vs[c][k] = tris[9 * t + 3 * c + k];
+            for (int k = 0; k < 3; ++k) {
+                vs[c][k] = tris[9 * t + 3 * c + k];
+                finite &= isfinite(vs[c][k]);
+            }
+        if (!finite) atomicOr(err, (unsigned long long)kErrKeyRange);
         // n = (v1 - v0) x (v2 - v0)
         const double e1x = vs[1][0] - vs[0][0], e1y = vs[1][1] - vs[0][1], e1z = vs[1][2] - vs[0][2];
         const double e2x = vs[2][0] - vs[0][0], e2y = vs[2][1] - vs[0][1], e2z = vs[2][2] - vs[0][2];
@@ -84,7 +92,7 @@ __global__ __launch_bounds__(kVoxThreads) void k_tri_setup(const double* __restr
         const int a = an0 >= an1 && an0 >= an2 ? 0 : (an1 >= an2 ? 1 : 2);
         const double ana = a == 0 ? an0 : (a == 1 ? an1 : an2);
         uint64_t nu = 0;
-        if (ana != 0.0) {  // else degenerate: no rows
+        if (finite && ana != 0.0) {  // else degenerate (or rejected): no rows
             const int u = (a + 1) % 3, w = (a + 2) % 3;
             S.a = a;
             S.u = u;
@@ -428,13 +436,13 @@ void launch_vox_compact(hipStream_t s, const uint64_t* stage, const uint32_t* ti
 }
 
 void launch_tri_setup(hipStream_t s, const double* tris, uint64_t n_tris, int depth, TriSetup* setup, uint32_t* rows,
-                      uint64_t* rect_columns)
+                      uint64_t* rect_columns, unsigned long long* err)
 {
     if (n_tris == 0) return;
     TimedLaunch t(kKVoxSetup, s);
     hipLaunchKernelGGL(k_tri_setup, dim3(uint32_t((n_tris + kVoxThreads - 1) / kVoxThreads)), dim3(kVoxThreads), 0, s,
                        tris, n_tris, int64_t(1) << depth, setup, rows,
-                       reinterpret_cast<unsigned long long*>(rect_columns));
+                       reinterpret_cast<unsigned long long*>(rect_columns), err);
 }
 
 void launch_row_setup(hipStream_t s, const TriSetup* setup, const uint64_t* row_off, const uint64_t* n_tris_dev,

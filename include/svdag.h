@@ -181,7 +181,8 @@ int svdag_read_dag_file(const char* path, double* aabb_min, double* aabb_max, ui
  * (x y z of its three corners) in a 2^depth grid; the thin surface rule of the
  * triangle-surface generator, evaluated identically (no fused multiply-adds),
  * so the voxel set equals svdag_gen_triangle_surfaces' for the same triangles.
- * *d_keys is a device allocation (svdag_free_keys); 1 <= depth <= 21. */
+ * *d_keys is a device allocation (svdag_free_keys); 1 <= depth <= 21.  A
+ * non-finite coordinate is SVDAG_ERR_ARGUMENT. */
 int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t n_tris,
                              int32_t depth, uint64_t** d_keys, uint64_t* n_keys);
 int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys);

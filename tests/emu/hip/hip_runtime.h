@@ -57,6 +57,7 @@ inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 #define __builtin_amdgcn_s_waitcnt(x) ((void)(x))
 #define __builtin_amdgcn_sched_barrier(x) ((void)(x))
 #define __builtin_nontemporal_load(p) (*(p))
+using std::isfinite;
 
 template <class T>
 T __shfl_up(T v, unsigned d, int = 64)

@@ -88,6 +88,10 @@ def test_voxelize_on_the_cpu_emulator():
             assert got.size == 0 or int(got.max()) < 8 ** depth
         buf, n = m.voxelize_triangles(np.zeros((0, 3, 3)), 5)
         assert n == 0
+        bad = edge_cases(32)
+        bad[2, 1, 0] = np.nan
+        with pytest.raises(m.SvdagError):
+            m.voxelize_triangles(bad, 5)
         # Two passes over the columns (no staging area): the same voxels, and
         # the same key sequence as the one-pass path.
         tris, depth = rng_cases[0]
