@@ -68,8 +68,8 @@ def algorithmic_bytes(kernel, st, key_bytes, launches):
         # grouped keys read once; (code, mask) per leaf into the chunk regions
         "leaf_reduce": key_bytes * n + 16 * l0,
         "leaf_compact": 32 * l0,
-        # mask read, slot write, unique list append
-        "leaf_insert": 12 * l0 + 12 * lu,
+        # mask read and slot write per leaf position, table key per unique mask
+        "leaf_insert": 12 * l0 + 8 * lu,
         # child code + slot + hash reads; parent code, slot, mask, 8 children, hash writes
         "level_build": sum(20 * child[l] + 53 * pre[l] for l in range(levels)),
         # rep + mask + children reads, rank + hash + mask + children writes per unique node
