@@ -221,6 +221,49 @@ def bench_trace(svd, words, depth, args, ctx):
             "rays_per_s": round(width * height / dt, 1), "hits": hits}
 
 
+# The reference's recorded sort throughputs (Python/variable64.csv:33 and
+# constant64.csv:15, 100M random 64-bit keys on a GTX 1080-class GPU): the only
+# numbers it publishes for a stage of this path.
+REFERENCE_SORT = {33: 1.751e9, 64: 0.925e9}
+
+
+def bench_sort(svd, args, ctx):
+    """svdag_sort_keys (the reference's SortKeys) on 100M random uint64 keys,
+    device-resident, at 33 bits (depth 11 Morton keys) and 64 bits.  Wall time
+    per call, best of `reps` after one warm-up."""
+    n, reps = 100_000_000, 5
+    rng = np.random.default_rng(0x5EED0005)
+    out = {}
+    d_out = svd.DeviceBuffer(8 * n)
+    L = svd._lib.lib()
+    try:
+        for bits in (33, 64):
+            hi = 1 << bits if bits < 64 else None
+            keys = (rng.integers(0, hi, size=n, dtype=np.uint64) if hi
+                    else rng.integers(0, 2 ** 63, size=n, dtype=np.uint64) << np.uint64(1))
+            d_keys = svd.DeviceBuffer.from_numpy(keys)
+            try:
+                best = None
+                for r in range(reps + 1):
+                    t0 = time.perf_counter()
+                    svd._lib.check(L.svdag_sort_keys(ctx.handle, d_keys.ptr, n, bits, d_out.ptr))
+                    dt = time.perf_counter() - t0
+                    if r:
+                        best = dt if best is None else min(best, dt)
+                ok = np.array_equal(d_out.to_numpy(np.uint64, 1 << 20), np.sort(keys)[: 1 << 20])
+            finally:
+                d_keys.free()
+            rate = n / best
+            out[str(bits)] = {"ms": round(best * 1e3, 3), "keys_per_s": round(rate, 1),
+                              "reference_keys_per_s": REFERENCE_SORT[bits],
+                              "vs_reference": round(rate / REFERENCE_SORT[bits], 1), "prefix_checked": bool(ok)}
+            log(f"sort_keys {bits} bits: {n} keys in {best * 1e3:.2f} ms = {rate / 1e9:.1f} G keys/s "
+                f"({rate / REFERENCE_SORT[bits]:.0f}x the reference's recorded rate)")
+    finally:
+        d_out.free()
+    return {"n": n, "by_bits": out}
+
+
 def bench_chunks(svd, _lib, args):
     """Plain (CreateDAG per chunk + MergeDAGs + CreateFinalDAG, the reference's
     flow) against batched (one create_dag_batched) and served (continuous
@@ -378,6 +421,7 @@ def main():
     ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
     ap.add_argument("--no-trace", action="store_true", help="skip the ray tracing timing")
     ap.add_argument("--no-autotune", action="store_true", help="batched mode with the default options")
+    ap.add_argument("--no-sort", action="store_true", help="skip the sort_keys timing")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
@@ -491,6 +535,7 @@ def main():
 
     vox_stats = None if args.no_voxelize else bench_voxelize(svd, spec, args, ctx)
     trace_stats = None if args.no_trace else bench_trace(svd, result_words, depth, args, ctx)
+    sort_stats = None if args.no_sort else bench_sort(svd, args, ctx)
 
     chunk_stats = None
     if not args.no_chunks:
@@ -505,7 +550,7 @@ def main():
         "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "check": check, "chunks": chunk_stats,
-        "voxelize": vox_stats, "trace": trace_stats,
+        "voxelize": vox_stats, "trace": trace_stats, "sort_keys": sort_stats,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -374,6 +374,32 @@ class _KeyBuffer(DeviceBuffer):
         self.ptr = None
 
 
+def sort_keys(keys, bits, ctx=None):
+    """The reference's SortKeys (src/DAGCompression.cu:336-357): uint64 keys
+    (numpy or DeviceBuffer) sorted on their low `bits` bits on the device.
+    Returns a numpy array for numpy input, else a new DeviceBuffer."""
+    ctx = ctx or default_context()
+    if not isinstance(keys, DeviceBuffer):
+        keys = np.ascontiguousarray(keys)
+    if np.dtype(keys.dtype) != np.uint64:
+        raise SvdagError(1, "sort_keys takes uint64 keys")
+    buf, n, _ = _as_device_keys(keys)
+    out = None
+    try:
+        out = DeviceBuffer(8 * n)
+        out.dtype, out.count = np.dtype(np.uint64), n
+        _lib.check(_lib.lib().svdag_sort_keys(ctx.handle, buf.ptr, n, int(bits), out.ptr))
+        if isinstance(keys, DeviceBuffer):
+            result, out = out, None
+            return result
+        return out.to_numpy(np.uint64, n)
+    finally:
+        if out is not None:
+            out.free()
+        if buf is not keys:
+            buf.free()
+
+
 def query_voxels(words, depth, codes, ctx=None):
     """Occupancy of Morton codes in a flat layout (host uint32 words of a
     depth-`depth` DAG), walked on the device as the reference's tracer reads

@@ -153,6 +153,7 @@ def bind_library(L):
         "svdag_read_dag_file": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_query_voxels": ([vp, vp, u64, i32, vp, u64, vp], i32),
+        "svdag_sort_keys": ([vp, vp, u64, i32, vp], i32),
         "svdag_voxelize_triangles": ([vp, vp, u64, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_free_keys": ([vp, vp], i32),
         "svdag_trace_params_from_view": ([dp, dp, dp, dp, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, dp, dp,

@@ -1532,6 +1532,38 @@ int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys)
     return 0;
 }
 
+int svdag_sort_keys(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t bits, uint64_t* d_out)
+{
+    TRY(check_ctx(ctx));
+    if (bits < 1 || bits > 64) return set_error(SVDAG_ERR_ARGUMENT, "bits %d outside [1, 64]", bits);
+    if (n >= (1ull << 32)) return set_error(SVDAG_ERR_ARGUMENT, "n = %llu keys: at most 2^32 - 1", (unsigned long long)n);
+    if (n && (!d_keys || !d_out)) return set_error(SVDAG_ERR_ARGUMENT, "null keys or output");
+    if (n && d_keys == d_out) return set_error(SVDAG_ERR_ARGUMENT, "output must not alias the input");
+    if (n == 0) return 0;
+    hipStream_t st = ctx->stream;
+    const SortPlan plan = make_sort_plan(0, bits, ctx->radix_bits, ctx->sort_ipt);
+    if (plan.passes == 0) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
+    Carver cv;
+    const uint64_t o_zero = cv.take(4 * sort_zero_words(plan));
+    const uint64_t o_n = cv.take(8), o_or = cv.take(8);
+    const uint64_t o_counts = cv.take(4 * sort_count_words(n));
+    const uint64_t o_a = cv.take(8 * n), o_b = cv.take(8 * n);
+    TRY(ensure_arena(ctx, cv.off));
+    char* A = ctx->arena;
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_zero), (o_n - o_zero) / 8, 0);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_n), 1, n);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_or), 1, 0);
+    const int where = launch_sort<uint64_t>(st, plan, d_keys, nullptr, reinterpret_cast<uint64_t*>(A + o_a),
+                                            reinterpret_cast<uint64_t*>(A + o_b), nullptr, nullptr, d_out, nullptr,
+                                            reinterpret_cast<const uint64_t*>(A + o_n), n,
+                                            reinterpret_cast<uint32_t*>(A + o_zero),
+                                            reinterpret_cast<uint32_t*>(A + o_counts),
+                                            reinterpret_cast<unsigned long long*>(A + o_or));
+    if (where != 2) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
+    HIP_TRY(hipGetLastError());
+    return finish(ctx, "sort_keys");
+}
+
 int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words, int32_t depth,
                        const uint64_t* d_codes, uint64_t n, uint8_t* d_out)
 {

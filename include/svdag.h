@@ -187,6 +187,16 @@ int svdag_voxelize_triangles(svdag_context* ctx, const double* d_tris, uint64_t 
                              int32_t depth, uint64_t** d_keys, uint64_t* n_keys);
 int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys);
 
+/* The reference's SortKeys (src/DAGCompression.cu:336-357; cub::DeviceRadixSort::
+ * SortKeys on bits [0, end_bit)): d_out = the n keys of d_keys in ascending order
+ * of their low `bits` bits.  Keys equal in those bits are identical under the
+ * reference's precondition (keys < 2^bits); otherwise their order is
+ * unspecified.  Device pointers; d_out must not alias d_keys, which is left
+ * untouched; 1 <= bits <= 64, n < 2^32.  Scratch: 16 bytes per key plus the
+ * digit counts, in the context's arena. */
+int svdag_sort_keys(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t bits,
+                    uint64_t* d_out);
+
 /* Occupancy of voxels in a flat layout of a depth-`depth` DAG, walked as the
  * reference's tracer reads it (src/DAGTracer.cu:85-176).  d_words, d_codes and
  * d_out are device pointers; d_out[i] = 1 if Morton code d_codes[i] is set, 0 if
