@@ -126,6 +126,9 @@ struct svdag_context {
     int device = 0;
     char* arena = nullptr;
     uint64_t arena_cap = 0;
+    // Host tile table of the last batched build (kept until its copy is done).
+    std::vector<SegTile> seg_tiles;
+    std::vector<uint32_t> seg_chunk_tiles;
     // Result blocks (DAG storage, final words): reused across calls, returned
     // by svdag_free_dag / svdag_free_words, released by svdag_free_all.
     std::multimap<uint64_t, void*> free_blocks;
@@ -482,7 +485,34 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits,
                                      ctx->sort_ipt);
     const uint64_t offsets_bytes = batch ? align_up(8 * (batch->num_chunks + 1)) : 0;
-    TRY(ensure_arena(ctx, L.total + offsets_bytes));
+    // Batched builds whose chunk-local sort spans at most kSegMaxBits bits sort
+    // inside each chunk (the chunk order is the input order): tile table laid
+    // out here, chunk by chunk.
+    const uint32_t seg_lo = uint32_t(L.key_plan.shift[0]);
+    const uint32_t seg_chunk_bits = batch ? uint32_t(3 * batch->chunk_depth) : 0;
+    const bool segmented = batch && L.key_plan.passes > 0 && seg_chunk_bits > seg_lo &&
+                           seg_chunk_bits - seg_lo <= kSegMaxBits;
+    uint64_t seg_bytes = 0, o_seg_tiles = 0, o_seg_ct = 0, o_seg_counts = 0, o_seg_totals = 0, o_seg_starts = 0;
+    if (segmented) {
+        ctx->seg_tiles.clear();
+        ctx->seg_chunk_tiles.assign(batch->num_chunks + 1, 0);
+        for (uint64_t ch = 0; ch < batch->num_chunks; ++ch) {
+            ctx->seg_chunk_tiles[ch] = uint32_t(ctx->seg_tiles.size());
+            for (uint64_t s0 = batch->h_offsets[ch]; s0 < batch->h_offsets[ch + 1]; s0 += kSegTileItems)
+                ctx->seg_tiles.push_back(SegTile{
+                    s0, uint32_t(std::min<uint64_t>(kSegTileItems, batch->h_offsets[ch + 1] - s0)), uint32_t(ch)});
+        }
+        ctx->seg_chunk_tiles[batch->num_chunks] = uint32_t(ctx->seg_tiles.size());
+        Carver sc;
+        sc.off = L.total + offsets_bytes;
+        o_seg_tiles = sc.take(sizeof(SegTile) * ctx->seg_tiles.size());
+        o_seg_ct = sc.take(4 * (batch->num_chunks + 1));
+        o_seg_counts = sc.take(4 * (uint64_t(ctx->seg_tiles.size()) << kMaxSegDigitBits));
+        o_seg_totals = sc.take(4 * (batch->num_chunks << kMaxSegDigitBits));
+        o_seg_starts = sc.take(4 * (batch->num_chunks << kMaxSegDigitBits));
+        seg_bytes = sc.off - (L.total + offsets_bytes);
+    }
+    TRY(ensure_arena(ctx, L.total + offsets_bytes + seg_bytes));
     char* A = ctx->arena;
     BuildCounters* c = reinterpret_cast<BuildCounters*>(A + L.counters);
     unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
@@ -522,10 +552,25 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         prefix.err = err;
         uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
         uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
-        where = launch_sort<uint64_t>(st, L.key_plan, reinterpret_cast<const uint64_t*>(d_keys), nullptr, ka, kb,
-                                      nullptr, nullptr, nullptr, nullptr, &c->n, n,
-                                      reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or,
-                                      prefix);
+        if (segmented) {
+            SegTile* d_tiles = reinterpret_cast<SegTile*>(A + o_seg_tiles);
+            uint32_t* d_ct = reinterpret_cast<uint32_t*>(A + o_seg_ct);
+            if (hipMemcpyAsync(d_tiles, ctx->seg_tiles.data(), sizeof(SegTile) * ctx->seg_tiles.size(),
+                               hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_ct, ctx->seg_chunk_tiles.data(), 4 * ctx->seg_chunk_tiles.size(),
+                               hipMemcpyHostToDevice, st) != hipSuccess)
+                return fail(set_error(SVDAG_ERR_HIP, "tile table upload failed"));
+            where = launch_segmented_sort(st, reinterpret_cast<const uint64_t*>(d_keys), ka, kb, d_tiles,
+                                          ctx->seg_tiles.size(), d_ct, d_off, batch->num_chunks, seg_lo,
+                                          seg_chunk_bits, reinterpret_cast<uint32_t*>(A + o_seg_counts),
+                                          reinterpret_cast<uint32_t*>(A + o_seg_totals),
+                                          reinterpret_cast<uint32_t*>(A + o_seg_starts), err);
+        } else {
+            where = launch_sort<uint64_t>(st, L.key_plan, reinterpret_cast<const uint64_t*>(d_keys), nullptr, ka, kb,
+                                          nullptr, nullptr, nullptr, nullptr, &c->n, n,
+                                          reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or,
+                                          prefix);
+        }
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     } else {
         K* ka = reinterpret_cast<K*>(A + L.keys_a);

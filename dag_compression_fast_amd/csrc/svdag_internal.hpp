@@ -124,6 +124,28 @@ struct ChunkPrefix {
     unsigned long long* err = nullptr;  // receives kErrKeyRange for a key >= 2^chunk_bits
 };
 
+// Segmented key sort of a batched build (chunks back to back, in chunk order):
+// tiles of at most kSegTileItems keys, none crossing a chunk, laid out by the
+// host chunk by chunk; chunk_tiles[c] = first tile of chunk c (num_chunks + 1
+// entries), chunk_start[c] = its first key.  Sorts the chunk-local bits
+// [lo, chunk_bits) (at most kSegMaxBits of them, in one or two passes of at
+// most kMaxSegDigitBits) and puts each chunk's index above chunk_bits.
+// Scratch: counts (ntiles << kMaxSegDigitBits u32), totals and starts
+// (num_chunks << kMaxSegDigitBits u32 each).  Returns 0 (result in keys_a), 1
+// (keys_b), or -1 when the bit span does not fit.
+struct SegTile {
+    uint64_t start;
+    uint32_t count;
+    uint32_t chunk;
+};
+constexpr uint32_t kSegTileItems = 4096;
+constexpr uint32_t kSegMaxBits = 12;
+constexpr uint32_t kMaxSegDigitBits = 8;
+int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+                          const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
+                          const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
+                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err);
+
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,
                 K* keys_a, K* keys_b, uint32_t* vals_a, uint32_t* vals_b, K* final_keys, uint32_t* final_vals,

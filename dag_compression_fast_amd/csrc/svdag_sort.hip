@@ -670,4 +670,231 @@ template int launch_sort<uint32_t>(hipStream_t, const SortPlan&, const uint32_t*
                                    uint32_t*, uint32_t*, uint32_t*, uint32_t*, uint32_t*, const uint64_t*, uint64_t,
                                    uint32_t*, uint32_t*, unsigned long long*, const ChunkPrefix&);
 
+// ---------------------------------------------------------------------------
+// Segmented key sort of a batched build.  The chunks arrive one after the
+// other in Morton order, so the chunk index (the top of the sorted bits) is
+// already in order: sorting the chunk-local bits [lo, chunk_bits) inside each
+// chunk finishes the job.  Tiles of at most kSegTile keys never cross a chunk
+// (the host lays them out, chunk by chunk).  Per pass: per-tile counts of the
+// pass digit (digit-major inside each chunk), a scan of each (chunk, digit)
+// row over the chunk's tiles, a scan of each chunk's digit totals, and the
+// scatter.  The first pass ranks unstably (its input order is arbitrary); the
+// second ranks each key by its combined (second digit, first digit) value,
+// which keeps the first pass's order between first digits inside each second
+// digit (the order inside one combined value is free: those keys share a
+// block group).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kSegThreads = 256;
+constexpr int kSegIpt = 16;
+constexpr uint32_t kSegTile = uint32_t(kSegThreads) * kSegIpt;
+constexpr uint32_t kSegMaxBins = 1u << kSegMaxBits;  // combined bins of the second pass
+constexpr int kSegBinsPerThread = kSegMaxBins / kSegThreads;
+static_assert(kSegTile == kSegTileItems, "tile size shared with the planner");
+static_assert(kSegTile <= 65536 && kSegMaxBins <= 65536, "bin and rank packed in 16 bits each");
+
+// Counts of tile t (local index lt in chunk c with nt tiles):
+// counts[chunk_tiles[c] * R + d * nt + lt].
+__device__ __forceinline__ uint64_t seg_count_index(const SegTile& tile, const uint32_t* chunk_tiles, uint64_t t,
+                                                    uint32_t R, uint32_t d)
+{
+    const uint32_t t0 = chunk_tiles[tile.chunk], nt = chunk_tiles[tile.chunk + 1] - t0;
+    return uint64_t(t0) * R + uint64_t(d) * nt + (t - t0);
+}
+
+// Count the pass digit ((key >> shift) & dmask) of every key of each tile.
+// The first pass also range-checks the chunk-local keys.
+template <bool FIRST>
+__global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint64_t* __restrict__ keys,
+                                                           const SegTile* __restrict__ tiles, uint64_t ntiles,
+                                                           const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
+                                                           uint32_t dmask, uint32_t chunk_bits,
+                                                           uint32_t* __restrict__ counts,
+                                                           unsigned long long* __restrict__ err)
+{
+    __shared__ uint32_t s_cnt[256];
+    const uint32_t R = dmask + 1;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < R; i += kSegThreads) s_cnt[i] = 0;
+        __syncthreads();
+        const SegTile tile = tiles[t];
+        bool bad = false;
+        for (uint32_t j = threadIdx.x; j < tile.count; j += kSegThreads) {
+            const uint64_t k = __builtin_nontemporal_load(keys + tile.start + j);
+            if constexpr (FIRST) bad |= (k >> chunk_bits) != 0;
+            atomicAdd(&s_cnt[uint32_t(k >> shift) & dmask], 1u);
+        }
+        if (FIRST && bad) atomicOr(err, (unsigned long long)kErrKeyRange);
+        __syncthreads();
+        for (uint32_t d = threadIdx.x; d < R; d += kSegThreads)
+            counts[seg_count_index(tile, chunk_tiles, t, R, d)] = s_cnt[d];
+        __syncthreads();
+    }
+}
+
+// One wave per (chunk, digit) row: exclusive scan of the row over the chunk's
+// tiles, in place; the row total to totals[c * R + d].
+__global__ __launch_bounds__(kWave) void k_seg_scan_rows(const uint32_t* __restrict__ chunk_tiles, uint64_t num_chunks,
+                                                         uint32_t R, uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ totals)
+{
+    const uint32_t lane = lane_id();
+    for (uint64_t row = blockIdx.x; row < num_chunks * R; row += gridDim.x) {
+        const uint64_t c = row / R;
+        const uint32_t d = uint32_t(row % R);
+        const uint32_t t0 = chunk_tiles[c], nt = chunk_tiles[c + 1] - t0;
+        uint32_t* r = counts + uint64_t(t0) * R + uint64_t(d) * nt;
+        uint32_t carry = 0;
+        for (uint32_t i0 = 0; i0 < nt; i0 += kWave) {
+            const uint32_t i = i0 + lane;
+            const uint32_t v = i < nt ? r[i] : 0u;
+            const uint32_t inc = wave_inclusive_scan(v);
+            if (i < nt) r[i] = carry + inc - v;
+            carry += uint32_t(__builtin_amdgcn_readlane(int(inc), kWave - 1));
+        }
+        if (lane == 0) totals[c * R + d] = carry;
+    }
+}
+
+// Per chunk: starts[c * R + d] = chunk start + keys of the chunk with a
+// smaller digit.
+__global__ __launch_bounds__(kSegThreads) void k_seg_scan_digits(const uint64_t* __restrict__ chunk_start,
+                                                                 uint64_t num_chunks, uint32_t R,
+                                                                 const uint32_t* __restrict__ totals,
+                                                                 uint32_t* __restrict__ starts)
+{
+    __shared__ uint32_t s_scan[kSegThreads / kWave];
+    for (uint64_t c = blockIdx.x; c < num_chunks; c += gridDim.x) {
+        const uint32_t d = threadIdx.x;
+        uint32_t unused;
+        const uint32_t ex = block_exclusive_scan<kSegThreads>(d < R ? totals[c * R + d] : 0u, s_scan, unused);
+        if (d < R) starts[c * R + d] = uint32_t(chunk_start[c]) + ex;
+    }
+}
+
+// The scatter of one pass.  FIRST: bins = the pass digit, and the keys get
+// their chunk index above chunk_bits.  Else: bins = (digit << prev_bits) |
+// previous digit.  Output offset of tile t's digit d: starts[c * R + d] plus
+// the row prefix in counts.
+template <bool FIRST>
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __restrict__ keys_in,
+                                                             uint64_t* __restrict__ keys_out,
+                                                             const SegTile* __restrict__ tiles, uint64_t ntiles,
+                                                             const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
+                                                             uint32_t dmask, uint32_t prev_shift, uint32_t prev_bits,
+                                                             uint32_t chunk_bits, const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ starts)
+{
+    __shared__ uint64_t s_keys[kSegTile];
+    __shared__ uint32_t s_bin[kSegMaxBins];
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_scan[kSegThreads / kWave];
+    const uint32_t R = dmask + 1;
+    const uint32_t nbins = FIRST ? R : (R << prev_bits);
+    const uint32_t prev_mask = (1u << prev_bits) - 1u;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const SegTile tile = tiles[t];
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSegThreads) s_bin[i] = 0;
+        uint64_t key[kSegIpt];
+        uint32_t br[kSegIpt];  // bin << 16 | rank within the bin
+#pragma unroll
+        for (int i = 0; i < kSegIpt; ++i) {
+            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
+            key[i] = j < tile.count ? __builtin_nontemporal_load(keys_in + tile.start + j) : 0ull;
+            if (FIRST) key[i] |= uint64_t(tile.chunk) << chunk_bits;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSegIpt; ++i) {
+            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
+            uint32_t bin = uint32_t(key[i] >> shift) & dmask;
+            if (!FIRST) bin = (bin << prev_bits) | (uint32_t(key[i] >> prev_shift) & prev_mask);
+            const uint32_t r = j < tile.count ? atomicAdd(&s_bin[bin], 1u) : 0u;
+            br[i] = bin << 16 | r;
+        }
+        __syncthreads();
+        // Exclusive scan of the bins, kSegBinsPerThread consecutive per thread.
+        uint32_t v[kSegBinsPerThread];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kSegBinsPerThread; ++q) {
+            const uint32_t b = threadIdx.x * kSegBinsPerThread + q;
+            v[q] = b < nbins ? s_bin[b] : 0u;
+            sum += v[q];
+        }
+        uint32_t unused;
+        uint32_t run = block_exclusive_scan<kSegThreads>(sum, s_scan, unused);
+#pragma unroll
+        for (int q = 0; q < kSegBinsPerThread; ++q) {
+            const uint32_t b = threadIdx.x * kSegBinsPerThread + q;
+            if (b < nbins) s_bin[b] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        // Digit d's keys start at its first bin inside the tile.
+        for (uint32_t d = threadIdx.x; d < R; d += kSegThreads)
+            s_base[d] = starts[uint64_t(tile.chunk) * R + d] + counts[seg_count_index(tile, chunk_tiles, t, R, d)] -
+                        s_bin[FIRST ? d : (d << prev_bits)];
+#pragma unroll
+        for (int i = 0; i < kSegIpt; ++i) {
+            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
+            if (j < tile.count) s_keys[s_bin[br[i] >> 16] + (br[i] & 0xFFFFu)] = key[i];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < tile.count; j += kSegThreads) {
+            const uint64_t k = s_keys[j];
+            keys_out[s_base[uint32_t(k >> shift) & dmask] + j] = k;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+                          const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
+                          const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
+                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err)
+{
+    if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || ntiles == 0) return -1;
+    const uint32_t B = chunk_bits - lo;
+    const uint32_t b1 = B > kMaxSegDigitBits ? (B + 1) / 2 : B;  // first pass digit width
+    const uint32_t b2 = B - b1;                                  // second (0: one pass)
+    const int cus = device_cu_count();
+    const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(ntiles, uint64_t(cus) * 64)));
+    const int chunk_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks, uint64_t(cus) * 8)));
+    auto pass = [&](bool first, const uint64_t* src, uint64_t* dst, uint32_t shift, uint32_t bits) {
+        const uint32_t mask = (1u << bits) - 1u, R = mask + 1;
+        const int row_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks * R, uint64_t(cus) * 64)));
+        {
+            TimedLaunch t(kKSortCount, stream);
+            if (first)
+                hipLaunchKernelGGL(k_seg_count<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, tiles, ntiles,
+                                   chunk_tiles, shift, mask, chunk_bits, counts, err);
+            else
+                hipLaunchKernelGGL(k_seg_count<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, tiles, ntiles,
+                                   chunk_tiles, shift, mask, chunk_bits, counts, err);
+        }
+        {
+            TimedLaunch t(kKSortScan, stream);
+            hipLaunchKernelGGL(k_seg_scan_rows, dim3(row_grid), dim3(kWave), 0, stream, chunk_tiles, num_chunks, R,
+                               counts, totals);
+            hipLaunchKernelGGL(k_seg_scan_digits, dim3(chunk_grid), dim3(kSegThreads), 0, stream, chunk_start,
+                               num_chunks, R, totals, starts);
+        }
+        TimedLaunch t(kKSortScatterKeys, stream);
+        if (first)
+            hipLaunchKernelGGL(k_seg_scatter<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, tiles, ntiles,
+                               chunk_tiles, shift, mask, 0u, 0u, chunk_bits, counts, starts);
+        else
+            hipLaunchKernelGGL(k_seg_scatter<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, tiles, ntiles,
+                               chunk_tiles, shift, mask, lo, b1, chunk_bits, counts, starts);
+    };
+    pass(true, keys_in, keys_a, lo, b1);
+    if (b2 == 0) return 0;
+    pass(false, keys_a, keys_b, lo + b1, b2);
+    return 1;
+}
+
 }  // namespace svdag

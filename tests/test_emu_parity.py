@@ -88,6 +88,8 @@ def test_emu_merge_and_batched(emu, oracle):
     import test_gpu_parity as T
     T.test_merge_dags_matches_oracle_and_global(emu, oracle)
     T.test_batched_matches_merge(emu, oracle)
+    for chunk_depth in (6, 8, 9, 10):
+        T.test_batched_chunk_depths(emu, oracle, chunk_depth)
 
 
 def test_emu_errors(emu):

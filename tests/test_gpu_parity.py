@@ -194,6 +194,22 @@ def test_batched_matches_merge(gpu, oracle):
     assert np.array_equal(gpu.create_final_dag(batched), gpu.create_final_dag(merged))
 
 
+@pytest.mark.parametrize("chunk_depth", [6, 8, 9, 10])
+def test_batched_chunk_depths(gpu, oracle, chunk_depth):
+    """Chunk-local sort spans of 3, 9 and 12 bits (sorted inside each chunk in
+    one or two passes) and 15 bits (the global sort); chunk sizes around the
+    4096-key tile, empty and single-key chunks."""
+    rng = np.random.default_rng(90 + chunk_depth)
+    sizes = [0, 1, 3000, 4096, 4097, 25000, 9000, 130]
+    chunks = [clustered_keys(rng, chunk_depth, n) if n else np.zeros(0, dtype=np.uint64) for n in sizes]
+    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(c) for c in chunks])
+    print(f"[gpu-parity] batched 8 chunks d{chunk_depth}", flush=True)
+    batched = gpu.create_dag_batched(np.concatenate(chunks), offsets, chunk_depth)
+    exp = oracle.merge_dags([oracle.create_dag(c, chunk_depth) for c in chunks])
+    assert_dag_equal(batched.to_numpy(), exp, f"batched d{chunk_depth}")
+
+
 @pytest.mark.parametrize("bits,tile", [(4, 4096), (6, 8192), (7, 4096), (9, 4096), (9, 8192)])
 def test_radix_digit_widths(gpu, oracle, bits, tile):
     from dag_compression_fast_amd import _lib
