@@ -390,6 +390,35 @@ def bench_chunks(svd, _lib, args):
             "check": "plain, batched and served final DAGs identical" if same else "MISMATCH"}
 
 
+def profile_batched(svd, _lib, args):
+    """N batched builds (+ flat layout) of the chunk workload and nothing else,
+    so that a profiler's kernel statistics cover exactly that path."""
+    L = _lib.lib()
+    chunks = chunk_workload(svd, args.threads)
+    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(c) for c in chunks])
+    d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
+    ctx = svd.Context()
+    times = []
+    try:
+        for _ in range(args.profile_batched):
+            t0 = time.perf_counter()
+            out = _lib.SvdagDag()
+            _lib.check(L.svdag_create_dag_batched(ctx.handle, d_keys.ptr,
+                                                  offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                  len(chunks), CHUNKS["depth"], ctypes.byref(out)))
+            words, nw = device_words(L, _lib, ctx, out)
+            times.append(time.perf_counter() - t0)
+            L.svdag_free_words(ctx.handle, words)
+            L.svdag_free_dag(ctx.handle, ctypes.byref(out))
+    finally:
+        d_keys.free()
+        ctx.close()
+    print(json.dumps({"batched_builds": len(times), "n_keys": int(offsets[-1]),
+                      "best_ms": round(min(times) * 1e3, 3)}), flush=True)
+    return 0
+
+
 def cpu_baseline(keys, depth, threads, reps):
     """The CPU oracle (a restatement of the reference algorithm) on the same keys."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -422,6 +451,8 @@ def main():
     ap.add_argument("--no-trace", action="store_true", help="skip the ray tracing timing")
     ap.add_argument("--no-autotune", action="store_true", help="batched mode with the default options")
     ap.add_argument("--no-sort", action="store_true", help="skip the sort_keys timing")
+    ap.add_argument("--profile-batched", type=int, default=0, metavar="N",
+                    help="only run N batched builds of the chunk workload (for a profiler), then exit")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
@@ -438,6 +469,8 @@ def main():
 
     if _lib.gpu_count() < 1:
         sys.exit("bench.py needs a visible GPU")
+    if args.profile_batched:
+        return profile_batched(svd, _lib, args)
     spec = WORKLOADS[args.workload]
     depth = spec["depth"]
     log(f"generating {args.workload} on {args.threads} host threads")

@@ -693,6 +693,8 @@ constexpr uint32_t kSegMaxBins = 1u << kSegMaxBits;  // combined bins of the sec
 constexpr int kSegBinsPerThread = kSegMaxBins / kSegThreads;
 static_assert(kSegTile == kSegTileItems, "tile size shared with the planner");
 static_assert(kSegTile <= 65536 && kSegMaxBins <= 65536, "bin and rank packed in 16 bits each");
+static_assert(kSegMaxBins == kSegThreads * kSegBinsPerThread && kSegBinsPerThread * kWave * (kSegThreads / kWave) ==
+                  kSegMaxBins, "bins covered by the row-wise scan");
 
 // Counts of tile t (local index lt in chunk c with nt tiles):
 // counts[chunk_tiles[c] * R + d * nt + lt].
@@ -814,22 +816,29 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
             br[i] = bin << 16 | r;
         }
         __syncthreads();
-        // Exclusive scan of the bins, kSegBinsPerThread consecutive per thread.
-        uint32_t v[kSegBinsPerThread];
-        uint32_t sum = 0;
+        // Exclusive scan of the bins.  Wave w takes bins [w * 64 * kSegBinsPerThread, ...)
+        // as rows of 64 consecutive bins (one per lane, so the LDS accesses
+        // are conflict-free), with a carry from row to row and wave to wave.
+        const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+        uint32_t ex[kSegBinsPerThread];
+        uint32_t carry = 0;
 #pragma unroll
         for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t b = threadIdx.x * kSegBinsPerThread + q;
-            v[q] = b < nbins ? s_bin[b] : 0u;
-            sum += v[q];
+            const uint32_t b = (w * kSegBinsPerThread + uint32_t(q)) * kWave + lane;
+            const uint32_t v = b < nbins ? s_bin[b] : 0u;
+            const uint32_t inc = wave_inclusive_scan(v);
+            ex[q] = carry + inc - v;
+            carry += uint32_t(__builtin_amdgcn_readlane(int(inc), kWave - 1));
         }
-        uint32_t unused;
-        uint32_t run = block_exclusive_scan<kSegThreads>(sum, s_scan, unused);
+        if (lane == 0) s_scan[w] = carry;
+        __syncthreads();
+        uint32_t base = 0;
+#pragma unroll
+        for (int i = 0; i < kSegThreads / kWave; ++i) base += uint32_t(i) < w ? s_scan[i] : 0u;
 #pragma unroll
         for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t b = threadIdx.x * kSegBinsPerThread + q;
-            if (b < nbins) s_bin[b] = run;
-            run += v[q];
+            const uint32_t b = (w * kSegBinsPerThread + uint32_t(q)) * kWave + lane;
+            if (b < nbins) s_bin[b] = base + ex[q];
         }
         __syncthreads();
         // Digit d's keys start at its first bin inside the tile.
