@@ -776,9 +776,11 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scan_digits(const uint64_t*
 }
 
 // The scatter of one pass.  FIRST: bins = the pass digit, and the keys get
-// their chunk index above chunk_bits.  Else: bins = (digit << prev_bits) |
-// previous digit.  Output offset of tile t's digit d: starts[c * R + d] plus
-// the row prefix in counts.
+// their chunk index above chunk_bits.  Else: bins = digit * span + (previous
+// digit - the tile's smallest previous digit), span = the tile's range of
+// previous digits (the input is sorted on them, so a tile usually holds one or
+// two).  Output offset of tile t's digit d: starts[c * R + d] plus the row
+// prefix in counts.
 template <bool FIRST>
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __restrict__ keys_in,
                                                              uint64_t* __restrict__ keys_out,
@@ -792,39 +794,71 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
     __shared__ uint32_t s_bin[kSegMaxBins];
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_scan[kSegThreads / kWave];
+    __shared__ uint32_t s_span[2 * (kSegThreads / kWave)];
     const uint32_t R = dmask + 1;
-    const uint32_t nbins = FIRST ? R : (R << prev_bits);
     const uint32_t prev_mask = (1u << prev_bits) - 1u;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const SegTile tile = tiles[t];
-        for (uint32_t i = threadIdx.x; i < nbins; i += kSegThreads) s_bin[i] = 0;
         uint64_t key[kSegIpt];
         uint32_t br[kSegIpt];  // bin << 16 | rank within the bin
+        uint32_t lo = 0xFFFFFFFFu, hi = 0;
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
             const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
             key[i] = j < tile.count ? __builtin_nontemporal_load(keys_in + tile.start + j) : 0ull;
             if (FIRST) key[i] |= uint64_t(tile.chunk) << chunk_bits;
+            if (!FIRST && j < tile.count) {
+                const uint32_t p = uint32_t(key[i] >> prev_shift) & prev_mask;
+                lo = min(lo, p);
+                hi = max(hi, p);
+            }
         }
+        uint32_t pmin = 0, span = 1;
+        if constexpr (!FIRST) {
+#pragma unroll
+            for (int o = kWave / 2; o >= 1; o >>= 1) {
+                lo = min(lo, uint32_t(__shfl_xor(int(lo), o, kWave)));
+                hi = max(hi, uint32_t(__shfl_xor(int(hi), o, kWave)));
+            }
+            if (lane == 0) {
+                s_span[w] = lo;
+                s_span[kSegThreads / kWave + w] = hi;
+            }
+            __syncthreads();
+            lo = s_span[0];
+            hi = s_span[kSegThreads / kWave];
+#pragma unroll
+            for (int i = 1; i < kSegThreads / kWave; ++i) {
+                lo = min(lo, s_span[i]);
+                hi = max(hi, s_span[kSegThreads / kWave + i]);
+            }
+            pmin = lo;
+            span = hi >= lo ? hi - lo + 1 : 1;
+        }
+        const uint32_t nbins = R * span;  // at most R << prev_bits <= kSegMaxBins
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSegThreads) s_bin[i] = 0;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
             const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
             uint32_t bin = uint32_t(key[i] >> shift) & dmask;
-            if (!FIRST) bin = (bin << prev_bits) | (uint32_t(key[i] >> prev_shift) & prev_mask);
+            if (!FIRST) bin = bin * span + ((uint32_t(key[i] >> prev_shift) & prev_mask) - pmin);
             const uint32_t r = j < tile.count ? atomicAdd(&s_bin[bin], 1u) : 0u;
             br[i] = bin << 16 | r;
         }
         __syncthreads();
         // Exclusive scan of the bins.  Wave w takes bins [w * 64 * kSegBinsPerThread, ...)
         // as rows of 64 consecutive bins (one per lane, so the LDS accesses
-        // are conflict-free), with a carry from row to row and wave to wave.
-        const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+        // are conflict-free), with a carry from row to row and wave to wave;
+        // rows past the tile's bins are skipped.
         uint32_t ex[kSegBinsPerThread];
         uint32_t carry = 0;
 #pragma unroll
         for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t b = (w * kSegBinsPerThread + uint32_t(q)) * kWave + lane;
+            const uint32_t row0 = (w * kSegBinsPerThread + uint32_t(q)) * kWave;
+            if (row0 >= nbins) break;  // wave-uniform
+            const uint32_t b = row0 + lane;
             const uint32_t v = b < nbins ? s_bin[b] : 0u;
             const uint32_t inc = wave_inclusive_scan(v);
             ex[q] = carry + inc - v;
@@ -837,14 +871,16 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
         for (int i = 0; i < kSegThreads / kWave; ++i) base += uint32_t(i) < w ? s_scan[i] : 0u;
 #pragma unroll
         for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t b = (w * kSegBinsPerThread + uint32_t(q)) * kWave + lane;
+            const uint32_t row0 = (w * kSegBinsPerThread + uint32_t(q)) * kWave;
+            if (row0 >= nbins) break;
+            const uint32_t b = row0 + lane;
             if (b < nbins) s_bin[b] = base + ex[q];
         }
         __syncthreads();
         // Digit d's keys start at its first bin inside the tile.
         for (uint32_t d = threadIdx.x; d < R; d += kSegThreads)
             s_base[d] = starts[uint64_t(tile.chunk) * R + d] + counts[seg_count_index(tile, chunk_tiles, t, R, d)] -
-                        s_bin[FIRST ? d : (d << prev_bits)];
+                        s_bin[d * span];
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
             const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
