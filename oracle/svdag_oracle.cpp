@@ -1,8 +1,8 @@
 /*
  * svdag_oracle.cpp -- CPU oracle for the sparse-voxel-DAG build.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the timed
- * host-core baseline.  Restates, step for step, the reference algorithm of
+ * TEST INFRASTRUCTURE ONLY: the expected results the HIP library's tests
+ * compare against, and the timed host-core baseline.  Restates, step for step, the reference algorithm of
  * Phyronnaz/DAG_Compression_Fast:
  *   CreateDAG        src/DAGCompression.cu:1003-1107
  *   SortLevel        src/DAGCompression.cu:864-914
