@@ -88,20 +88,6 @@ def read_traffic(kernel, workload):
     return float(m.group(1)) if m else None
 
 
-CHUNKS = dict(desc="512 sub-volumes of 512^3 (chunk depth 9 -> depth 12), lognormal keys per chunk "
-                   "(median 2e5, sigma 1, 1e3..4e6), sphere-shell or triangle-surface content, 20-30% duplicates",
-              seed=0x5EED0003, num=512, depth=9)
-
-
-def chunk_workload(svd, threads):
-    """The 512-chunk workload: host arrays per chunk (chunk-local keys)."""
-    from concurrent.futures import ThreadPoolExecutor
-    n, _, _ = svd.gen_chunk_plan(CHUNKS["seed"], CHUNKS["num"])
-    with ThreadPoolExecutor(threads) as ex:
-        return list(ex.map(lambda c: svd.gen_chunk(CHUNKS["seed"], c, CHUNKS["depth"], capacity=int(n[c])),
-                           range(CHUNKS["num"])))
-
-
 def device_words(L, _lib, ctx, dag):
     """Flat layout of a device DAG, left in device memory: (pointer, count)."""
     words, nw = ctypes.c_void_p(), ctypes.c_uint64()
@@ -221,10 +207,11 @@ def bench_trace(svd, words, depth, args, ctx):
             "rays_per_s": round(width * height / dt, 1), "hits": hits}
 
 
-# The reference's recorded sort throughputs (Python/variable64.csv:33 and
-# constant64.csv:15, 100M random 64-bit keys on a GTX 1080-class GPU): the only
-# numbers it publishes for a stage of this path.
-REFERENCE_SORT = {33: 1.751e9, 64: 0.925e9}
+# The reference's recorded sort throughputs (100M random keys on a GTX
+# 1080-class GPU): 64-bit keys on 33 bits (Python/variable64.csv:33) and 64
+# bits (constant64.csv:15), 32-bit keys on 27 bits (variable32.csv:27): the
+# only numbers it publishes for a stage of this path.
+REFERENCE_SORT = {33: 1.751e9, 64: 0.925e9, "u32_27": 4.167e9}
 
 
 def bench_sort(svd, args, ctx):
@@ -237,28 +224,33 @@ def bench_sort(svd, args, ctx):
     d_out = svd.DeviceBuffer(8 * n)
     L = svd._lib.lib()
     try:
-        for bits in (33, 64):
+        for name in (33, 64, "u32_27"):
+            bits = 27 if name == "u32_27" else name
             hi = 1 << bits if bits < 64 else None
             keys = (rng.integers(0, hi, size=n, dtype=np.uint64) if hi
                     else rng.integers(0, 2 ** 63, size=n, dtype=np.uint64) << np.uint64(1))
+            fn = L.svdag_sort_keys
+            if name == "u32_27":
+                keys = keys.astype(np.uint32)
+                fn = L.svdag_sort_keys_u32
             d_keys = svd.DeviceBuffer.from_numpy(keys)
             try:
                 best = None
                 for r in range(reps + 1):
                     t0 = time.perf_counter()
-                    svd._lib.check(L.svdag_sort_keys(ctx.handle, d_keys.ptr, n, bits, d_out.ptr))
+                    svd._lib.check(fn(ctx.handle, d_keys.ptr, n, bits, d_out.ptr))
                     dt = time.perf_counter() - t0
                     if r:
                         best = dt if best is None else min(best, dt)
-                ok = np.array_equal(d_out.to_numpy(np.uint64, 1 << 20), np.sort(keys)[: 1 << 20])
+                ok = np.array_equal(d_out.to_numpy(keys.dtype, 1 << 20), np.sort(keys)[: 1 << 20])
             finally:
                 d_keys.free()
             rate = n / best
-            out[str(bits)] = {"ms": round(best * 1e3, 3), "keys_per_s": round(rate, 1),
-                              "reference_keys_per_s": REFERENCE_SORT[bits],
-                              "vs_reference": round(rate / REFERENCE_SORT[bits], 1), "prefix_checked": bool(ok)}
-            log(f"sort_keys {bits} bits: {n} keys in {best * 1e3:.2f} ms = {rate / 1e9:.1f} G keys/s "
-                f"({rate / REFERENCE_SORT[bits]:.0f}x the reference's recorded rate)")
+            out[str(name)] = {"ms": round(best * 1e3, 3), "keys_per_s": round(rate, 1),
+                              "reference_keys_per_s": REFERENCE_SORT[name],
+                              "vs_reference": round(rate / REFERENCE_SORT[name], 1), "prefix_checked": bool(ok)}
+            log(f"sort_keys {name} bits: {n} keys in {best * 1e3:.2f} ms = {rate / 1e9:.1f} G keys/s "
+                f"({rate / REFERENCE_SORT[name]:.0f}x the reference's recorded rate)")
     finally:
         d_out.free()
     return {"n": n, "by_bits": out}
@@ -266,20 +258,22 @@ def bench_sort(svd, args, ctx):
 
 def bench_chunks(svd, _lib, args):
     """Plain (CreateDAG per chunk + MergeDAGs + CreateFinalDAG, the reference's
-    flow) against batched (one create_dag_batched) and served (continuous
-    admission) on the same 512 uneven chunks; all three must give the same DAG."""
+    flow), one plain build of the union of the chunks, batched (one
+    create_dag_batched) and served (continuous admission) on the same 512
+    uneven chunks; all must give the oracle's DAG of the union."""
     from dag_compression_fast_amd import serve
+    from dag_compression_fast_amd import workloads as W
     L, H = _lib.lib(), _lib.hip()
-    chunks = chunk_workload(svd, args.threads)
+    spec = W.CHUNKS
+    chunks = W.chunk_workload(args.threads)
     sizes = np.array([len(c) for c in chunks], dtype=np.uint64)
-    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
-    offsets[1:] = np.cumsum(sizes)
+    offsets = W.chunk_offsets(chunks)
     total = int(offsets[-1])
     log(f"chunks: {len(chunks)} chunks, {total} keys")
     d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
     ctx = svd.Context()
-    depth = CHUNKS["depth"]
-    k_levels = 3
+    depth = spec["depth"]
+    global_depth = depth + spec["levels_above"]
 
     def plain():
         dags = (_lib.SvdagDag * len(chunks))()
@@ -317,10 +311,22 @@ def bench_chunks(svd, _lib, args):
 
     t_plain, w_plain = timed(plain, args.chunk_reps)
     log(f"chunks plain: {t_plain * 1e3:.1f} ms")
+    # One plain build of the union (global codes at depth 12) from device-resident keys.
+    union = W.chunk_union(chunks, depth)
+    d_union = svd.DeviceBuffer.from_numpy(union)
+
+    def union_build():
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag(ctx.handle, d_union.ptr, total, global_depth, ctypes.byref(out)))
+        return out
+
+    t_union, w_union = timed(union_build, args.chunk_reps)
+    d_union.free()
+    log(f"chunks union, one plain build: {t_union * 1e3:.2f} ms")
     tuned = None
     if not args.no_autotune:
-        # Digit width, sort tile and table load factor for the batched build,
-        # picked by timing it (every candidate must give the same DAG).
+        # Table load factor and the segmented sort's digit split for the
+        # batched build, picked by timing it (every candidate must give the same DAG).
         from dag_compression_fast_amd import autotune
         state = {}
 
@@ -336,17 +342,18 @@ def bench_chunks(svd, _lib, args):
             return fetch_words(L, _lib, ctx, *state.pop("words"))
 
         t0 = time.perf_counter()
-        res = autotune.autotune(ctx, tune_build, tune_fetch, reps=3)
+        res = autotune.autotune(ctx, tune_build, tune_fetch, reps=3, space=autotune.space_for(depth),
+                                chunk_depth=depth)
         if "words" in state:
             L.svdag_free_words(ctx.handle, state.pop("words")[0])
+        times = [ms for _, ms in res["table"]]
         tuned = {"best": res["best"], "best_ms": round(res["best_ms"], 3), "candidates": len(res["table"]),
-                 "default_ms": round(dict((tuple(sorted(st.items())), ms) for st, ms in res["table"])
-                                     [tuple(sorted(autotune.DEFAULTS.items()))], 3),
+                 "default_ms": round(res["default_ms"], 3), "worst_ms": round(max(times), 3),
                  "tune_s": round(time.perf_counter() - t0, 2)}
-        log(f"autotune: {res['best']} {res['best_ms']:.2f} ms (default {tuned['default_ms']:.2f} ms) "
-            f"over {len(res['table'])} settings in {tuned['tune_s']:.1f} s")
+        log(f"autotune: {res['best']} {res['best_ms']:.2f} ms (default {tuned['default_ms']:.2f} ms, worst "
+            f"{tuned['worst_ms']:.2f} ms) over {len(res['table'])} settings in {tuned['tune_s']:.1f} s")
     t_batch, w_batch = timed(batched, args.chunk_reps)
-    log(f"chunks batched: {t_batch * 1e3:.1f} ms")
+    log(f"chunks batched: {t_batch * 1e3:.2f} ms")
     if args.kernel_table:
         ctx.profile_enable(None)
         ctx.profile_reset()
@@ -355,10 +362,44 @@ def bench_chunks(svd, _lib, args):
         ctx.profile_enable([])
         for k, (ms, cnt) in sorted(table.items(), key=lambda kv: -kv[1][0])[:12]:
             log(f"  batched {k:20s} {ms:8.3f} ms  {cnt:4d} launches")
-    same = np.array_equal(w_plain, w_batch)
+    same = np.array_equal(w_plain, w_batch) and np.array_equal(w_union, w_batch)
 
+    served = bench_served(svd, _lib, args, chunks, w_batch)
+    same = same and all(s["bit_exact"] for s in served)
+    d_keys.free()
+    ctx.close()
+
+    check = "plain, union, batched and served final DAGs identical" if same else "MISMATCH"
+    cpu = None
+    if not args.no_cpu_baseline and same:
+        # The oracle on the union: the CPU baseline of this workload and the check.
+        log(f"chunks: CPU oracle on the union ({total} keys), {args.threads} threads")
+        t0 = time.perf_counter()
+        ora = oracle_build(union, global_depth, args.threads)
+        t_cpu = time.perf_counter() - t0
+        if np.array_equal(ora, w_batch):
+            check = "bit-exact vs oracle (union of the chunks at depth 12; plain, union, batched and served identical)"
+        else:
+            check = "MISMATCH vs oracle"
+        cpu = {"value": round(total / t_cpu, 1), "unit": "voxels/s", "cores": args.threads, "kind": "port",
+               "sample": "full workload, one run: create_dag of the union + create_final_dag"}
+    return {"workload": W.CHUNKS["desc"], "n_keys": total, "plain_voxels_per_s": round(total / t_plain, 1),
+            "plain_ms": round(t_plain * 1e3, 2), "union_voxels_per_s": round(total / t_union, 1),
+            "union_ms": round(t_union * 1e3, 3), "batched_voxels_per_s": round(total / t_batch, 1),
+            "batched_ms": round(t_batch * 1e3, 3), "batched_vs_plain": round(t_plain / t_batch, 2),
+            "batched_vs_union": round(t_union / t_batch, 3),
+            "served": served, "autotune": tuned, "cpu_baseline": cpu, "check": check}
+
+
+def bench_served(svd, _lib, args, chunks, w_expected):
+    """Served mode on the chunk workload: Poisson arrivals at each rate, each
+    request one chunk at its own slot; the final global DAG must equal the
+    batched build's."""
+    from dag_compression_fast_amd import serve
+    L = _lib.lib()
+    total = sum(len(c) for c in chunks)
     served = []
-    server = serve.ChunkServer(depth, slots=args.serve_slots, max_chunk_keys=int(sizes.max()))
+    server = serve.ChunkServer(9, slots=args.serve_slots, max_chunk_keys=max(len(c) for c in chunks))
     try:
         for rate in args.serve_rates:
             arrivals = serve.poisson_arrivals(rate, len(chunks), seed=int(rate))
@@ -369,34 +410,138 @@ def bench_chunks(svd, _lib, args):
             span = max(r.done for r in results) - min(r.arrival for r in results)
             merged = server.merge(results, len(chunks))
             w_served = final_words(L, _lib, merged.ctx, merged.raw)
-            same_served = np.array_equal(w_served, w_batch)
             stats = serve.latency_stats(results)
             served.append({"rate_per_s": rate, "requests": len(reqs), "voxels_per_s": round(total / span, 1),
-                           **{k: round(v, 3) for k, v in stats.items()}, "bit_exact": bool(same_served)})
+                           **{k: round(v, 3) for k, v in stats.items()},
+                           "bit_exact": bool(np.array_equal(w_served, w_expected))})
             log(f"served {rate}/s: {total / span / 1e9:.2f} Gvox/s p50 {stats['p50_ms']:.2f} ms "
                 f"p99 {stats['p99_ms']:.2f} ms ({time.perf_counter() - t:.1f} s)")
             merged.free()
             for r in results:
                 r.dag.free()
-            same = same and same_served
     finally:
         server.close()
+    return served
+
+
+def bench_noise(svd, _lib, args):
+    """The low-redundancy value-noise grid (1024^3, depth 10, 8 batches of
+    512^3 built on their own and merged, the reference's many-chunk flow) and
+    the 512^3 cube with 32-bit keys; keys device-resident, results ending as
+    device-resident words.  Checked against the oracle in the CPU-baseline leg."""
+    from dag_compression_fast_amd import workloads as W
+    L, H = _lib.lib(), _lib.hip()
+    spec = W.NOISE
+    t0 = time.perf_counter()
+    batches = W.noise_batches(args.threads)
+    sizes = [len(b) for b in batches]
+    total = sum(sizes)
+    log(f"noise1024: {total} keys in {len(batches)} batches (generated in {time.perf_counter() - t0:.1f} s)")
+    sub = spec["batch_depth"]
+    d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(batches))
+    offsets = np.zeros(len(batches) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum(sizes)
+    ctx = svd.Context()
+
+    def per_batch():
+        dags = (_lib.SvdagDag * len(batches))()
+        for i in range(len(batches)):
+            ptr = ctypes.c_void_p(d_keys.address + 8 * int(offsets[i]))
+            _lib.check(L.svdag_create_dag(ctx.handle, ptr, sizes[i], sub, ctypes.byref(dags[i])))
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_merge_dags(ctx.handle, dags, len(batches), ctypes.byref(out)))
+        for i in range(len(batches)):
+            L.svdag_free_dag(ctx.handle, ctypes.byref(dags[i]))
+        return out
+
+    def batched():
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag_batched(ctx.handle, d_keys.ptr,
+                                              offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                              len(batches), sub, ctypes.byref(out)))
+        return out
+
+    def timed(fn, reps):
+        best, words, counts = None, None, None
+        for r in range(reps + 1):
+            _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+            t = time.perf_counter()
+            dag = fn()
+            d_words, nw = device_words(L, _lib, ctx, dag)
+            dt = time.perf_counter() - t
+            words = fetch_words(L, _lib, ctx, d_words, nw)
+            counts = [int(dag.levels[l].num_nodes) for l in range(dag.num_levels)] + [int(dag.num_leaves)]
+            if r > 0:
+                best = dt if best is None else min(best, dt)
+            L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+        return best, words, counts
+
+    t_plain, w_plain, counts = timed(per_batch, args.chunk_reps)
+    t_batch, w_batch, _ = timed(batched, args.chunk_reps)
     d_keys.free()
+    log(f"noise1024: 8 builds + merge {t_plain * 1e3:.2f} ms, batched {t_batch * 1e3:.2f} ms; "
+        f"nodes per level {counts}")
+
+    # 512^3 with 32-bit keys, and the same keys as 64-bit keys.
+    keys32 = W.noise_u32(args.threads)
+    n32 = len(keys32)
+    d32 = svd.DeviceBuffer.from_numpy(keys32)
+    d64 = svd.DeviceBuffer.from_numpy(keys32.astype(np.uint64))
+
+    def u32():
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag_u32(ctx.handle, d32.ptr, n32, W.NOISE_U32["depth"], ctypes.byref(out)))
+        return out
+
+    def u64():
+        out = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag(ctx.handle, d64.ptr, n32, W.NOISE_U32["depth"], ctypes.byref(out)))
+        return out
+
+    t32, w32, _ = timed(u32, args.chunk_reps)
+    t64, w64, _ = timed(u64, args.chunk_reps)
+    d32.free()
+    d64.free()
+    log(f"noise512: u32 keys {t32 * 1e3:.2f} ms, u64 keys {t64 * 1e3:.2f} ms ({n32} keys)")
+    same = np.array_equal(w_plain, w_batch) and np.array_equal(w32, w64)
+    check = "per-batch + merge and batched identical; u32 and u64 identical" if same else "MISMATCH"
+    cpu = None
+    if not args.no_cpu_baseline and same:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_ctypes
+        log(f"noise1024: CPU oracle, 8 builds + merge, {args.threads} threads")
+        t0 = time.perf_counter()
+        ora = oracle_ctypes.create_final_dag(oracle_ctypes.merge_dags(
+            [oracle_ctypes.create_dag(b, sub, threads=args.threads) for b in batches], threads=args.threads))
+        t_cpu = time.perf_counter() - t0
+        ora32 = oracle_build(keys32, W.NOISE_U32["depth"], args.threads)
+        ok = np.array_equal(ora, w_plain) and np.array_equal(ora32, w32)
+        check = ("bit-exact vs oracle (1024^3: 8 batch builds merged; 512^3 u32)" if ok else "MISMATCH vs oracle")
+        cpu = {"value": round(total / t_cpu, 1), "unit": "voxels/s", "cores": args.threads, "kind": "port",
+               "sample": "full 1024^3 workload, one run: 8 create_dag + merge_dags + create_final_dag"}
     ctx.close()
-    return {"workload": CHUNKS["desc"], "n_keys": total, "plain_voxels_per_s": round(total / t_plain, 1),
-            "plain_ms": round(t_plain * 1e3, 2), "batched_voxels_per_s": round(total / t_batch, 1),
-            "batched_ms": round(t_batch * 1e3, 2), "batched_vs_plain": round(t_plain / t_batch, 2),
-            "served": served, "serve_slots": args.serve_slots, "autotune": tuned,
-            "check": "plain, batched and served final DAGs identical" if same else "MISMATCH"}
+    return {"workload": spec["desc"], "n_keys": total, "nodes_per_level": counts,
+            "plain_ms": round(t_plain * 1e3, 3), "plain_voxels_per_s": round(total / t_plain, 1),
+            "batched_ms": round(t_batch * 1e3, 3), "batched_voxels_per_s": round(total / t_batch, 1),
+            "u32": {"workload": W.NOISE_U32["desc"], "n_keys": n32, "ms": round(t32 * 1e3, 3),
+                    "voxels_per_s": round(n32 / t32, 1), "u64_ms": round(t64 * 1e3, 3)},
+            "cpu_baseline": cpu, "check": check}
+
+
+def oracle_build(keys, depth, threads):
+    """Oracle CreateDAG + CreateFinalDAG of host keys: the flat words."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes
+    return oracle_ctypes.create_final_dag(oracle_ctypes.create_dag(keys, depth, threads=threads))
 
 
 def profile_batched(svd, _lib, args):
     """N batched builds (+ flat layout) of the chunk workload and nothing else,
     so that a profiler's kernel statistics cover exactly that path."""
+    from dag_compression_fast_amd import workloads as W
     L = _lib.lib()
-    chunks = chunk_workload(svd, args.threads)
-    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
-    offsets[1:] = np.cumsum([len(c) for c in chunks])
+    chunks = W.chunk_workload(args.threads)
+    offsets = W.chunk_offsets(chunks)
     d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
     ctx = svd.Context()
     times = []
@@ -406,7 +551,7 @@ def profile_batched(svd, _lib, args):
             out = _lib.SvdagDag()
             _lib.check(L.svdag_create_dag_batched(ctx.handle, d_keys.ptr,
                                                   offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-                                                  len(chunks), CHUNKS["depth"], ctypes.byref(out)))
+                                                  len(chunks), W.CHUNKS["depth"], ctypes.byref(out)))
             words, nw = device_words(L, _lib, ctx, out)
             times.append(time.perf_counter() - t0)
             L.svdag_free_words(ctx.handle, words)
@@ -415,6 +560,41 @@ def profile_batched(svd, _lib, args):
         d_keys.free()
         ctx.close()
     print(json.dumps({"batched_builds": len(times), "n_keys": int(offsets[-1]),
+                      "best_ms": round(min(times) * 1e3, 3)}), flush=True)
+    return 0
+
+
+def profile_noise(svd, _lib, args):
+    """N rounds of the value-noise flow (8 batch builds + merge + flat layout)
+    and nothing else, for a profiler: the dedup kernels at worst-case load."""
+    from dag_compression_fast_amd import workloads as W
+    L = _lib.lib()
+    batches = W.noise_batches(args.threads)
+    sizes = [len(b) for b in batches]
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(batches))
+    ctx = svd.Context()
+    times = []
+    try:
+        for _ in range(args.profile_noise):
+            t0 = time.perf_counter()
+            dags = (_lib.SvdagDag * len(batches))()
+            for i in range(len(batches)):
+                ptr = ctypes.c_void_p(d_keys.address + 8 * int(offsets[i]))
+                _lib.check(L.svdag_create_dag(ctx.handle, ptr, sizes[i], W.NOISE["batch_depth"],
+                                              ctypes.byref(dags[i])))
+            out = _lib.SvdagDag()
+            _lib.check(L.svdag_merge_dags(ctx.handle, dags, len(batches), ctypes.byref(out)))
+            words, nw = device_words(L, _lib, ctx, out)
+            times.append(time.perf_counter() - t0)
+            L.svdag_free_words(ctx.handle, words)
+            L.svdag_free_dag(ctx.handle, ctypes.byref(out))
+            for i in range(len(batches)):
+                L.svdag_free_dag(ctx.handle, ctypes.byref(dags[i]))
+    finally:
+        d_keys.free()
+        ctx.close()
+    print(json.dumps({"noise_rounds": len(times), "n_keys": int(offsets[-1]),
                       "best_ms": round(min(times) * 1e3, 3)}), flush=True)
     return 0
 
@@ -447,12 +627,15 @@ def main():
     ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
     ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
     ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
+    ap.add_argument("--no-noise", action="store_true", help="skip the value-noise workloads")
     ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
     ap.add_argument("--no-trace", action="store_true", help="skip the ray tracing timing")
     ap.add_argument("--no-autotune", action="store_true", help="batched mode with the default options")
     ap.add_argument("--no-sort", action="store_true", help="skip the sort_keys timing")
     ap.add_argument("--profile-batched", type=int, default=0, metavar="N",
                     help="only run N batched builds of the chunk workload (for a profiler), then exit")
+    ap.add_argument("--profile-noise", type=int, default=0, metavar="N",
+                    help="only run N rounds of the value-noise flow (for a profiler), then exit")
     ap.add_argument("--chunk-reps", type=int, default=2)
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
                     help="Poisson arrival rates (requests/s) of the served run")
@@ -471,6 +654,8 @@ def main():
         sys.exit("bench.py needs a visible GPU")
     if args.profile_batched:
         return profile_batched(svd, _lib, args)
+    if args.profile_noise:
+        return profile_noise(svd, _lib, args)
     spec = WORKLOADS[args.workload]
     depth = spec["depth"]
     log(f"generating {args.workload} on {args.threads} host threads")
@@ -566,6 +751,28 @@ def main():
         if not same:
             log("GPU result differs from the oracle")
 
+    # The reference's contract returns the DAG in host memory (FCpuDag, every
+    # level copied out, src/DAGCompression.cu:1073-1074, 1089): create_dag +
+    # svdag_dag_to_host, from device-resident keys.
+    dropin_times = []
+    for r in range(4):
+        _lib.hip_check(H.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        t = time.perf_counter()
+        dag = _lib.SvdagDag()
+        _lib.check(L.svdag_create_dag(ctx.handle, d_keys.ptr, n, depth, ctypes.byref(dag)))
+        host = _lib.SvdagDag()
+        _lib.check(L.svdag_dag_to_host(ctx.handle, ctypes.byref(dag), ctypes.byref(host)))
+        dt = time.perf_counter() - t
+        if r:
+            dropin_times.append(dt)
+        L.svdag_free_dag(ctx.handle, ctypes.byref(host))
+        L.svdag_free_dag(ctx.handle, ctypes.byref(dag))
+    dropin = {"ms": round(1e3 * float(np.median(dropin_times)), 3),
+              "voxels_per_s": round(n / float(np.median(dropin_times)), 1),
+              "what": "create_dag + svdag_dag_to_host (the level arrays in host memory, as CreateDAG returns "
+                      "them), keys device-resident; median of 3 after 1 warm-up"}
+    log(f"drop-in (create_dag + copy to host): {dropin['ms']:.2f} ms")
+
     vox_stats = None if args.no_voxelize else bench_voxelize(svd, spec, args, ctx)
     trace_stats = None if args.no_trace else bench_trace(svd, result_words, depth, args, ctx)
     sort_stats = None if args.no_sort else bench_sort(svd, args, ctx)
@@ -573,8 +780,13 @@ def main():
     chunk_stats = None
     if not args.no_chunks:
         chunk_stats = bench_chunks(svd, _lib, args)
-        if chunk_stats["check"] == "MISMATCH":
-            log("chunk workload: plain, batched and served results differ")
+        if "MISMATCH" in chunk_stats["check"]:
+            log(f"chunk workload: {chunk_stats['check']}")
+    noise_stats = None
+    if not args.no_noise:
+        noise_stats = bench_noise(svd, _lib, args)
+        if "MISMATCH" in noise_stats["check"]:
+            log(f"noise workload: {noise_stats['check']}")
 
     out = {
         "metric": METRIC, "value": round(n / (ms_per_step / 1e3), 1), "unit": "voxels/s", "n_gpus": 1,
@@ -583,11 +795,12 @@ def main():
         "config": {"workload": spec["desc"], "n_keys": n, "depth": depth, "mode": "plain",
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "check": check, "chunks": chunk_stats,
-        "voxelize": vox_stats, "trace": trace_stats, "sort_keys": sort_stats,
+        "dropin": dropin, "noise": noise_stats, "voxelize": vox_stats, "trace": trace_stats,
+        "sort_keys": sort_stats,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
-    ok = check != "MISMATCH vs oracle" and (chunk_stats is None or chunk_stats["check"] != "MISMATCH")
+    ok = "MISMATCH" not in check and all(x is None or "MISMATCH" not in x["check"] for x in (chunk_stats, noise_stats))
     return 0 if ok else 1
 
 

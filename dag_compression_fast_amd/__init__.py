@@ -375,24 +375,27 @@ class _KeyBuffer(DeviceBuffer):
 
 
 def sort_keys(keys, bits, ctx=None):
-    """The reference's SortKeys (src/DAGCompression.cu:336-357): uint64 keys
-    (numpy or DeviceBuffer) sorted on their low `bits` bits on the device.
-    Returns a numpy array for numpy input, else a new DeviceBuffer."""
+    """The reference's SortKeys (src/DAGCompression.cu:336-357): uint64 or
+    uint32 keys (numpy or DeviceBuffer) sorted stably on their low `bits` bits
+    on the device.  Returns a numpy array for numpy input, else a new
+    DeviceBuffer."""
     ctx = ctx or default_context()
     if not isinstance(keys, DeviceBuffer):
         keys = np.ascontiguousarray(keys)
-    if np.dtype(keys.dtype) != np.uint64:
-        raise SvdagError(1, "sort_keys takes uint64 keys")
+    dt = np.dtype(keys.dtype)
+    if dt not in (np.dtype(np.uint64), np.dtype(np.uint32)):
+        raise SvdagError(1, "sort_keys takes uint64 or uint32 keys")
     buf, n, _ = _as_device_keys(keys)
+    fn = _lib.lib().svdag_sort_keys if dt == np.uint64 else _lib.lib().svdag_sort_keys_u32
     out = None
     try:
-        out = DeviceBuffer(8 * n)
-        out.dtype, out.count = np.dtype(np.uint64), n
-        _lib.check(_lib.lib().svdag_sort_keys(ctx.handle, buf.ptr, n, int(bits), out.ptr))
+        out = DeviceBuffer(dt.itemsize * n)
+        out.dtype, out.count = dt, n
+        _lib.check(fn(ctx.handle, buf.ptr, n, int(bits), out.ptr))
         if isinstance(keys, DeviceBuffer):
             result, out = out, None
             return result
-        return out.to_numpy(np.uint64, n)
+        return out.to_numpy(dt, n)
     finally:
         if out is not None:
             out.free()

@@ -27,6 +27,7 @@ OPT_RADIX_BITS = 6
 OPT_SORT_TILE = 7
 OPT_TABLE_ESTIMATE = 8
 OPT_VOXEL_STAGE_BYTES = 9
+OPT_SEG_FIRST_BITS = 10
 
 STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "HIP", 3: "MEMORY", 4: "INTERNAL", 5: "CAPACITY"}
 
@@ -154,6 +155,7 @@ def bind_library(L):
                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_query_voxels": ([vp, vp, u64, i32, vp, u64, vp], i32),
         "svdag_sort_keys": ([vp, vp, u64, i32, vp], i32),
+        "svdag_sort_keys_u32": ([vp, vp, u64, i32, vp], i32),
         "svdag_voxelize_triangles": ([vp, vp, u64, i32, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_free_keys": ([vp, vp], i32),
         "svdag_trace_params_from_view": ([dp, dp, dp, dp, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, dp, dp,

@@ -1,13 +1,20 @@
 """Autotuner for the build's speed options.
 
 The key-sort digit width (``SVDAG_OPT_RADIX_BITS``), the key-sort tile
-(``SVDAG_OPT_SORT_TILE``) and the dedup table load factor
+(``SVDAG_OPT_SORT_TILE``), the digit split of the batched build's per-chunk
+sort (``SVDAG_OPT_SEG_FIRST_BITS``) and the dedup table load factor
 (``SVDAG_OPT_TABLE_LOAD_SHIFT``) change how fast a build runs, never what it
 produces.  ``autotune`` times a caller-supplied build under every combination
 of a search space on one context, checks that every combination gives the same
 flat layout as the first, leaves the fastest combination set on the context and
 returns the whole table.  Results can be kept in a JSON cache keyed by device
 and workload, so a later run starts from the stored choice.
+
+Which options matter depends on the build: a plain build (and a batched build
+whose chunk-local bits above the leaf groups exceed 12, chunk depth >= 10)
+runs the global key sort, whose digit width and tile are tuned; a batched build
+of chunk depth <= 9 sorts inside each chunk, where only the digit split of that
+sort and the table load factor have an effect (``space_for``).
 
 The reference has no counterpart: its sort and table parameters are
 compile-time constants (src/Core.h:3-18, src/DAGCompression.cu:336-394).
@@ -27,13 +34,51 @@ SPACE = {
     "table_load_shift": (1, 2),
 }
 
+# Batched builds sorted inside each chunk: the first pass's digit width of that
+# sort, and the table load factor.
+SEG_SPACE = {
+    "seg_first_bits": (4, 5, 6, 7, 8),
+    "table_load_shift": (1, 2, 3),
+}
+
 _OPTION_IDS = {
     "radix_bits": _lib.OPT_RADIX_BITS,
     "sort_tile": _lib.OPT_SORT_TILE,
     "table_load_shift": _lib.OPT_TABLE_LOAD_SHIFT,
+    "seg_first_bits": _lib.OPT_SEG_FIRST_BITS,
 }
 
-DEFAULTS = {"radix_bits": 9, "sort_tile": 4096, "table_load_shift": 1}
+DEFAULTS = {"radix_bits": 9, "sort_tile": 4096, "table_load_shift": 1, "seg_first_bits": 0}
+
+# The key sort skips a leaf's 6 voxel bits and its 9-bit place in its block of
+# 512 leaves; the batched build sorts inside each chunk when at most this many
+# chunk-local bits remain.
+_SORT_SKIP_BITS = 15
+_SEG_MAX_BITS = 12
+
+
+def segmented(chunk_depth):
+    """Whether a batched build of this chunk depth sorts inside each chunk."""
+    bits = 3 * int(chunk_depth) - _SORT_SKIP_BITS
+    return 0 < bits <= _SEG_MAX_BITS
+
+
+def space_for(chunk_depth=None):
+    """The search space that matters for a batched build of `chunk_depth`
+    (None: a plain build)."""
+    return SEG_SPACE if chunk_depth is not None and segmented(chunk_depth) else SPACE
+
+
+def default_setting(space, chunk_depth=None):
+    """The library's default setting expressed in `space`'s options (the
+    segmented sort's default split is half its bits)."""
+    out = {}
+    for name in space:
+        v = DEFAULTS[name]
+        if name == "seg_first_bits" and v == 0 and chunk_depth is not None:
+            v = (3 * int(chunk_depth) - _SORT_SKIP_BITS + 1) // 2
+        out[name] = v
+    return out
 
 
 class TuneMismatch(RuntimeError):
@@ -53,11 +98,12 @@ def candidates(space=None):
         yield dict(zip(names, values))
 
 
-def autotune(ctx, build, fetch, reps=3, space=None, clock=time.perf_counter):
+def autotune(ctx, build, fetch, reps=3, space=None, clock=time.perf_counter, chunk_depth=None):
     """Time `build(ctx)` (one complete build on the context, synchronised, its
     result kept by the callee) under every candidate setting; `fetch()` returns
     the last build's flat layout as a numpy array for the equality check.
-    Returns {"best": setting, "best_ms": ms, "table": [(setting, ms), ...]}
+    Returns {"best": setting, "best_ms": ms, "table": [(setting, ms), ...],
+    "default_ms": ms of the library's default setting when it is a candidate}
     and leaves the best setting applied to ctx."""
     table = []
     reference = None
@@ -78,7 +124,9 @@ def autotune(ctx, build, fetch, reps=3, space=None, clock=time.perf_counter):
         table.append((setting, best * 1e3))
     best_setting, best_ms = min(table, key=lambda row: row[1])
     apply(ctx, best_setting)
-    return {"best": best_setting, "best_ms": best_ms, "table": table}
+    want = default_setting(space or SPACE, chunk_depth)
+    default_ms = next((ms for st, ms in table if st == want), None)
+    return {"best": best_setting, "best_ms": best_ms, "table": table, "default_ms": default_ms}
 
 
 def cache_path():
@@ -94,7 +142,7 @@ def load_cached(key, path=None):
             entry = json.load(f).get(key)
     except (OSError, ValueError):
         return None
-    if not entry or set(entry.get("best", {})) != set(SPACE):
+    if not entry or not entry.get("best") or not set(entry["best"]) <= set(_OPTION_IDS):
         return None
     return entry["best"]
 

@@ -140,6 +140,7 @@ struct svdag_context {
     uint32_t load_shift = 1;
     int radix_bits = 9;
     int sort_ipt = kSortIptSmall;
+    uint32_t seg_first_bits = 0;
     bool table_estimate = true;
     uint64_t vox_stage_max = kVoxStageMaxBytes;
     int host_threads = 1;
@@ -564,7 +565,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
                                           ctx->seg_tiles.size(), d_ct, d_off, batch->num_chunks, seg_lo,
                                           seg_chunk_bits, reinterpret_cast<uint32_t*>(A + o_seg_counts),
                                           reinterpret_cast<uint32_t*>(A + o_seg_totals),
-                                          reinterpret_cast<uint32_t*>(A + o_seg_starts), err);
+                                          reinterpret_cast<uint32_t*>(A + o_seg_starts), err, ctx->seg_first_bits);
         } else {
             where = launch_sort<uint64_t>(st, L.key_plan, reinterpret_cast<const uint64_t*>(d_keys), nullptr, ka, kb,
                                           nullptr, nullptr, nullptr, nullptr, &c->n, n,
@@ -1052,6 +1053,42 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     return 0;
 }
 
+// SortKeys (src/DAGCompression.cu:336-357: cub::DeviceRadixSort::SortKeys on
+// bits [0, bits)), stable like CUB's: every pass ranks stably.
+template <typename K>
+int sort_keys_entry(svdag_context* ctx, const K* d_keys, uint64_t n, int32_t bits, K* d_out)
+{
+    TRY(check_ctx(ctx));
+    const int key_bits = int(8 * sizeof(K));
+    if (bits < 1 || bits > key_bits) return set_error(SVDAG_ERR_ARGUMENT, "bits %d outside [1, %d]", bits, key_bits);
+    if (n >= (1ull << 32)) return set_error(SVDAG_ERR_ARGUMENT, "n = %llu keys: at most 2^32 - 1", (unsigned long long)n);
+    if (n && (!d_keys || !d_out)) return set_error(SVDAG_ERR_ARGUMENT, "null keys or output");
+    if (n && d_keys == d_out) return set_error(SVDAG_ERR_ARGUMENT, "output must not alias the input");
+    if (n == 0) return 0;
+    hipStream_t st = ctx->stream;
+    SortPlan plan = make_sort_plan(0, bits, ctx->radix_bits, ctx->sort_ipt);
+    if (plan.passes == 0) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
+    plan.stable = true;
+    Carver cv;
+    const uint64_t o_zero = cv.take(4 * sort_zero_words(plan));
+    const uint64_t o_n = cv.take(8), o_or = cv.take(8);
+    const uint64_t o_counts = cv.take(4 * sort_count_words(n));
+    const uint64_t o_a = cv.take(sizeof(K) * n), o_b = cv.take(sizeof(K) * n);
+    TRY(ensure_arena(ctx, cv.off));
+    char* A = ctx->arena;
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_zero), (o_n - o_zero) / 8, 0);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_n), 1, n);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_or), 1, 0);
+    const int where = launch_sort<K>(st, plan, d_keys, nullptr, reinterpret_cast<K*>(A + o_a),
+                                     reinterpret_cast<K*>(A + o_b), nullptr, nullptr, d_out, nullptr,
+                                     reinterpret_cast<const uint64_t*>(A + o_n), n,
+                                     reinterpret_cast<uint32_t*>(A + o_zero), reinterpret_cast<uint32_t*>(A + o_counts),
+                                     reinterpret_cast<unsigned long long*>(A + o_or));
+    if (where != 2) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
+    HIP_TRY(hipGetLastError());
+    return finish(ctx, "sort_keys");
+}
+
 }  // namespace
 
 extern "C" {
@@ -1115,6 +1152,11 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         ctx->radix_bits = int(value);
         return 0;
     case SVDAG_OPT_TABLE_ESTIMATE: ctx->table_estimate = value != 0; return 0;
+    case SVDAG_OPT_SEG_FIRST_BITS:
+        if (value < 0 || value > kMaxSegDigitBits)
+            return set_error(SVDAG_ERR_ARGUMENT, "segmented first-pass bits must be 0..%u", kMaxSegDigitBits);
+        ctx->seg_first_bits = uint32_t(value);
+        return 0;
     case SVDAG_OPT_VOXEL_STAGE_BYTES:
         if (value < 0) return set_error(SVDAG_ERR_ARGUMENT, "stage bytes must be >= 0");
         ctx->vox_stage_max = uint64_t(value);
@@ -1579,34 +1621,12 @@ int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys)
 
 int svdag_sort_keys(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t bits, uint64_t* d_out)
 {
-    TRY(check_ctx(ctx));
-    if (bits < 1 || bits > 64) return set_error(SVDAG_ERR_ARGUMENT, "bits %d outside [1, 64]", bits);
-    if (n >= (1ull << 32)) return set_error(SVDAG_ERR_ARGUMENT, "n = %llu keys: at most 2^32 - 1", (unsigned long long)n);
-    if (n && (!d_keys || !d_out)) return set_error(SVDAG_ERR_ARGUMENT, "null keys or output");
-    if (n && d_keys == d_out) return set_error(SVDAG_ERR_ARGUMENT, "output must not alias the input");
-    if (n == 0) return 0;
-    hipStream_t st = ctx->stream;
-    const SortPlan plan = make_sort_plan(0, bits, ctx->radix_bits, ctx->sort_ipt);
-    if (plan.passes == 0) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
-    Carver cv;
-    const uint64_t o_zero = cv.take(4 * sort_zero_words(plan));
-    const uint64_t o_n = cv.take(8), o_or = cv.take(8);
-    const uint64_t o_counts = cv.take(4 * sort_count_words(n));
-    const uint64_t o_a = cv.take(8 * n), o_b = cv.take(8 * n);
-    TRY(ensure_arena(ctx, cv.off));
-    char* A = ctx->arena;
-    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_zero), (o_n - o_zero) / 8, 0);
-    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_n), 1, n);
-    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_or), 1, 0);
-    const int where = launch_sort<uint64_t>(st, plan, d_keys, nullptr, reinterpret_cast<uint64_t*>(A + o_a),
-                                            reinterpret_cast<uint64_t*>(A + o_b), nullptr, nullptr, d_out, nullptr,
-                                            reinterpret_cast<const uint64_t*>(A + o_n), n,
-                                            reinterpret_cast<uint32_t*>(A + o_zero),
-                                            reinterpret_cast<uint32_t*>(A + o_counts),
-                                            reinterpret_cast<unsigned long long*>(A + o_or));
-    if (where != 2) return set_error(SVDAG_ERR_INTERNAL, "sort plan failed");
-    HIP_TRY(hipGetLastError());
-    return finish(ctx, "sort_keys");
+    return sort_keys_entry<uint64_t>(ctx, d_keys, n, bits, d_out);
+}
+
+int svdag_sort_keys_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n, int32_t bits, uint32_t* d_out)
+{
+    return sort_keys_entry<uint32_t>(ctx, d_keys, n, bits, d_out);
 }
 
 int svdag_query_voxels(svdag_context* ctx, const uint32_t* d_words, uint64_t n_words, int32_t depth,

@@ -89,6 +89,9 @@ struct SortPlan {
     uint32_t shift[kMaxPasses];
     uint32_t bits[kMaxPasses];
     int ipt;  // keys per thread and tile
+    // Key-only sorts rank their first pass unstably unless this is set; keys
+    // equal on the sorted bits then keep their input order (a stable sort).
+    bool stable;
 };
 
 // Digits of width <= max_bits covering bits [lo, hi), spread evenly.
@@ -107,8 +110,8 @@ inline uint64_t sort_zero_words(const SortPlan& p) { return uint64_t(p.passes) *
 inline uint64_t sort_count_words(uint64_t n_bound) { return uint64_t(kRadix) * sort_tiles(n_bound, kSortIptPairs); }
 
 // LSD radix sort of keys (and optionally u32 values) on the plan's bits.  With
-// values, keys equal on those bits keep their input order; without, their
-// order is unspecified (the first pass ranks unstably).
+// values, or with plan.stable, keys equal on those bits keep their input order;
+// otherwise their order is unspecified (the first pass ranks unstably).
 // n is read from device memory (*n_ptr, < 2^30); n_bound is a host upper bound
 // sizing the grid and the count buffer.  Passes ping-pong between (keys_a,
 // keys_b); the first pass reads keys_in.  Returns which buffer holds the result
@@ -132,7 +135,9 @@ struct ChunkPrefix {
 // most kMaxSegDigitBits) and puts each chunk's index above chunk_bits.
 // Scratch: counts (ntiles << kMaxSegDigitBits u32), totals and starts
 // (num_chunks << kMaxSegDigitBits u32 each).  Returns 0 (result in keys_a), 1
-// (keys_b), or -1 when the bit span does not fit.
+// (keys_b), or -1 when the bit span does not fit.  first_bits: the first
+// pass's digit width (0: half the bits), clamped so both passes have at most
+// kMaxSegDigitBits.
 struct SegTile {
     uint64_t start;
     uint32_t count;
@@ -144,7 +149,8 @@ constexpr uint32_t kMaxSegDigitBits = 8;
 int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
                           const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
                           const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
-                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err);
+                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err,
+                          uint32_t first_bits = 0);
 
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,

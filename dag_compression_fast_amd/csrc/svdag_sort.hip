@@ -251,10 +251,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_count(const K* __restrict
 }
 
 // Block d: exclusive scan of row d of the tile counts, offset by the digit's
-// global start (sum of the pass histogram below d).  In place.
+// global start (sum of the pass histogram below d).  In place.  Rows of at
+// most kScanItems waves' worth of tiles take a one-wave block.
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 4;
+constexpr uint64_t kSmallScanTiles = uint64_t(kWave) * kScanItems;
 
+template <int kScanThreads>
 __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ hist,
                                                             const uint64_t* __restrict__ n_ptr, uint64_t stride,
@@ -294,7 +297,11 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
 // Ranking of a scatter pass: unstable (the first pass of a key-only sort),
 // stable, or adaptive: unstable for a tile whose keys all agree on the bits
 // the earlier passes sorted (their order within a digit is then free; the
-// tiles follow the input order), stable otherwise.
+// tiles follow the input order), stable otherwise.  A stable key-only sort
+// (keys equal on the sorted bits keep their input order) ranks adaptively too
+// when no key has a bit at or above the sorted bits (*key_or, OR of all keys):
+// keys equal on the sorted bits are then identical; otherwise every tile ranks
+// stably.
 enum RankMode { kRankUnstable = 0, kRankStable = 1, kRankAdaptive = 2 };
 
 template <typename K, bool PAIRS, int MODE, int IPT, int THREADS>
@@ -305,7 +312,9 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
                                                                uint32_t digit_mask,
                                                                const uint32_t* __restrict__ offsets, uint64_t stride,
                                                                ChunkPrefix prefix, uint32_t sorted_shift,
-                                                               uint32_t sorted_mask)
+                                                               uint32_t sorted_mask,
+                                                               const unsigned long long* __restrict__ key_or,
+                                                               uint32_t end_bit)
 {
     constexpr bool MATCH = MODE != kRankUnstable;  // needs the stable ranking's tables
     constexpr bool DCNT = MODE != kRankStable;     // needs the unstable ranking's counters
@@ -336,6 +345,8 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
     const uint64_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t radix = digit_mask + 1;
 
+    // Keys above the sorted bits: every tile ranks stably.
+    const bool wide = MODE == kRankAdaptive && key_or && end_bit < 64 && (*key_or >> end_bit) != 0;
     const uint32_t wave_base = w * kWave * IPT;
     TileWalk walk = xcd_tiles(ntiles);
     K key[IPT];
@@ -415,7 +426,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
                 lo = min(lo, s_span[i]);
                 hi = max(hi, s_span[W + i]);
             }
-            stable = lo != hi;
+            stable = lo != hi || wide;
         }
 
         // Ranking, digit starts and staging; FULL (every key of the tile present,
@@ -548,7 +559,8 @@ constexpr unsigned kUnstableExtraLds = 20 * 1024;
 template <typename K, int IPT>
 static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t* vsrc, uint32_t* vdst,
                         const uint64_t* n_ptr, uint32_t shift, uint32_t dmask, uint32_t* counts, uint32_t* hist,
-                        uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix, uint32_t sorted_lo)
+                        uint64_t stride, uint32_t grid, bool first, const ChunkPrefix& prefix, uint32_t sorted_lo,
+                        bool stable, const unsigned long long* key_or, uint32_t end_bit)
 {
     const bool counted = first;  // the histogram kernel counted the first pass's tiles
     // Bits the earlier passes sorted: [sorted_lo, shift).  The adaptive ranking
@@ -572,15 +584,22 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
     }
     {
         TimedLaunch t(kKSortScan, stream);
-        hipLaunchKernelGGL(k_sort_scan, dim3(dmask + 1), dim3(kScanThreads), 0, stream, counts, hist, n_ptr, stride,
-                           uint64_t(kSortThreads) * IPT);
+        if (stride <= kSmallScanTiles)
+            hipLaunchKernelGGL(k_sort_scan<kWave>, dim3(dmask + 1), dim3(kWave), 0, stream, counts, hist, n_ptr,
+                               stride, uint64_t(kSortThreads) * IPT);
+        else
+            hipLaunchKernelGGL(k_sort_scan<kScanThreads>, dim3(dmask + 1), dim3(kScanThreads), 0, stream, counts,
+                               hist, n_ptr, stride, uint64_t(kSortThreads) * IPT);
     }
+    // A stable key-only sort ranks adaptively when it can (see k_sort_scatter);
+    // its first pass has no earlier bits, so every tile is uniform there.
+    const unsigned long long* wide_or = stable ? key_or : nullptr;
     if (vsrc) {
         TimedLaunch t(kKSortScatterPairs, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, true, kRankStable, IPT, kSortThreads>), dim3(scatter_grid),
                            dim3(kSortThreads), 0, stream, src, dst, vsrc, vdst, n_ptr, shift, dmask, counts, stride,
-                           prefix, sorted_lo, sorted_mask);
-    } else if (first) {
+                           prefix, sorted_lo, sorted_mask, nullptr, 64u);
+    } else if (first && !stable) {
         // Key-only first pass: the order within a digit is free.  Its kernel
         // needs fewer registers and would fit three workgroups per CU; the extra
         // LDS holds it to two, as the stable pass, which measured faster (more
@@ -588,20 +607,20 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
         TimedLaunch t(kKSortScatterKeys, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, false, kRankUnstable, IPT, kSortThreads>), dim3(scatter_grid),
                            dim3(kSortThreads), kUnstableExtraLds, stream, src, dst, nullptr, nullptr, n_ptr, shift,
-                           dmask, counts, stride, prefix, sorted_lo, sorted_mask);
-    } else if (span_fits) {
+                           dmask, counts, stride, prefix, sorted_lo, sorted_mask, nullptr, 64u);
+    } else if (span_fits || (first && wide_or)) {
         // Later key-only passes: tiles uniform in the earlier passes' bits (the
         // common case: the input is sorted on them, in runs longer than a tile)
         // rank unstably, the others stably.
         TimedLaunch t(kKSortScatterKeys, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, false, kRankAdaptive, IPT, kSortThreads>), dim3(scatter_grid),
                            dim3(kSortThreads), 0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts,
-                           stride, prefix, sorted_lo, sorted_mask);
+                           stride, prefix, sorted_lo, sorted_mask, wide_or, end_bit);
     } else {
         TimedLaunch t(kKSortScatterKeys, stream);
         hipLaunchKernelGGL((k_sort_scatter<K, false, kRankStable, IPT, kSortThreads>), dim3(scatter_grid),
                            dim3(kSortThreads), 0, stream, src, dst, nullptr, nullptr, n_ptr, shift, dmask, counts,
-                           stride, prefix, sorted_lo, sorted_mask);
+                           stride, prefix, sorted_lo, sorted_mask, nullptr, 64u);
     }
 }
 
@@ -651,7 +670,8 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
             if (last && final_vals) vdst = final_vals;
             const uint32_t dmask = (1u << plan.bits[p]) - 1u;
             launch_pass<K, I>(stream, src, dst, vsrc, vdst, n_ptr, plan.shift[p], dmask, counts, hist + p * kRadix,
-                              stride, grid, p == 0, p == 0 ? prefix : ChunkPrefix{}, plan.shift[0]);
+                              stride, grid, p == 0, p == 0 ? prefix : ChunkPrefix{}, plan.shift[0], plan.stable,
+                              key_or, plan.shift[plan.passes - 1] + plan.bits[plan.passes - 1]);
             src = dst;
             vsrc = vdst;
         }
@@ -900,11 +920,17 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
 int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
                           const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
                           const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
-                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err)
+                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err,
+                          uint32_t first_bits)
 {
     if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || ntiles == 0) return -1;
     const uint32_t B = chunk_bits - lo;
-    const uint32_t b1 = B > kMaxSegDigitBits ? (B + 1) / 2 : B;  // first pass digit width
+    // First pass digit width: half the bits unless asked otherwise, and never
+    // leaving either pass wider than kMaxSegDigitBits.
+    const uint32_t lo_b1 = B > kMaxSegDigitBits ? B - kMaxSegDigitBits : 1u;
+    const uint32_t hi_b1 = std::min(B, kMaxSegDigitBits);
+    const uint32_t b1 = first_bits ? std::min(hi_b1, std::max(lo_b1, first_bits))
+                                   : (B > kMaxSegDigitBits ? (B + 1) / 2 : B);
     const uint32_t b2 = B - b1;                                  // second (0: one pass)
     const int cus = device_cu_count();
     const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(ntiles, uint64_t(cus) * 64)));

@@ -88,10 +88,14 @@ typedef enum svdag_option {
                                          estimate of the leaf masks (default); 0: from the leaf
                                          positions.  An estimate that proves too small makes the
                                          call rebuild with the bound-sized table. */
-    SVDAG_OPT_VOXEL_STAGE_BYTES = 9   /* largest staging area of svdag_voxelize_triangles
+    SVDAG_OPT_VOXEL_STAGE_BYTES = 9,  /* largest staging area of svdag_voxelize_triangles
                                          (16 bytes per voxel column, default 4 GiB): within it
                                          the columns are evaluated once, past it twice (count,
                                          then write); 0 always takes two passes */
+    SVDAG_OPT_SEG_FIRST_BITS = 10     /* batched builds that sort inside each chunk (chunk
+                                         depth <= 9): digit width of that sort's first pass,
+                                         0 = half the chunk-local bits (default); clamped so
+                                         that neither pass is wider than 8 bits */
 } svdag_option;
 
 /* --- context ------------------------------------------------------------ */
@@ -189,13 +193,16 @@ int svdag_free_keys(svdag_context* ctx, uint64_t* d_keys);
 
 /* The reference's SortKeys (src/DAGCompression.cu:336-357; cub::DeviceRadixSort::
  * SortKeys on bits [0, end_bit)): d_out = the n keys of d_keys in ascending order
- * of their low `bits` bits.  Keys equal in those bits are identical under the
- * reference's precondition (keys < 2^bits); otherwise their order is
- * unspecified.  Device pointers; d_out must not alias d_keys, which is left
+ * of their low `bits` bits, stable as CUB's: keys equal in those bits keep their
+ * input order.  Device pointers; d_out must not alias d_keys, which is left
  * untouched; 1 <= bits <= 64, n < 2^32.  Scratch: 16 bytes per key plus the
  * digit counts, in the context's arena. */
 int svdag_sort_keys(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t bits,
                     uint64_t* d_out);
+/* The same for 32-bit keys (1 <= bits <= 32; 8 bytes of scratch per key): the
+ * reference's 32-bit Morton variant (MORTON_BITS, src/Voxelizer.cpp:207). */
+int svdag_sort_keys_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n, int32_t bits,
+                        uint32_t* d_out);
 
 /* Occupancy of voxels in a flat layout of a depth-`depth` DAG, walked as the
  * reference's tracer reads it (src/DAGTracer.cu:85-176).  d_words, d_codes and

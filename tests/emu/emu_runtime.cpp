@@ -262,6 +262,24 @@ unsigned long long exchange(unsigned long long v, int src)
 
 std::mutex g_launch_mutex;  // launches from several host threads run one at a time
 
+// Block schedule: 0 = blocks taken in increasing order by a pool of OS threads
+// (concurrent); 1 = one block at a time in decreasing order.  Under the second,
+// a block that waited for a lower-numbered block of the same launch would never
+// see it run (the spin guard aborts), and a block that read what another block
+// of the launch writes would read it before it is written.
+int g_schedule = 0;
+// Launch-configuration errors, reported by the next hipGetLastError (as the HIP
+// runtime reports an invalid configuration) instead of running the kernel.
+int g_sticky_error = 0;
+uint64_t g_launches = 0, g_bad_launches = 0;
+
+bool valid_config(dim3 grid, dim3 block)
+{
+    const uint64_t threads = uint64_t(block.x) * block.y * block.z;
+    return grid.x >= 1 && grid.y >= 1 && grid.z >= 1 && grid.x <= 0x7FFFFFFFu && grid.y <= 65535u &&
+           grid.z <= 65535u && threads >= 1 && threads <= 1024 && block.z <= 1024;
+}
+
 void launch(dim3 grid, dim3 block, std::function<void()> body)
 {
     std::lock_guard<std::mutex> serial(g_launch_mutex);
@@ -269,22 +287,33 @@ void launch(dim3 grid, dim3 block, std::function<void()> body)
         std::fprintf(stderr, "emu: launch from inside a kernel\n");
         std::abort();
     }
+    ++g_launches;
+    if (getenv("EMU_TRACE")) std::fprintf(stderr, "launch grid %u %u %u block %u\n", grid.x, grid.y, grid.z, block.x * block.y * block.z);
+    if (!valid_config(grid, block)) {
+        std::fprintf(stderr, "emu: invalid launch configuration grid (%u, %u, %u) block (%u, %u, %u)\n", grid.x,
+                     grid.y, grid.z, block.x, block.y, block.z);
+        ++g_bad_launches;
+        g_sticky_error = 9;  // hipErrorInvalidConfiguration
+        return;
+    }
     g_grid_dim = grid;
     g_block_dim = block;
     g_body = &body;
     const unsigned nthreads = block.x * block.y * block.z;
     const unsigned nblocks = grid.x * grid.y * grid.z;
     std::atomic<unsigned> next{0};
+    const bool reverse = g_schedule == 1;
     auto worker = [&]() {
         for (;;) {
-            const unsigned b = next.fetch_add(1);
-            if (b >= nblocks) break;
+            const unsigned i = next.fetch_add(1);
+            if (i >= nblocks) break;
+            const unsigned b = reverse ? nblocks - 1 - i : i;
             g_block = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
             g_block_start = std::chrono::steady_clock::now();
             run_block(nthreads);
         }
     };
-    const unsigned workers = std::min<unsigned>(nblocks, unsigned(g_launch_threads));
+    const unsigned workers = reverse ? 1u : std::min<unsigned>(nblocks, unsigned(g_launch_threads));
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < workers; ++t) pool.emplace_back(worker);
     worker();
@@ -315,7 +344,16 @@ hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, int k, hipStream_t) 
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
-hipError_t hipGetLastError() { return hipSuccess; }
+hipError_t hipGetLastError()
+{
+    const hipError_t e = emu::g_sticky_error;
+    emu::g_sticky_error = 0;
+    return e;
+}
+hipError_t hipPeekAtLastError() { return emu::g_sticky_error; }
+void emu_set_schedule(int mode) { emu::g_schedule = mode; }
+uint64_t emu_launch_count() { return emu::g_launches; }
+uint64_t emu_bad_launch_count() { return emu::g_bad_launches; }
 const char* hipGetErrorString(hipError_t) { return "emulator error"; }
 hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }

@@ -3,9 +3,12 @@
 // Lets the product's .hip sources compile with g++: every GPU thread of a block
 // runs as a cooperative fiber; __syncthreads, wave ballots and shuffles are
 // rendezvous points.  Blocks run concurrently on a pool of OS threads (LDS is
-// thread_local, atomics are real), so inter-block waits such as decoupled
-// look-back are exercised.  Device memory is host memory filled with a garbage
-// pattern, so code that relies on zeroed allocations misbehaves here too.
+// thread_local, atomics are real), or, under emu_set_schedule(1), one at a time
+// in decreasing block order, so a block that depends on another block of the
+// same launch shows up as a stall or a wrong result.  Launch configurations are
+// checked (a zero or oversized grid or block is an error of hipGetLastError).
+// Device memory is host memory filled with a garbage pattern, so code that
+// relies on zeroed allocations misbehaves here too.
 #pragma once
 
 #include <cstddef>

@@ -83,6 +83,23 @@ def test_option_ids_match_the_header():
     for name, opt in (("RADIX_BITS", _lib.OPT_RADIX_BITS), ("SORT_TILE", _lib.OPT_SORT_TILE),
                       ("TABLE_LOAD_SHIFT", _lib.OPT_TABLE_LOAD_SHIFT)):
         assert f"SVDAG_OPT_{name} = {opt}," in text
+    assert f"SVDAG_OPT_SEG_FIRST_BITS = {_lib.OPT_SEG_FIRST_BITS} " in text
+
+
+def test_space_follows_the_build_kind():
+    # Chunk depth 9 leaves 12 chunk-local bits to sort: the per-chunk sort.
+    assert autotune.space_for(9) is autotune.SEG_SPACE
+    assert autotune.space_for(6) is autotune.SEG_SPACE
+    assert autotune.space_for(10) is autotune.SPACE
+    assert autotune.space_for(None) is autotune.SPACE
+    assert autotune.default_setting(autotune.SEG_SPACE, 9) == {"seg_first_bits": 6, "table_load_shift": 1}
+
+
+def test_default_time_is_reported():
+    ctx = FakeCtx()
+    build, fetch, clock = make_run(ctx)
+    res = autotune.autotune(ctx, build, fetch, reps=1, clock=clock)
+    assert res["default_ms"] == pytest.approx(fake_cost(autotune.default_setting(autotune.SPACE)))
 
 
 @pytest.mark.gpu
@@ -105,6 +122,11 @@ def test_autotune_batched_build_on_gpu(gpu):
         res = autotune.autotune(ctx, build, lambda: state["words"], reps=1,
                                 space={"radix_bits": (6, 9), "sort_tile": (4096,), "table_load_shift": (1, 2)})
         assert len(res["table"]) == 4
+        assert np.array_equal(gpu.create_final_dag(gpu.create_dag_batched(keys, offsets, depth, ctx=ctx)), want)
+        # Chunk depth 6 sorts inside each chunk (3 bits): its digit split and the load factor.
+        res = autotune.autotune(ctx, build, lambda: state["words"], reps=1, space=autotune.space_for(depth),
+                                chunk_depth=depth)
+        assert len(res["table"]) == 5 * 3
         assert np.array_equal(gpu.create_final_dag(gpu.create_dag_batched(keys, offsets, depth, ctx=ctx)), want)
     finally:
         ctx.close()

@@ -90,6 +90,8 @@ def test_emu_merge_and_batched(emu, oracle):
     T.test_batched_matches_merge(emu, oracle)
     for chunk_depth in (6, 8, 9, 10):
         T.test_batched_chunk_depths(emu, oracle, chunk_depth)
+    for first_bits in (4, 8):
+        T.test_batched_segmented_digit_split(emu, oracle, first_bits)
 
 
 def test_emu_errors(emu):

@@ -8,11 +8,6 @@ import pytest
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "svdag_golden.npz")
 
-# Inputs whose builds stalled on the GPU under an earlier kernel design (no
-# cause shown): they stay off the GPU and are covered by the CPU emulator.
-GPU_SKIP = {"kat_d3_origin", "kat_d3_corner", "kat_d4_origin", "kat_d4_corner", "cube_d4"}
-
-
 def load():
     with np.load(GOLDEN) as z:
         data = {k: z[k] for k in z.files}
@@ -54,9 +49,7 @@ def test_emulator_reproduces_golden(emu, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", [pytest.param(c, marks=pytest.mark.skip(
-    reason="stalled once on the GPU under an earlier design; covered by the CPU emulator"))
-    if c[0] in GPU_SKIP else c for c in CASES], ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_gpu_reproduces_golden(gpu, case):
     name, keys, depth, counts, leaves, words = case
     print(f"[golden] {name}: depth={depth} n={len(keys)}", flush=True)

@@ -45,11 +45,10 @@ def check_case(gpu, oracle, keys, depth, what):
     return dag, words
 
 
-# The single-key closed forms hung inside create_dag on the first GPU run; no
-# cause was found on the CPU (the emulator and oracle pass them), so they stay
-# off the GPU and run on the CPU kernel emulator instead (test_emu_parity.py).
-SINGLE_KEY_SKIP = "single-key build hung once on the GPU with no cause shown; covered by the CPU emulator"
-
+# Inputs that once stalled under an earlier kernel design whose workgroups
+# waited on each other (decoupled look-back, tile tickets); that design is gone
+# and no kernel waits on another workgroup (tests/test_emu_schedule.py runs
+# these inputs with every launch's blocks one at a time in reverse order).
 KATS = [
     ([0], 3, [0x1, 0x2, 0x1, 0x0]),
     ([0x1FF], 3, [0x80, 0x2, 0x0, 0x80000000]),
@@ -58,7 +57,6 @@ KATS = [
 ]
 
 
-@pytest.mark.skip(reason=SINGLE_KEY_SKIP)
 @pytest.mark.parametrize("keys,depth,words", KATS)
 def test_closed_form_kats(gpu, keys, depth, words):
     print(f"[gpu-parity] KAT depth={depth} keys={keys}", flush=True)
@@ -66,14 +64,7 @@ def test_closed_form_kats(gpu, keys, depth, words):
     assert gpu.create_final_dag(dag).tolist() == words
 
 
-# Full cubes of depth 3 and 4 stalled inside create_dag on the GPU with an
-# earlier kernel design (inter-workgroup waits); no cause was shown, so these
-# inputs stay off the GPU and run on the CPU kernel emulator instead.
-CUBE_SKIP = "stalled once on the GPU with no cause shown; covered by the CPU emulator"
-
-
-@pytest.mark.parametrize("depth", [pytest.param(3, marks=pytest.mark.skip(reason=CUBE_SKIP)),
-                                   pytest.param(4, marks=pytest.mark.skip(reason=CUBE_SKIP)), 5, 6])
+@pytest.mark.parametrize("depth", [3, 4, 5, 6])
 def test_full_cube(gpu, depth):
     print(f"[gpu-parity] full cube depth={depth}", flush=True)
     keys = np.arange(8 ** depth, dtype=np.uint64)
@@ -83,8 +74,7 @@ def test_full_cube(gpu, depth):
     assert words[-2] == 0xFFFFFFFF and words[-1] == 0xFFFFFFFF
 
 
-@pytest.mark.parametrize("depth,n", [pytest.param(3, 1, marks=pytest.mark.skip(reason=SINGLE_KEY_SKIP)),
-                                     (3, 50), (4, 300), (5, 5000), (6, 20000), (8, 100000),
+@pytest.mark.parametrize("depth,n", [(3, 1), (3, 50), (4, 300), (5, 5000), (6, 20000), (8, 100000),
                                      (10, 200000), (12, 300000), (16, 200000), (21, 100000)])
 def test_random_uniform(gpu, oracle, depth, n):
     rng = np.random.default_rng(1000 + depth)
@@ -226,5 +216,27 @@ def test_radix_digit_widths(gpu, oracle, bits, tile):
         assert_dag_equal(dag.to_numpy(), exp, f"radix {bits} tile {tile}")
         assert np.array_equal(gpu.create_final_dag(dag), oracle.create_final_dag(exp))
         dag.free()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("first_bits", [4, 5, 8])
+def test_batched_segmented_digit_split(gpu, oracle, first_bits):
+    """The per-chunk sort of chunk depth 9 (12 bits) with another digit split
+    (4 + 8, 5 + 7, 8 + 4 bits) gives the same DAG."""
+    from dag_compression_fast_amd import _lib
+    rng = np.random.default_rng(400 + first_bits)
+    sizes = [0, 1, 5000, 4096, 70000, 0, 12000, 300]
+    chunks = [clustered_keys(rng, 9, n) if n else np.zeros(0, dtype=np.uint64) for n in sizes]
+    offsets = np.zeros(len(chunks) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(c) for c in chunks])
+    ctx = gpu.Context()
+    try:
+        ctx.set_option(_lib.OPT_SEG_FIRST_BITS, first_bits)
+        print(f"[gpu-parity] batched d9 first bits {first_bits}", flush=True)
+        batched = gpu.create_dag_batched(np.concatenate(chunks), offsets, 9, ctx=ctx)
+        exp = oracle.merge_dags([oracle.create_dag(c, 9) for c in chunks])
+        assert_dag_equal(batched.to_numpy(), exp, f"segmented split {first_bits}")
+        batched.free()
     finally:
         ctx.close()
