@@ -124,6 +124,8 @@ def bind_library(L):
         "svdag_scratch_bytes": ([u64, i32, i32], u64),
         "svdag_create_dag": ([vp, vp, u64, i32, dagp], i32),
         "svdag_create_dag_u32": ([vp, vp, u64, i32, dagp], i32),
+        "svdag_create_dag_noalloc": ([vp, vp, u64, i32, vp, u64, dagp], i32),
+        "svdag_result_bytes": ([u64, i32], u64),
         "svdag_merge_dags": ([vp, dagp, u64, dagp], i32),
         "svdag_create_dag_batched": ([vp, vp, ctypes.POINTER(u64), u64, i32, dagp], i32),
         "svdag_create_final_dag": ([vp, dagp, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),

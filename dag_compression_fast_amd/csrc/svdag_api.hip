@@ -19,12 +19,19 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 using namespace svdag;
+
+// Fixed device scratch of svdag_create_dag_noalloc, one copy per GPU that the
+// library runs on, held until the process ends.  In host code this name is a
+// host-side stand-in, not a device address: host code uses only the address
+// that hipGetSymbolAddress returns for the current device (fixed_scratch).
+static __device__ __align__(256) char scratch[SVDAG_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -473,9 +480,30 @@ struct BatchInfo {
 // caller repeats the build with bound-sized tables.
 constexpr int kRetryWithBounds = -100;
 
+// Memory of a build that allocates nothing: scratch in place of the context's
+// arena, the caller's buffer in place of a pooled result block.  Such a build
+// also keeps to the context's stream (no side stream, which is created on
+// first use) and times no kernels (timing events are created on demand).
+struct FixedMemory {
+    char* arena;
+    uint64_t arena_bytes;
+    char* storage;
+    uint64_t storage_bytes;
+};
+
+// Result storage of a build over n keys at `depth` (the layout of plan_storage).
+uint64_t result_bytes(uint64_t n, int depth)
+{
+    const int levels = depth - 2;
+    const uint64_t leaf_bound = std::min<uint64_t>(n, pow8_capped(depth - 2, n));
+    uint64_t caps[SVDAG_MAX_LEVELS];
+    for (int l = 0; l < levels; ++l) caps[l] = pow8_capped(l, leaf_bound);
+    return plan_storage(levels, caps, leaf_bound).total;
+}
+
 template <typename K>
 int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag* out, const BatchInfo* batch,
-          bool estimate)
+          bool estimate, const FixedMemory* fixed = nullptr)
 {
     hipStream_t st = ctx->stream;
     const int levels = depth - 2;
@@ -513,8 +541,12 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         o_seg_starts = sc.take(4 * (batch->num_chunks << kMaxSegDigitBits));
         seg_bytes = sc.off - (L.total + offsets_bytes);
     }
-    TRY(ensure_arena(ctx, L.total + offsets_bytes + seg_bytes));
-    char* A = ctx->arena;
+    const uint64_t scratch_need = L.total + offsets_bytes + seg_bytes;
+    if (fixed && scratch_need > fixed->arena_bytes)
+        return set_error(SVDAG_ERR_MEMORY, "build needs %llu scratch bytes, the fixed scratch holds %llu",
+                         (unsigned long long)scratch_need, (unsigned long long)fixed->arena_bytes);
+    if (!fixed) TRY(ensure_arena(ctx, scratch_need));
+    char* A = fixed ? fixed->arena : ctx->arena;
     BuildCounters* c = reinterpret_cast<BuildCounters*>(A + L.counters);
     unsigned long long* err = reinterpret_cast<unsigned long long*>(&c->error);
     uint32_t* sort_counts = reinterpret_cast<uint32_t*>(A + L.sort_counts);
@@ -525,12 +557,19 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     for (int l = 0; l < levels; ++l) caps[l] = pow8_capped(l, L.leaf_bound);
     const DagStorage S = plan_storage(levels, caps, L.leaf_bound);
     void* storage = nullptr;
-    TRY(device_alloc(ctx, &storage, S.total));
+    if (fixed) {
+        if (S.total > fixed->storage_bytes)
+            return set_error(SVDAG_ERR_MEMORY, "result needs %llu bytes, the caller's storage holds %llu",
+                             (unsigned long long)S.total, (unsigned long long)fixed->storage_bytes);
+        storage = fixed->storage;
+    } else {
+        TRY(device_alloc(ctx, &storage, S.total));
+    }
     char* O = static_cast<char*>(storage);
     bool forked = false;  // work is in flight on the side stream
     auto fail = [&](int code) {
         if (forked) (void)hipStreamSynchronize(ctx->side);
-        device_release(ctx, storage);
+        if (!fixed) device_release(ctx, storage);
         out->storage = nullptr;
         return code;
     };
@@ -628,7 +667,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     // keep one stream (many concurrent contexts would share hardware queues).
     uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
     hipStream_t ls = st;
-    if (n >= kOverlapMinKeys && ensure_side(ctx) && hipEventRecord(ctx->fork_ev, st) == hipSuccess &&
+    if (n >= kOverlapMinKeys && !fixed && ensure_side(ctx) && hipEventRecord(ctx->fork_ev, st) == hipSuccess &&
         hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0) == hipSuccess) {
         ls = ctx->side;
         forked = true;
@@ -792,7 +831,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     if (h.error & kErrInternal) return fail(set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow"));
     if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
 
-    out->storage = storage;
+    out->storage = fixed ? nullptr : storage;  // the caller's buffer is not the library's to free
     out->leaves = leaves_out;
     out->num_leaves = h.unique_leaves;
     out->leaf_positions = h.leaf_positions;
@@ -1185,6 +1224,59 @@ int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int
     if (n && !d_keys) return set_error(SVDAG_ERR_ARGUMENT, "null keys");
     const int rc = build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, ctx->table_estimate);
     return rc == kRetryWithBounds ? build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, false) : rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device address of `scratch` on each device, resolved once per device (with
+// that device current) on the first no-allocation call there.
+constexpr int kMaxDevices = 64;
+std::mutex g_fixed_mutex;
+char* g_fixed_scratch[kMaxDevices] = {};
+
+int fixed_scratch(int device, char** out)
+{
+    if (device < 0 || device >= kMaxDevices) return set_error(SVDAG_ERR_ARGUMENT, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_fixed_mutex);
+    if (!g_fixed_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(SVDAG_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                             hipGetErrorString(e));
+        g_fixed_scratch[device] = static_cast<char*>(p);
+    }
+    *out = g_fixed_scratch[device];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t svdag_result_bytes(uint64_t n, int32_t depth)
+{
+    if (depth < 3 || depth > 21 || n == 0) return 0;
+    return result_bytes(n, depth);
+}
+
+int svdag_create_dag_noalloc(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t depth,
+                             void* d_storage, uint64_t storage_bytes, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    // Kernel timing creates its events on demand: off for this call.
+    active_recorder() = nullptr;
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    if (depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "depth %d outside [3, 21]", depth);
+    if (n >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "%llu keys: at most 2^30 - 1 per call", (unsigned long long)n);
+    if (n && (!d_keys || !d_storage)) return set_error(SVDAG_ERR_ARGUMENT, "null keys or storage");
+    char* base = nullptr;
+    if (n) TRY(fixed_scratch(ctx->device, &base));
+    const FixedMemory fm{base, SVDAG_NOALLOC_SCRATCH_BYTES, static_cast<char*>(d_storage), storage_bytes};
+    const int rc = build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, ctx->table_estimate, &fm);
+    return rc == kRetryWithBounds ? build<uint64_t>(ctx, d_keys, n, depth, out, nullptr, false, &fm) : rc;
 }
 
 int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)

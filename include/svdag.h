@@ -123,6 +123,27 @@ uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes);
 int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n,
                      int32_t depth, svdag_dag* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc, work set up for
+ * graph capture): svdag_create_dag with its scratch in a fixed device array of
+ * the library (SVDAG_NOALLOC_SCRATCH_BYTES, one copy per GPU, held until the
+ * process ends) and its result in the caller's device buffer d_storage, of
+ * storage_bytes >= svdag_result_bytes(n, depth).  Calls no hipMalloc, hipFree
+ * or allocating routine.  A build needing more scratch than the array holds
+ * (svdag_scratch_bytes(n, depth, 8) at load shift 1; about 4M keys at depth 9,
+ * 1M at depth 10 to 12) or more storage than given fails with SVDAG_ERR_MEMORY
+ * before any device work.  `out` points into d_storage and owns nothing
+ * (out->storage is NULL; svdag_free_dag releases nothing).  Every call on a
+ * GPU shares that GPU's array, which keeps the last call's contents (each call
+ * clears what must start at zero): calls that overlap on one GPU, from any
+ * context or stream, race and must be serialised by the caller. */
+#define SVDAG_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int svdag_create_dag_noalloc(svdag_context* ctx, const uint64_t* d_keys, uint64_t n,
+                             int32_t depth, void* d_storage, uint64_t storage_bytes,
+                             svdag_dag* out);
+/* Result bytes svdag_create_dag_noalloc needs for n keys at depth. */
+uint64_t svdag_result_bytes(uint64_t n, int32_t depth);
+
 /* Same with 32-bit keys (depth <= 10): the 512^3 u32 variant. */
 int svdag_create_dag_u32(svdag_context* ctx, const uint32_t* d_keys, uint64_t n,
                          int32_t depth, svdag_dag* out);

@@ -288,7 +288,6 @@ void launch(dim3 grid, dim3 block, std::function<void()> body)
         std::abort();
     }
     ++g_launches;
-    if (getenv("EMU_TRACE")) std::fprintf(stderr, "launch grid %u %u %u block %u\n", grid.x, grid.y, grid.z, block.x * block.y * block.z);
     if (!valid_config(grid, block)) {
         std::fprintf(stderr, "emu: invalid launch configuration grid (%u, %u, %u) block (%u, %u, %u)\n", grid.x,
                      grid.y, grid.z, block.x, block.y, block.z);
@@ -335,6 +334,12 @@ static void* emu_alloc(size_t n)
 
 hipError_t hipMalloc(void** p, size_t n) { *p = emu_alloc(n); return *p ? hipSuccess : hipErrorOutOfMemory; }
 hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+// Device globals are host globals here: the symbol is its own address.
+hipError_t hipGetSymbolAddress(void** p, const void* symbol)
+{
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 hipError_t hipMallocAsync(void** p, size_t n, hipStream_t) { return hipMalloc(p, n); }
 hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
 hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }

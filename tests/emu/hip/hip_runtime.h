@@ -19,6 +19,8 @@
 
 #define __global__
 #define __device__
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
@@ -163,6 +165,7 @@ struct hipDeviceProp_t { int multiProcessorCount; char name[256]; };
 extern "C" {
 hipError_t hipMalloc(void** p, size_t n);
 hipError_t hipFree(void* p);
+hipError_t hipGetSymbolAddress(void** p, const void* symbol);
 hipError_t hipMallocAsync(void** p, size_t n, hipStream_t s);
 hipError_t hipFreeAsync(void* p, hipStream_t s);
 hipError_t hipHostMalloc(void** p, size_t n, unsigned flags);
