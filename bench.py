@@ -364,8 +364,9 @@ def bench_chunks(svd, _lib, args):
             log(f"  batched {k:20s} {ms:8.3f} ms  {cnt:4d} launches")
     same = np.array_equal(w_plain, w_batch) and np.array_equal(w_union, w_batch)
 
-    served = bench_served(svd, _lib, args, chunks, w_batch)
-    same = same and all(s["bit_exact"] for s in served)
+    served = bench_served(svd, _lib, args, d_keys, offsets, w_batch)
+    same = same and served["saturated"]["bit_exact_vs_batched"] and all(
+        r["same_dag_as_other_rates"] and r["bit_exact_vs_merge"] for r in served.get("sweep", {}).get("rates", []))
     d_keys.free()
     ctx.close()
 
@@ -388,40 +389,123 @@ def bench_chunks(svd, _lib, args):
             "union_ms": round(t_union * 1e3, 3), "batched_voxels_per_s": round(total / t_batch, 1),
             "batched_ms": round(t_batch * 1e3, 3), "batched_vs_plain": round(t_plain / t_batch, 2),
             "batched_vs_union": round(t_union / t_batch, 3),
-            "served": served, "autotune": tuned, "cpu_baseline": cpu, "check": check}
+            "served": served, "served_vs_batched": round(served["saturated"]["voxels_per_s"] * t_batch / total, 3),
+            "autotune": tuned, "cpu_baseline": cpu, "check": check}
 
 
-def bench_served(svd, _lib, args, chunks, w_expected):
-    """Served mode on the chunk workload: Poisson arrivals at each rate, each
-    request one chunk at its own slot; the final global DAG must equal the
-    batched build's."""
+def bench_served(svd, _lib, args, d_keys, offsets, w_batch):
+    """Served mode (continuous batching into a resident global DAG).
+
+    Saturated: the 512 chunks already in device memory all arrive at once;
+    rounds are cut by the HBM budget; the final DAG must equal the batched
+    build's.  Sweep: 2000 requests drawn as the 512-chunk workload at distinct
+    slots of a 4096-slot grid, Poisson arrivals at each rate (2000 requests, or
+    30 s at the lowest rates); every rate must end with the same global DAG,
+    which must equal the reference's flow over the same chunks (8 batched
+    builds of 512 slots merged by MergeDAGs)."""
     from dag_compression_fast_amd import serve
-    L = _lib.lib()
-    total = sum(len(c) for c in chunks)
-    served = []
-    server = serve.ChunkServer(9, slots=args.serve_slots, max_chunk_keys=max(len(c) for c in chunks))
-    try:
-        for rate in args.serve_rates:
-            arrivals = serve.poisson_arrivals(rate, len(chunks), seed=int(rate))
-            order = np.random.default_rng(7).permutation(len(chunks))
-            reqs = [serve.Request(i, int(order[i]), chunks[order[i]], float(arrivals[i])) for i in range(len(chunks))]
-            t = time.perf_counter()
+    from dag_compression_fast_amd import workloads as W
+    L, H = _lib.lib(), _lib.hip()
+    out = {}
+    total = int(offsets[-1])
+    views = [svd.DeviceView(d_keys, 8 * int(offsets[i]), int(offsets[i + 1] - offsets[i]))
+             for i in range(len(offsets) - 1)]
+    # One context for every server: a running server's scratch and result
+    # blocks are warm.  The first saturated run warms them; the second is measured.
+    sctx = svd.Context()
+    for run in range(2):
+        server = serve.ChunkServer(W.CHUNKS["depth"], W.CHUNKS["levels_above"], max_nodes=max(1 << 20, total // 16),
+                                   ctx=sctx)
+        try:
+            reqs = [serve.Request(i, i, views[i], 0.0) for i in range(len(views))]
             results = server.run(reqs)
             span = max(r.done for r in results) - min(r.arrival for r in results)
-            merged = server.merge(results, len(chunks))
-            w_served = final_words(L, _lib, merged.ctx, merged.raw)
+            dag = server.dag()
+            same = np.array_equal(final_words(L, _lib, sctx, dag.raw), w_batch)
+            L.svdag_free_dag(sctx.handle, ctypes.byref(dag.raw))
+            rounds = server.rounds
+        finally:
+            server.close()
+    stats = serve.latency_stats(results)
+    out["saturated"] = {"requests": len(reqs), "rounds": rounds, "voxels_per_s": round(total / span, 1),
+                        "ms": round(span * 1e3, 3), **{k: round(v, 3) for k, v in stats.items()},
+                        "bit_exact_vs_batched": bool(same)}
+    log(f"served saturated: {len(reqs)} requests in {rounds} rounds, {span * 1e3:.2f} ms = "
+        f"{total / span / 1e9:.1f} G voxels/s, identical to batched: {same}")
+    if not args.serve_rates:
+        sctx.close()
+        return out
+
+    spec = W.SERVED
+    t0 = time.perf_counter()
+    chunks = W.chunk_workload(args.threads, seed=spec["seed"], num=spec["requests"], depth=spec["depth"])
+    sizes = np.array([len(c) for c in chunks], dtype=np.uint64)
+    c_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    all_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
+    del chunks
+    nslots = 8 ** spec["slot_levels"]
+    slot_of = np.random.default_rng(spec["seed"]).permutation(nslots)[: spec["requests"]]
+    views = [svd.DeviceView(all_keys, 8 * int(c_off[i]), int(sizes[i])) for i in range(spec["requests"])]
+    log(f"served sweep: {spec['requests']} requests, {int(c_off[-1])} keys, {nslots} slots "
+        f"(generated and uploaded in {time.perf_counter() - t0:.1f} s)")
+    sweep, words_seen = [], {}
+    try:
+        for rate in args.serve_rates:
+            n_req = min(spec["requests"], max(1, int(np.ceil(spec["min_seconds"] * rate))))
+            arrivals = serve.poisson_arrivals(rate, n_req, seed=int(rate))
+            reqs = [serve.Request(i, int(slot_of[i]), views[i], float(arrivals[i])) for i in range(n_req)]
+            keys = int(sizes[:n_req].sum())
+            server = serve.ChunkServer(spec["depth"], spec["slot_levels"], max_nodes=max(1 << 20, keys // 16),
+                                       ctx=sctx)
+            try:
+                t = time.perf_counter()
+                results = server.run(reqs)
+                span = max(r.done for r in results) - min(r.arrival for r in results)
+                dag = server.dag()
+                words = final_words(L, _lib, sctx, dag.raw)
+                L.svdag_free_dag(sctx.handle, ctypes.byref(dag.raw))
+                rounds = server.rounds
+            finally:
+                server.close()
             stats = serve.latency_stats(results)
-            served.append({"rate_per_s": rate, "requests": len(reqs), "voxels_per_s": round(total / span, 1),
-                           **{k: round(v, 3) for k, v in stats.items()},
-                           "bit_exact": bool(np.array_equal(w_served, w_expected))})
-            log(f"served {rate}/s: {total / span / 1e9:.2f} Gvox/s p50 {stats['p50_ms']:.2f} ms "
-                f"p99 {stats['p99_ms']:.2f} ms ({time.perf_counter() - t:.1f} s)")
-            merged.free()
-            for r in results:
-                r.dag.free()
+            key = n_req
+            if key not in words_seen:
+                words_seen[key] = words
+            same = np.array_equal(words, words_seen[key])
+            sweep.append({"rate_per_s": rate, "requests": n_req, "seconds": round(span, 2), "rounds": rounds,
+                          "voxels_per_s": round(keys / span, 1), **{k: round(v, 3) for k, v in stats.items()},
+                          "same_dag_as_other_rates": bool(same)})
+            log(f"served {rate:.0f}/s: {n_req} requests, {rounds} rounds, {keys / span / 1e9:.2f} G voxels/s, "
+                f"p50 {stats['p50_ms']:.2f} ms p99 {stats['p99_ms']:.2f} ms ({time.perf_counter() - t:.1f} s)")
+        # The reference's flow over the same chunks: the 8 octants of the
+        # slot grid (512 slots each, batched) merged by MergeDAGs.
+        ref_words = {}
+        per = nslots // 8
+        for n_req in sorted(words_seen):
+            by_slot = {int(slot_of[i]): views[i] for i in range(n_req)}
+            dags = (_lib.SvdagDag * 8)()
+            ctx = svd.Context()
+            try:
+                for o in range(8):
+                    bufs = [by_slot.get(o * per + j) for j in range(per)]
+                    d = svd.create_dag_batched_gather(bufs, spec["depth"], ctx=ctx)
+                    dags[o] = d.raw
+                    d.raw = _lib.SvdagDag()  # ownership moves to the array
+                merged = _lib.SvdagDag()
+                _lib.check(L.svdag_merge_dags(ctx.handle, dags, 8, ctypes.byref(merged)))
+                for o in range(8):
+                    L.svdag_free_dag(ctx.handle, ctypes.byref(dags[o]))
+                ref_words[n_req] = final_words(L, _lib, ctx, merged)
+                L.svdag_free_dag(ctx.handle, ctypes.byref(merged))
+            finally:
+                ctx.close()
+        for row in sweep:
+            row["bit_exact_vs_merge"] = bool(np.array_equal(words_seen[row["requests"]], ref_words[row["requests"]]))
     finally:
-        server.close()
-    return served
+        all_keys.free()
+        sctx.close()
+    out["sweep"] = {"workload": spec["desc"], "rates": sweep}
+    return out
 
 
 def bench_noise(svd, _lib, args):
@@ -637,9 +721,8 @@ def main():
     ap.add_argument("--profile-noise", type=int, default=0, metavar="N",
                     help="only run N rounds of the value-noise flow (for a profiler), then exit")
     ap.add_argument("--chunk-reps", type=int, default=2)
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[200.0, 1000.0, 20000.0],
-                    help="Poisson arrival rates (requests/s) of the served run")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0],
+                    help="Poisson arrival rates (requests/s) of the served sweep (none: saturated run only)")
     args = ap.parse_args()
 
     if args.gpus != 1:

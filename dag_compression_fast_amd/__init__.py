@@ -22,7 +22,8 @@ from . import _lib
 from ._lib import SvdagError, SVDAG_DEVICE, SVDAG_HOST
 
 __all__ = [
-    "Context", "DeviceBuffer", "Dag", "SvdagError", "create_dag", "create_dag_batched", "merge_dags",
+    "Context", "DeviceBuffer", "DeviceView", "Dag", "ResidentDag", "SvdagError", "create_dag", "create_dag_batched",
+    "create_dag_batched_gather", "merge_dags",
     "create_final_dag", "free_all", "default_context", "morton_encode", "gen_sphere_shells",
     "gen_triangle_surfaces", "gen_value_noise", "gen_chunk_plan", "gen_chunk",
 ]
@@ -67,6 +68,26 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView(DeviceBuffer):
+    """A range of another DeviceBuffer (count elements of dtype from byte
+    offset_bytes); it owns nothing, and freeing it does nothing."""
+
+    def __init__(self, base, offset_bytes, count, dtype=np.uint64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        if offset_bytes < 0 or offset_bytes + self.nbytes > base.nbytes:
+            raise ValueError("view outside its buffer")
+        self.ptr = ctypes.c_void_p(base.address + int(offset_bytes))
+        self.base = base
+
+    def free(self):
+        self.ptr = None
+
+    def __del__(self):
+        pass
 
 
 def _copy_d2h(addr, dtype, count):
@@ -260,6 +281,76 @@ def create_dag_batched(keys, offsets, chunk_depth, ctx=None):
         ctx.handle, buf.ptr, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(offs) - 1, int(chunk_depth),
         ctypes.byref(out)))
     return Dag(out, ctx)
+
+
+def create_dag_batched_gather(chunk_keys, chunk_depth, ctx=None, num_chunks=None):
+    """The batched build with each chunk's keys in its own DeviceBuffer, chunks
+    in Morton order: a list (None for an empty chunk) or a dict {chunk index:
+    DeviceBuffer} with num_chunks chunks in all."""
+    ctx = ctx or default_context()
+    items = chunk_keys.items() if isinstance(chunk_keys, dict) else enumerate(chunk_keys)
+    num = int(num_chunks) if num_chunks is not None else len(chunk_keys)
+    ptrs = np.zeros(num, dtype=np.uint64)
+    counts = np.zeros(num, dtype=np.uint64)
+    for i, b in items:
+        if b is not None and b.count:
+            if np.dtype(b.dtype) != np.uint64:
+                raise SvdagError(1, "batched keys must be uint64")
+            ptrs[i] = b.address
+            counts[i] = b.count
+    out = _lib.SvdagDag()
+    _lib.check(_lib.lib().svdag_create_dag_batched_gather(
+        ctx.handle, ptrs.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p)),
+        counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), num, int(chunk_depth), ctypes.byref(out)))
+    return Dag(out, ctx)
+
+
+class ResidentDag:
+    """A global DAG kept on the device while chunks are added to it
+    (svdag_resident_*): 8^slot_levels chunk slots of side 2^chunk_depth, one
+    persistent dedup table per level, each for min(max_nodes, 8^l) distinct
+    nodes.  ``insert`` adds the chunks of a round
+    (a batched build over the slots) and returns each added chunk's root hash
+    by slot; ``dag`` returns the global DAG in the reference's form (levels in
+    ascending hash order), equal to MergeDAGs over the added chunks."""
+
+    def __init__(self, chunk_depth, slot_levels, max_nodes, ctx=None):
+        self.ctx = ctx or default_context()
+        self.chunk_depth = int(chunk_depth)
+        self.slot_levels = int(slot_levels)
+        self.num_slots = 8 ** self.slot_levels
+        self.handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().svdag_resident_create(self.ctx.handle, self.chunk_depth, self.slot_levels,
+                                                    int(max_nodes), ctypes.byref(self.handle)))
+
+    def insert(self, round_dag):
+        hashes = np.zeros(self.num_slots, dtype=np.uint64)
+        _lib.check(_lib.lib().svdag_resident_insert(self.handle, ctypes.byref(round_dag.raw),
+                                                    hashes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return hashes
+
+    def counts(self):
+        nodes = np.zeros(self.chunk_depth - 1, dtype=np.uint64)
+        chunks = ctypes.c_uint64()
+        _lib.check(_lib.lib().svdag_resident_counts(self.handle, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                    ctypes.byref(chunks)))
+        return nodes, int(chunks.value)
+
+    def dag(self):
+        out = _lib.SvdagDag()
+        _lib.check(_lib.lib().svdag_resident_dag(self.handle, ctypes.byref(out)))
+        return Dag(out, self.ctx)
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            _lib.lib().svdag_resident_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
 
 
 def merge_dags(dags, ctx=None):

@@ -128,6 +128,12 @@ def bind_library(L):
         "svdag_result_bytes": ([u64, i32], u64),
         "svdag_merge_dags": ([vp, dagp, u64, dagp], i32),
         "svdag_create_dag_batched": ([vp, vp, ctypes.POINTER(u64), u64, i32, dagp], i32),
+        "svdag_create_dag_batched_gather": ([vp, ctypes.POINTER(vp), ctypes.POINTER(u64), u64, i32, dagp], i32),
+        "svdag_resident_create": ([vp, i32, i32, u64, ctypes.POINTER(vp)], i32),
+        "svdag_resident_destroy": ([vp], i32),
+        "svdag_resident_insert": ([vp, dagp, ctypes.POINTER(u64)], i32),
+        "svdag_resident_counts": ([vp, ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
+        "svdag_resident_dag": ([vp, dagp], i32),
         "svdag_create_final_dag": ([vp, dagp, ctypes.POINTER(vp), ctypes.POINTER(u64)], i32),
         "svdag_dag_to_host": ([vp, dagp, dagp], i32),
         "svdag_dag_to_device": ([vp, dagp, dagp], i32),

@@ -41,7 +41,7 @@ const char* const kKernelNames[kNumKernelIds] = {
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
     "slot_list",      "query_voxels",    "vox_setup",   "vox_count",         "vox_write",
-    "trace_paths"};
+    "trace_paths",    "resident_insert"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.
@@ -136,6 +136,8 @@ struct svdag_context {
     // Host tile table of the last batched build (kept until its copy is done).
     std::vector<SegTile> seg_tiles;
     std::vector<uint32_t> seg_chunk_tiles;
+    // Source table of the last gathered build (kept until its copy is done).
+    std::vector<GatherSource> gather_src;
     // Result blocks (DAG storage, final words): reused across calls, returned
     // by svdag_free_dag / svdag_free_words, released by svdag_free_all.
     std::multimap<uint64_t, void*> free_blocks;
@@ -820,7 +822,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
         return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed: %s", hipGetErrorString(hipGetLastError())));
     mark(ctx, "counter read-back");
-    TRY(finish(ctx, "create_dag"));
+    // On a stall or an asynchronous error the result block goes back to the
+    // pool (stream order keeps its later reuse safe).
+    if (const int rc = finish(ctx, "create_dag")) return fail(rc);
     const BuildCounters& h = *ctx->h_counters;
     if (!batch && (h.key_or & ~code_mask))
         return fail(set_error(SVDAG_ERR_ARGUMENT, "a key is >= 2^(3*depth) for depth %d", depth));
@@ -1074,7 +1078,7 @@ int merge_device(svdag_context* ctx, const svdag_dag* dags, uint64_t count, int 
     if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
         return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed: %s", hipGetErrorString(hipGetLastError())));
     mark(ctx, "counter read-back");
-    TRY(finish(ctx, "merge_dags"));
+    if (const int rc = finish(ctx, "merge_dags")) return fail(rc);
     const BuildCounters& h = *ctx->h_counters;
     if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
     if (h.error & kErrInternal) return fail(set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow"));
@@ -1312,6 +1316,52 @@ int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys, const u
     BatchInfo b{offsets, num_chunks, chunk_depth};
     const int rc = build<uint64_t>(ctx, d_keys, n, depth, out, &b, ctx->table_estimate);
     return rc == kRetryWithBounds ? build<uint64_t>(ctx, d_keys, n, depth, out, &b, false) : rc;
+}
+
+int svdag_create_dag_batched_gather(svdag_context* ctx, const uint64_t* const* d_chunk_keys, const uint64_t* counts,
+                                    uint64_t num_chunks, int32_t chunk_depth, svdag_dag* out)
+{
+    TRY(check_ctx(ctx));
+    if (!d_chunk_keys || !counts || !out) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    if (num_chunks == 0 || num_chunks > (1ull << 21)) return set_error(SVDAG_ERR_ARGUMENT, "chunk count out of range");
+    std::vector<uint64_t> offsets(num_chunks + 1, 0);
+    for (uint64_t i = 0; i < num_chunks; ++i) {
+        if (counts[i] && !d_chunk_keys[i]) return set_error(SVDAG_ERR_ARGUMENT, "chunk %llu has keys but no pointer", (unsigned long long)i);
+        offsets[i + 1] = offsets[i] + counts[i];
+        if (offsets[i + 1] >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "more than 2^30 - 1 keys");
+    }
+    const uint64_t n = offsets[num_chunks];
+    void* keys = nullptr;
+    if (n) {
+        // The chunks' keys back to back, in chunk order, in a pooled block
+        // (released in stream order once the build has read them).
+        TRY(device_alloc(ctx, &keys, 8 * n));
+        // The host table stays in the context until the next gathered build,
+        // by which time this call's work (and so the copy) has finished.
+        std::vector<GatherSource>& src = ctx->gather_src;
+        src.clear();
+        for (uint64_t i = 0; i < num_chunks; ++i)
+            if (counts[i]) src.push_back(GatherSource{d_chunk_keys[i], offsets[i]});
+        void* table = nullptr;
+        int rc = device_alloc(ctx, &table, sizeof(GatherSource) * src.size());
+        if (rc == 0 && hipMemcpyAsync(table, src.data(), sizeof(GatherSource) * src.size(), hipMemcpyHostToDevice,
+                                      ctx->stream) != hipSuccess)
+            rc = set_error(SVDAG_ERR_HIP, "gather table upload failed");
+        if (rc == 0) {
+            launch_gather_keys(ctx->stream, static_cast<const GatherSource*>(table), uint32_t(src.size()), n,
+                               static_cast<uint64_t*>(keys));
+            if (hipGetLastError() != hipSuccess) rc = set_error(SVDAG_ERR_HIP, "gather launch failed");
+        }
+        device_release(ctx, table);
+        if (rc != 0) {
+            device_release(ctx, keys);
+            return rc;
+        }
+    }
+    const int rc = svdag_create_dag_batched(ctx, static_cast<const uint64_t*>(keys), offsets.data(), num_chunks,
+                                            chunk_depth, out);
+    device_release(ctx, keys);
+    return rc;
 }
 
 int svdag_merge_dags(svdag_context* ctx, const svdag_dag* dags, uint64_t count, svdag_dag* out)
@@ -1814,6 +1864,407 @@ int svdag_profile_read(svdag_context* ctx, int32_t kernel, double* total_ms, uin
 const char* svdag_kernel_name(int32_t kernel)
 {
     return kernel >= 0 && kernel < kNumKernelIds ? kKernelNames[kernel] : nullptr;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Resident DAG (served mode).  The union of every chunk DAG inserted so far,
+// kept on the device between insertions: per level one persistent ordered
+// dedup table (hash -> slot) with the nodes' records stored at their slots
+// (children as slots of the level below), a hashed leaf table, and per chunk
+// slot its root's slot.  A node's identity is its hash, so inserting a round
+// (a batched build of some chunks) adds exactly the nodes the union gains;
+// the reference's MergeDAGs would give the same levels
+// (src/DAGCompression.cu:1109-1320, 1447-1548).  svdag_resident_dag orders the
+// levels by hash (the level extract), ranks the children, and builds the top
+// levels over the filled chunk slots.
+// ---------------------------------------------------------------------------
+struct svdag_resident {
+    svdag_context* ctx = nullptr;
+    int chunk_depth = 0, top = 0, depth = 0, rlevels = 0;  // resident levels: global top .. depth - 3
+    char* mem = nullptr;
+    uint64_t mem_bytes = 0;
+    uint64_t* tkeys[SVDAG_MAX_LEVELS] = {};
+    uint32_t* trep[SVDAG_MAX_LEVELS] = {};
+    uint8_t* rmasks[SVDAG_MAX_LEVELS] = {};
+    uint32_t* rchildren[SVDAG_MAX_LEVELS] = {};
+    uint32_t lg[SVDAG_MAX_LEVELS] = {};
+    uint64_t slots[SVDAG_MAX_LEVELS] = {}, cap[SVDAG_MAX_LEVELS] = {};
+    uint64_t* leaf_keys = nullptr;
+    uint32_t leaf_lg = 0;
+    uint64_t leaf_slots = 0, leaf_cap = 0;
+    uint32_t* root_slot = nullptr;         // [8^top]
+    uint64_t* root_hash = nullptr;         // [8^top]
+    // Device state words: [0, rlevels] placed keys per resident level (leaves
+    // last), then the error bits; then the tables' log2 sizes as u32.
+    unsigned long long* state = nullptr;
+    uint32_t* d_lg = nullptr;              // [rlevels + 1], leaves last
+    uint64_t* h_state = nullptr;           // pinned: state words, then 8^top root hashes
+    uint64_t occupied[SVDAG_MAX_LEVELS + 1] = {};
+    uint64_t chunks = 0;
+    bool broken = false;
+};
+
+namespace {
+
+int resident_check(svdag_resident* r)
+{
+    if (!r) return set_error(SVDAG_ERR_ARGUMENT, "null resident DAG");
+    TRY(check_ctx(r->ctx));
+    if (r->broken) return set_error(SVDAG_ERR_INTERNAL, "resident DAG is unusable after a failed insertion");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int svdag_resident_create(svdag_context* ctx, int32_t chunk_depth, int32_t slot_levels, uint64_t max_nodes,
+                          svdag_resident** out)
+{
+    TRY(check_ctx(ctx));
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    *out = nullptr;
+    if (slot_levels < 1 || slot_levels > kMaxResidentTop)
+        return set_error(SVDAG_ERR_ARGUMENT, "slot levels %d outside [1, %d]", slot_levels, kMaxResidentTop);
+    if (chunk_depth < 3 || chunk_depth + slot_levels > 21)
+        return set_error(SVDAG_ERR_ARGUMENT, "chunk depth %d with %d slot levels out of range", chunk_depth, slot_levels);
+    if (max_nodes == 0 || max_nodes >= (1ull << 31)) return set_error(SVDAG_ERR_ARGUMENT, "max_nodes must be in [1, 2^31)");
+    auto* r = new svdag_resident();
+    r->ctx = ctx;
+    r->chunk_depth = chunk_depth;
+    r->top = slot_levels;
+    r->depth = chunk_depth + slot_levels;
+    r->rlevels = chunk_depth - 2;
+    const uint32_t shift = ctx->load_shift;
+    const uint64_t nslots = 1ull << (3 * r->top);
+    Carver cv;
+    uint64_t o_keys[SVDAG_MAX_LEVELS], o_rep[SVDAG_MAX_LEVELS], o_masks[SVDAG_MAX_LEVELS], o_children[SVDAG_MAX_LEVELS];
+    for (int i = 0; i < r->rlevels; ++i) {
+        r->cap[i] = pow8_capped(r->top + i, max_nodes);
+        r->lg[i] = table_lg(r->cap[i], shift);
+        r->slots[i] = (1ull << r->lg[i]) + kTableOverflow;
+        o_keys[i] = cv.take(8 * r->slots[i]);
+        o_rep[i] = cv.take(4 * r->slots[i]);
+        o_masks[i] = cv.take(r->slots[i]);
+        o_children[i] = cv.take(32 * r->slots[i]);
+    }
+    r->leaf_cap = pow8_capped(r->depth - 2, max_nodes);
+    r->leaf_lg = table_lg(r->leaf_cap, shift);
+    r->leaf_slots = 1ull << r->leaf_lg;
+    const uint64_t o_leaf = cv.take(8 * r->leaf_slots);
+    const uint64_t o_root_slot = cv.take(4 * nslots), o_root_hash = cv.take(8 * nslots);
+    const uint64_t o_state = cv.take(8 * (SVDAG_MAX_LEVELS + 2));
+    const uint64_t o_lg = cv.take(4 * (SVDAG_MAX_LEVELS + 1));
+    r->mem_bytes = cv.off;
+    auto fail = [&](int code) {
+        if (r->h_state) (void)hipHostFree(r->h_state);
+        if (r->mem) (void)hipFree(r->mem);
+        delete r;
+        return code;
+    };
+    if (ctx->budget && ctx->arena_cap + ctx->pool_bytes + r->mem_bytes > ctx->budget)
+        return fail(set_error(SVDAG_ERR_MEMORY, "resident tables of %llu bytes exceed the memory budget",
+                              (unsigned long long)r->mem_bytes));
+    if (hipMalloc(reinterpret_cast<void**>(&r->mem), r->mem_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        r->mem = nullptr;
+        return fail(set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of resident tables",
+                              (unsigned long long)r->mem_bytes));
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&r->h_state), 8 * (SVDAG_MAX_LEVELS + 2 + nslots),
+                      hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        r->h_state = nullptr;
+        return fail(set_error(SVDAG_ERR_HIP, "cannot allocate pinned staging"));
+    }
+    char* M = r->mem;
+    for (int i = 0; i < r->rlevels; ++i) {
+        r->tkeys[i] = reinterpret_cast<uint64_t*>(M + o_keys[i]);
+        r->trep[i] = reinterpret_cast<uint32_t*>(M + o_rep[i]);
+        r->rmasks[i] = reinterpret_cast<uint8_t*>(M + o_masks[i]);
+        r->rchildren[i] = reinterpret_cast<uint32_t*>(M + o_children[i]);
+    }
+    r->leaf_keys = reinterpret_cast<uint64_t*>(M + o_leaf);
+    r->root_slot = reinterpret_cast<uint32_t*>(M + o_root_slot);
+    r->root_hash = reinterpret_cast<uint64_t*>(M + o_root_hash);
+    r->state = reinterpret_cast<unsigned long long*>(M + o_state);
+    r->d_lg = reinterpret_cast<uint32_t*>(M + o_lg);
+    // Empty tables, no chunk slot filled, zero counts; the log2 sizes as
+    // device words (the extract reads them there).
+    hipStream_t st = ctx->stream;
+    for (int i = 0; i < r->rlevels; ++i) launch_fill_u64(st, r->tkeys[i], r->slots[i], 0);
+    launch_fill_u64(st, r->leaf_keys, r->leaf_slots, 0);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(r->root_slot), (nslots + 1) / 2, ~0ull);
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(r->state), SVDAG_MAX_LEVELS + 2, 0);
+    std::vector<uint32_t> lgs(SVDAG_MAX_LEVELS + 1, 0);
+    for (int i = 0; i < r->rlevels; ++i) lgs[i] = r->lg[i];
+    lgs[r->rlevels] = r->leaf_lg;
+    if (hipMemcpyAsync(r->d_lg, lgs.data(), 4 * lgs.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "resident table set-up failed"));
+    if (hipGetLastError() != hipSuccess) return fail(set_error(SVDAG_ERR_HIP, "resident table set-up failed"));
+    *out = r;
+    return 0;
+}
+
+int svdag_resident_destroy(svdag_resident* r)
+{
+    if (!r) return 0;
+    if (r->ctx) {
+        (void)hipSetDevice(r->ctx->device);
+        (void)hipStreamSynchronize(r->ctx->stream);
+    }
+    if (r->mem) (void)hipFree(r->mem);
+    if (r->h_state) (void)hipHostFree(r->h_state);
+    delete r;
+    return 0;
+}
+
+int svdag_resident_insert(svdag_resident* r, const svdag_dag* round, uint64_t* root_hashes)
+{
+    TRY(resident_check(r));
+    svdag_context* ctx = r->ctx;
+    if (!round) return set_error(SVDAG_ERR_ARGUMENT, "null round");
+    if (round->num_levels != r->depth - 2)
+        return set_error(SVDAG_ERR_ARGUMENT, "round has %d levels, the resident DAG %d", round->num_levels,
+                         r->depth - 2);
+    const uint64_t nslots = 1ull << (3 * r->top);
+    if (root_hashes) std::memset(root_hashes, 0, 8 * nslots);
+    if (round->num_leaves == 0) return 0;
+    if (round->location != SVDAG_DEVICE) return set_error(SVDAG_ERR_ARGUMENT, "round must be device-resident");
+    if (round->levels[0].num_nodes != 1) return set_error(SVDAG_ERR_ARGUMENT, "round has no single root");
+    // Admission: the worst case (every node of the round new) must fit each table.
+    for (int i = 0; i < r->rlevels; ++i)
+        if (r->occupied[i] + round->levels[r->top + i].num_nodes > r->cap[i])
+            return set_error(SVDAG_ERR_MEMORY, "resident level %d would exceed its capacity of %llu nodes", r->top + i,
+                             (unsigned long long)r->cap[i]);
+    if (r->occupied[r->rlevels] + round->num_leaves > r->leaf_cap)
+        return set_error(SVDAG_ERR_MEMORY, "resident leaves would exceed their capacity of %llu",
+                         (unsigned long long)r->leaf_cap);
+    uint64_t widest = round->num_leaves;
+    for (int i = 0; i < r->rlevels; ++i) widest = std::max(widest, round->levels[r->top + i].num_nodes);
+    Carver cv;
+    const uint64_t o_map0 = cv.take(4 * widest), o_map1 = cv.take(4 * widest);
+    TRY(ensure_arena(ctx, cv.off));
+    uint32_t* map[2] = {reinterpret_cast<uint32_t*>(ctx->arena + o_map0), reinterpret_cast<uint32_t*>(ctx->arena + o_map1)};
+    hipStream_t st = ctx->stream;
+    unsigned long long* err = r->state + SVDAG_MAX_LEVELS + 1;
+    // Leaves, then levels bottom-up: each round node's resident slot is the
+    // child map of the level above.
+    launch_resident_leaves(st, round->leaves, round->num_leaves, r->leaf_keys, r->leaf_lg, map[0],
+                           r->state + r->rlevels, err);
+    int cur = 0;
+    for (int i = r->rlevels - 1; i >= 0; --i) {
+        const svdag_level& lv = round->levels[r->top + i];
+        ResidentLevelArgs a{};
+        a.hashes = lv.hashes;
+        a.masks = lv.masks;
+        a.children = lv.children;
+        a.count = lv.num_nodes;
+        a.child_slot_of = map[cur];
+        a.table_keys = r->tkeys[i];
+        a.table_rep = r->trep[i];
+        a.lg = r->lg[i];
+        a.rec_masks = r->rmasks[i];
+        a.rec_children = r->rchildren[i];
+        a.slot_of = map[cur ^ 1];
+        a.occupied = r->state + i;
+        a.err = err;
+        launch_resident_level(st, a);
+        cur ^= 1;
+    }
+    ResidentRootArgs ra{};
+    ra.top_levels = r->top;
+    ra.has_root = true;
+    for (int l = 0; l < r->top; ++l) {
+        ra.masks[l] = round->levels[l].masks;
+        ra.children[l] = round->levels[l].children;
+    }
+    ra.root_slot_of = map[cur];
+    ra.root_table_keys = r->tkeys[0];
+    ra.root_slot = r->root_slot;
+    ra.root_hash = r->root_hash;
+    ra.err = err;
+    launch_resident_roots(st, ra);
+    if (hipGetLastError() != hipSuccess) {
+        r->broken = true;
+        return set_error(SVDAG_ERR_HIP, "resident insert launch failed");
+    }
+    if (hipMemcpyAsync(r->h_state, r->state, 8 * (SVDAG_MAX_LEVELS + 2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(r->h_state + SVDAG_MAX_LEVELS + 2, r->root_hash, 8 * nslots, hipMemcpyDeviceToHost, st) !=
+            hipSuccess) {
+        r->broken = true;
+        return set_error(SVDAG_ERR_HIP, "resident state read-back failed");
+    }
+    mark(ctx, "resident insert");
+    if (const int rc = finish(ctx, "resident_insert")) {
+        r->broken = true;
+        return rc;
+    }
+    const uint64_t e = r->h_state[SVDAG_MAX_LEVELS + 1];
+    if (e) {
+        r->broken = true;
+        if (e & kErrSlotFilled) return set_error(SVDAG_ERR_ARGUMENT, "a chunk of the round is at a slot already filled");
+        return set_error(SVDAG_ERR_MEMORY, "resident table probe overflow");
+    }
+    for (int i = 0; i <= r->rlevels; ++i) r->occupied[i] = r->h_state[i];
+    const uint64_t* hashes = r->h_state + SVDAG_MAX_LEVELS + 2;
+    for (uint64_t s = 0; s < nslots; ++s) r->chunks += hashes[s] != 0;
+    if (root_hashes) std::memcpy(root_hashes, hashes, 8 * nslots);
+    return 0;
+}
+
+int svdag_resident_counts(svdag_resident* r, uint64_t* nodes, uint64_t* chunks)
+{
+    if (!r) return set_error(SVDAG_ERR_ARGUMENT, "null resident DAG");
+    if (nodes)
+        for (int i = 0; i <= r->rlevels; ++i) nodes[i] = r->occupied[i];
+    if (chunks) *chunks = r->chunks;
+    return 0;
+}
+
+int svdag_resident_dag(svdag_resident* r, svdag_dag* out)
+{
+    TRY(resident_check(r));
+    if (!out) return set_error(SVDAG_ERR_ARGUMENT, "null output");
+    svdag_context* ctx = r->ctx;
+    const int levels = r->depth - 2;
+    empty_dag(out, levels, SVDAG_DEVICE);
+    if (r->chunks == 0) return 0;
+    hipStream_t st = ctx->stream;
+    const uint64_t nslots = 1ull << (3 * r->top);
+    const uint64_t L = r->occupied[r->rlevels];
+    const SortPlan leaf_plan = make_sort_plan(0, 64);
+    uint64_t max_slots = r->leaf_slots;
+    for (int i = 0; i < r->rlevels; ++i) max_slots = std::max(max_slots, r->slots[i]);
+    const uint64_t stage_tiles = std::max({tiles(max_slots, kExtractTileItems), tiles(r->leaf_slots, kListTileItems),
+                                           uint64_t(1)});
+    Carver cv;
+    const uint64_t o_zero = cv.off;
+    const uint64_t o_counters = cv.take(sizeof(BuildCounters));
+    const uint64_t o_sort_zero = cv.take(4 * sort_zero_words(leaf_plan));
+    const uint64_t zero_end = cv.off;
+    const uint64_t o_sort_counts = cv.take(4 * sort_count_words(L));
+    const uint64_t o_tile_counts = cv.take(4 * stage_tiles), o_tile_offsets = cv.take(8 * stage_tiles);
+    uint64_t o_lk[2], o_ls[2];
+    for (int i = 0; i < 2; ++i) {
+        o_lk[i] = cv.take(8 * L);
+        o_ls[i] = cv.take(4 * L);
+    }
+    const uint64_t o_sorted = cv.take(4 * L), o_leaf_rank = cv.take(4 * r->leaf_slots);
+    const uint64_t o_rank[2] = {cv.take(4 * max_slots), cv.take(4 * max_slots)};
+    const uint64_t o_codes = cv.take(8 * nslots), o_rslots = cv.take(4 * nslots), o_rcount = cv.take(8);
+    uint64_t o_tcodes[2], o_tranks[2];
+    for (int i = 0; i < 2; ++i) {
+        o_tcodes[i] = cv.take(8 * kTailMaxParents);
+        o_tranks[i] = cv.take(4 * kTailMaxParents);
+    }
+    const uint64_t o_tmasks = cv.take(kTailMaxParents), o_tchildren = cv.take(32 * kTailMaxParents);
+    const uint64_t o_thash = cv.take(64 * kTailMaxParents);
+    TRY(ensure_arena(ctx, cv.off));
+    char* A = ctx->arena;
+    BuildCounters* c = reinterpret_cast<BuildCounters*>(A + o_counters);
+    uint32_t* tile_counts = reinterpret_cast<uint32_t*>(A + o_tile_counts);
+    uint64_t* tile_offsets = reinterpret_cast<uint64_t*>(A + o_tile_offsets);
+
+    uint64_t caps[SVDAG_MAX_LEVELS];
+    for (int l = 0; l < levels; ++l) caps[l] = l < r->top ? (1ull << (3 * l)) : r->occupied[l - r->top];
+    const DagStorage S = plan_storage(levels, caps, L);
+    void* storage = nullptr;
+    TRY(device_alloc(ctx, &storage, S.total));
+    char* O = static_cast<char*>(storage);
+    auto fail = [&](int code) {
+        device_release(ctx, storage);
+        return code;
+    };
+    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + o_zero), (zero_end - o_zero) / 8, 0);
+    // Leaves in ascending mask order, and each leaf slot's rank.
+    uint64_t* lk0 = reinterpret_cast<uint64_t*>(A + o_lk[0]);
+    uint64_t* lk1 = reinterpret_cast<uint64_t*>(A + o_lk[1]);
+    uint32_t* ls0 = reinterpret_cast<uint32_t*>(A + o_ls[0]);
+    uint32_t* ls1 = reinterpret_cast<uint32_t*>(A + o_ls[1]);
+    uint32_t* sorted = reinterpret_cast<uint32_t*>(A + o_sorted);
+    uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + o_leaf_rank);
+    uint64_t* leaves_out = reinterpret_cast<uint64_t*>(O + S.leaves);
+    launch_slot_list(st, r->leaf_keys, r->d_lg + r->rlevels, r->leaf_slots, tile_counts, tile_offsets,
+                     &c->table_slots, lk0, ls0, &c->unique_leaves);
+    if (launch_sort<uint64_t>(st, leaf_plan, lk0, ls0, lk1, lk0, ls1, ls0, leaves_out, sorted, &c->unique_leaves, L,
+                              reinterpret_cast<uint32_t*>(A + o_sort_zero),
+                              reinterpret_cast<uint32_t*>(A + o_sort_counts), nullptr) < 0)
+        return fail(set_error(SVDAG_ERR_INTERNAL, "leaf sort plan failed"));
+    launch_rank_of_slot(st, sorted, &c->unique_leaves, leaf_rank, L);
+    // Resident levels bottom-up, each in ascending hash order with its
+    // children's slots turned into ranks of the level below.
+    const uint32_t* below = leaf_rank;
+    int cur = 0;
+    for (int i = r->rlevels - 1; i >= 0; --i, cur ^= 1) {
+        const int l = r->top + i;
+        LevelExtractArgs e{};
+        e.table_keys = r->tkeys[i];
+        e.table_rep = r->trep[i];
+        e.table_log2 = r->d_lg + i;
+        e.rank_of_slot = reinterpret_cast<uint32_t*>(A + o_rank[cur]);
+        e.parent_masks = r->rmasks[i];
+        e.parent_children = r->rchildren[i];
+        e.child_rank_of_slot = below;
+        e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        e.tile_counts = tile_counts;
+        e.offsets = tile_offsets;
+        e.slots_scratch = &c->table_slots;
+        e.q_count = &c->uniq[l];
+        e.w_count = &c->words[l];
+        launch_level_extract(st, e, r->slots[i]);
+        below = e.rank_of_slot;
+    }
+    // Top levels over the filled chunk slots (their codes are the slot indices).
+    uint64_t* codes = reinterpret_cast<uint64_t*>(A + o_codes);
+    uint32_t* rslots = reinterpret_cast<uint32_t*>(A + o_rslots);
+    uint64_t* rcount = reinterpret_cast<uint64_t*>(A + o_rcount);
+    launch_resident_root_list(st, r->root_slot, uint32_t(nslots), codes, rslots, rcount);
+    TailArgs t{};
+    t.top = r->top - 1;
+    t.child_codes = codes;
+    t.child_slots = rslots;
+    t.child_rank_of_slot = below;
+    t.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[r->top]);
+    t.child_count = rcount;
+    for (int i = 0; i < 2; ++i) {
+        t.codes[i] = reinterpret_cast<uint64_t*>(A + o_tcodes[i]);
+        t.ranks[i] = reinterpret_cast<uint32_t*>(A + o_tranks[i]);
+    }
+    t.masks = reinterpret_cast<uint8_t*>(A + o_tmasks);
+    t.children = reinterpret_cast<uint32_t*>(A + o_tchildren);
+    t.child_hash = reinterpret_cast<uint64_t*>(A + o_thash);
+    for (int l = 0; l < r->top; ++l) {
+        t.out_hashes[l] = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        t.out_masks[l] = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        t.out_children[l] = reinterpret_cast<uint32_t*>(O + S.children[l]);
+    }
+    t.counters = c;
+    t.check_collisions = 0;
+    launch_level_tail(st, t);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(SVDAG_ERR_HIP, "resident extract launch failed"));
+    if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed"));
+    mark(ctx, "resident extract");
+    if (const int rc = finish(ctx, "resident_dag")) return fail(rc);
+    const BuildCounters& h = *ctx->h_counters;
+    if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "resident root level holds %llu nodes", (unsigned long long)h.uniq[0]));
+    out->storage = storage;
+    out->leaves = leaves_out;
+    out->num_leaves = h.unique_leaves;
+    for (int l = 0; l < levels; ++l) {
+        out->levels[l].hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
+        out->levels[l].children = reinterpret_cast<uint32_t*>(O + S.children[l]);
+        out->levels[l].masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
+        out->levels[l].num_nodes = h.uniq[l];
+        out->level_words[l] = h.words[l];
+    }
+    return 0;
 }
 
 }  // extern "C"

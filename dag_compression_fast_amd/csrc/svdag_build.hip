@@ -366,7 +366,16 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
 }
 
 // Every leaf position's mask (never 0) enters the table and records its slot.
+// A mask shared by many leaves (all-ones inside a solid, the same few patterns
+// along a flat surface) would otherwise turn into a stream of probes and
+// atomics on one table slot: a CU whose L1 still holds the slot's old 0
+// settles each such probe with a CAS on the same L2 line.  So each wave first
+// elects one lane per distinct mask: every lane writes its lane index into a
+// per-wave LDS bucket chosen by its mask (one write wins), and a lane whose
+// bucket winner holds the same mask takes the winner's slot instead of probing.
+// Lanes that lose their bucket to another mask probe themselves.
 constexpr int kInsertIpt = 4;
+constexpr uint32_t kElectBuckets = 128;
 
 __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __restrict__ lp_mask,
                                                           const uint64_t* __restrict__ count_ptr,
@@ -375,17 +384,35 @@ __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __rest
                                                           uint32_t* __restrict__ lp_slot,
                                                           unsigned long long* __restrict__ err)
 {
+    __shared__ uint32_t s_elect[kThreads / kWave][kElectBuckets];
     const uint64_t n = *count_ptr;
     const uint32_t lg = *log2_ptr;
+    const uint32_t lane = lane_id();
+    uint32_t* const elect = s_elect[threadIdx.x / kWave];
     constexpr uint64_t per_round = uint64_t(kThreads) * kInsertIpt;
-    for (uint64_t r0 = uint64_t(blockIdx.x) * per_round + threadIdx.x; r0 < n; r0 += uint64_t(gridDim.x) * per_round) {
+    // The loop bound is wave-uniform, so every lane of a wave takes part in the
+    // shuffles below.
+    for (uint64_t r0 = uint64_t(blockIdx.x) * per_round + (threadIdx.x & ~uint32_t(kWave - 1)); r0 < n;
+         r0 += uint64_t(gridDim.x) * per_round) {
 #pragma unroll
         for (int j = 0; j < kInsertIpt; ++j) {
-            const uint64_t q = r0 + uint64_t(j) * kThreads;
-            if (q < n) {
+            const uint64_t q = r0 + uint64_t(j) * kThreads + lane;
+            const bool valid = q < n;
+            const uint64_t mask = valid ? lp_mask[q] : 0ull;
+            const uint32_t bucket = uint32_t(fmix64(mask)) & (kElectBuckets - 1);
+            if (valid) elect[bucket] = lane;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t winner = valid ? elect[bucket] : lane;
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t winner_mask = __shfl(mask, int(winner), kWave);
+            const bool follower = valid && winner != lane && winner_mask == mask;
+            uint32_t slot = 0;
+            if (valid && !follower) {
                 bool won;
-                lp_slot[q] = hashed_insert(tk, lg, lp_mask[q], won, err);
+                slot = hashed_insert(tk, lg, mask, won, err);
             }
+            slot = uint32_t(__shfl(int(slot), int(follower ? winner : lane), kWave));
+            if (valid) lp_slot[q] = slot;
         }
     }
 }
@@ -524,11 +551,13 @@ __global__ void k_rank_of_slot(const uint32_t* __restrict__ sorted_slots, const 
         rank_of_slot[sorted_slots[r]] = uint32_t(r);
 }
 
-// Ordered-table insert: home = top lg bits of the hash, linear probing forward
+// Ordered-table probe: home = top lg bits of the hash, linear probing forward
 // into an overflow tail (no wrap), so every key sits at or after its home slot.
-__device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, uint32_t* __restrict__ rep, uint32_t lg,
-                                                   uint64_t limit, uint64_t h, uint32_t p, BuildCounters* c)
+// `won` tells whether this call placed the key.
+__device__ __forceinline__ uint32_t ordered_probe(uint64_t* __restrict__ tk, uint32_t lg, uint64_t limit, uint64_t h,
+                                                  bool& won, unsigned long long* err)
 {
+    won = false;
     uint64_t slot = h >> (64 - lg);
     for (; slot < limit; ++slot) {
         const uint64_t cur = probe_load(&tk[slot]);
@@ -536,14 +565,24 @@ __device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, ui
         if (cur == 0) {
             const uint64_t prev = atomicCAS((unsigned long long*)&tk[slot], 0ull, (unsigned long long)h);
             if (prev == 0) {
-                rep[slot] = p;
+                won = true;
                 return uint32_t(slot);
             }
             if (prev == h) return uint32_t(slot);
         }
     }
-    atomicOr((unsigned long long*)&c->error, (unsigned long long)kErrTableOverflow);
+    atomicOr(err, (unsigned long long)kErrTableOverflow);
     return uint32_t(limit - 1);
+}
+
+// Ordered-table insert; the placing call records p as the key's representative.
+__device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, uint32_t* __restrict__ rep, uint32_t lg,
+                                                   uint64_t limit, uint64_t h, uint32_t p, BuildCounters* c)
+{
+    bool won;
+    const uint32_t slot = ordered_probe(tk, lg, limit, h, won, (unsigned long long*)&c->error);
+    if (won) rep[slot] = p;
+    return slot;
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,6 +1223,135 @@ __global__ void k_merge_insert(const MergeSource* __restrict__ src, uint32_t nsr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Resident DAG (served mode): one dedup table per level that persists across
+// insertions.  A resident node is identified by its table slot; its record
+// (mask, children as slots of the level below) is stored at that slot by the
+// call that placed its key, and rep[slot] = slot, so the level extract reads
+// a resident table like a build's.  Wave-aggregated counts of placed keys
+// track each table's occupancy.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void count_placed(bool won, unsigned long long* occupied)
+{
+    const uint64_t wins = __ballot(won);
+    if (wins && lane_id() == uint32_t(__ffsll((long long)wins) - 1))
+        atomicAdd(occupied, (unsigned long long)__popcll(wins));
+}
+
+// Leaves of a round (unique masks) into the resident leaf table.
+__global__ void k_resident_leaves(const uint64_t* __restrict__ leaves, uint64_t count, uint64_t* __restrict__ tk,
+                                  uint32_t lg, uint32_t* __restrict__ slot_of, unsigned long long* __restrict__ occupied,
+                                  unsigned long long* __restrict__ err)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    // Wave-uniform trip count (count_placed is a wave ballot).
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < count; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        bool won = false;
+        if (i < count) slot_of[i] = hashed_insert(tk, lg, leaves[i], won, err);
+        count_placed(won, occupied);
+    }
+}
+
+// Nodes of one round level into the resident table of that level; children
+// (round ranks of the level below) become resident slots through child_slot_of.
+__global__ void k_resident_level(ResidentLevelArgs a)
+{
+    const uint64_t limit = (1ull << a.lg) + kTableOverflow;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < a.count; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        bool won = false;
+        if (i < a.count) {
+            const uint32_t slot = ordered_probe(a.table_keys, a.lg, limit, a.hashes[i], won, a.err);
+            if (won) {
+                const uint4* cs = reinterpret_cast<const uint4*>(a.children + 8 * i);
+                const uint4 c0 = cs[0], c1 = cs[1];
+                uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (ch[k] != kEmptyChild) ch[k] = a.child_slot_of[ch[k]];
+                uint4* dst = reinterpret_cast<uint4*>(a.rec_children + 8 * uint64_t(slot));
+                dst[0] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
+                dst[1] = make_uint4(ch[4], ch[5], ch[6], ch[7]);
+                a.rec_masks[slot] = a.masks[i];
+                a.table_rep[slot] = slot;
+            }
+            a.slot_of[i] = slot;
+        }
+        count_placed(won, a.occupied);
+    }
+}
+
+// One thread per chunk slot s: walks the round's top levels along s's child
+// indices; a chunk present in the round gets its root's resident slot and hash.
+// A chunk at a slot that already holds one sets kErrSlotFilled.
+constexpr int kRootThreads = 256;
+
+__global__ __launch_bounds__(kRootThreads) void k_resident_roots(ResidentRootArgs a)
+{
+    const uint32_t slots = 1u << (3 * a.top_levels);
+    for (uint32_t s = threadIdx.x; s < slots; s += kRootThreads) {
+        uint64_t node = 0;
+        bool present = a.has_root;
+        for (int l = 0; l < a.top_levels && present; ++l) {
+            const uint32_t b = (s >> (3 * (a.top_levels - 1 - l))) & 7u;
+            if (!((a.masks[l][node] >> b) & 1u)) present = false;
+            else node = a.children[l][8 * node + b];
+        }
+        uint64_t hash = 0;
+        if (present) {
+            const uint32_t slot = a.root_slot_of[node];
+            if (a.root_slot[s] != kEmptyChild) {
+                atomicOr(a.err, (unsigned long long)kErrSlotFilled);
+            } else {
+                a.root_slot[s] = slot;
+                hash = a.root_table_keys[slot];
+            }
+        }
+        a.root_hash[s] = hash;
+    }
+}
+
+// Keys of several device arrays, back to back: thread i finds its source by a
+// binary search over the destination starts.
+__global__ void k_gather_keys(const GatherSource* __restrict__ src, uint32_t nsrc, uint64_t total,
+                              uint64_t* __restrict__ dst)
+{
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        uint32_t lo = 0, hi = nsrc;  // src[lo].start <= i < src[hi].start
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (src[mid].start <= i) lo = mid;
+            else hi = mid;
+        }
+        dst[i] = __builtin_nontemporal_load(src[lo].keys + (i - src[lo].start));
+    }
+}
+
+// The filled chunk slots in slot (Morton) order: codes, root slots, count.
+__global__ __launch_bounds__(kRootThreads) void k_resident_root_list(const uint32_t* __restrict__ root_slot,
+                                                                     uint32_t slots, uint64_t* __restrict__ codes,
+                                                                     uint32_t* __restrict__ list_slots,
+                                                                     uint64_t* __restrict__ count)
+{
+    __shared__ uint32_t s_scan[kRootThreads / kWave];
+    uint32_t placed = 0;
+    for (uint32_t base = 0; base < slots; base += kRootThreads) {
+        const uint32_t s = base + threadIdx.x;
+        const bool filled = s < slots && root_slot[s] != kEmptyChild;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kRootThreads>(filled ? 1u : 0u, s_scan, total);
+        if (filled) {
+            codes[placed + ex] = s;
+            list_slots[placed + ex] = root_slot[s];
+        }
+        placed += total;
+    }
+    if (threadIdx.x == 0) *count = placed;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1410,6 +1578,45 @@ void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes
         TimedLaunch t(kKFinalWrite, s);
         hipLaunchKernelGGL(k_final_write, dim3(grid_for((work + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, a);
     }
+}
+
+void launch_resident_leaves(hipStream_t s, const uint64_t* leaves, uint64_t count, uint64_t* table_keys, uint32_t lg,
+                            uint32_t* slot_of, unsigned long long* occupied, unsigned long long* err)
+{
+    if (count == 0) return;
+    TimedLaunch t(kKResidentInsert, s);
+    hipLaunchKernelGGL(k_resident_leaves, dim3(grid_for((count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       leaves, count, table_keys, lg, slot_of, occupied, err);
+}
+
+void launch_resident_level(hipStream_t s, const ResidentLevelArgs& a)
+{
+    if (a.count == 0) return;
+    TimedLaunch t(kKResidentInsert, s);
+    hipLaunchKernelGGL(k_resident_level, dim3(grid_for((a.count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
+                       a);
+}
+
+void launch_gather_keys(hipStream_t s, const GatherSource* src, uint32_t nsrc, uint64_t total, uint64_t* dst)
+{
+    if (total == 0 || nsrc == 0) return;
+    TimedLaunch t(kKFill, s);
+    hipLaunchKernelGGL(k_gather_keys, dim3(grid_for((total + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, src,
+                       nsrc, total, dst);
+}
+
+void launch_resident_roots(hipStream_t s, const ResidentRootArgs& a)
+{
+    TimedLaunch t(kKResidentInsert, s);
+    hipLaunchKernelGGL(k_resident_roots, dim3(1), dim3(kRootThreads), 0, s, a);
+}
+
+void launch_resident_root_list(hipStream_t s, const uint32_t* root_slot, uint32_t slots, uint64_t* codes,
+                               uint32_t* list_slots, uint64_t* count)
+{
+    TimedLaunch t(kKResidentInsert, s);
+    hipLaunchKernelGGL(k_resident_root_list, dim3(1), dim3(kRootThreads), 0, s, root_slot, slots, codes, list_slots,
+                       count);
 }
 
 void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t num_sources, uint64_t total,

@@ -46,6 +46,7 @@ enum KernelId : int {
     kKVoxCount,
     kKVoxWrite,
     kKTracePaths,
+    kKResidentInsert,
     kNumKernelIds
 };
 
@@ -180,6 +181,7 @@ struct BuildCounters {
 constexpr uint64_t kErrTableOverflow = 1;
 constexpr uint64_t kErrKeyRange = 2;
 constexpr uint64_t kErrInternal = 4;  // a bound the kernels rely on did not hold
+constexpr uint64_t kErrSlotFilled = 8;  // a resident chunk slot received a second chunk
 
 // Tile sizes of the compaction stages; a stage over at most n items needs
 // tiles(n, kXxxTile) u32 counts and u64 offsets of scratch.
@@ -425,5 +427,54 @@ void launch_merge_insert(hipStream_t s, const MergeSource* d_sources, uint32_t n
                          uint32_t* children_out, uint64_t* table_keys, uint32_t* table_rep, const uint32_t* table_log2,
                          uint32_t* slot_of, BuildCounters* counters, int ordered, uint64_t* list_keys,
                          uint32_t* list_slots);
+
+// ---- gathered chunks ------------------------------------------------------
+// Source i's keys go to dst[start, start of source i + 1) (the last source up
+// to `total`); starts strictly increase.
+struct GatherSource {
+    const uint64_t* keys;
+    uint64_t start;
+};
+void launch_gather_keys(hipStream_t s, const GatherSource* src, uint32_t nsrc, uint64_t total, uint64_t* dst);
+
+// ---- resident DAG (served mode) ------------------------------------------
+// Nodes of one round level into the persistent table of that level.
+struct ResidentLevelArgs {
+    const uint64_t* hashes;       // the round level (ascending hash order)
+    const uint8_t* masks;
+    const uint32_t* children;     // round ranks of the level below
+    uint64_t count;
+    const uint32_t* child_slot_of;  // round rank of the level below -> resident slot
+    uint64_t* table_keys;         // resident ordered table (2^lg + kTableOverflow slots)
+    uint32_t* table_rep;
+    uint32_t lg;
+    uint8_t* rec_masks;           // slot-indexed records
+    uint32_t* rec_children;
+    uint32_t* slot_of;            // out: resident slot of each round node
+    unsigned long long* occupied; // += keys placed
+    unsigned long long* err;
+};
+void launch_resident_level(hipStream_t s, const ResidentLevelArgs& a);
+void launch_resident_leaves(hipStream_t s, const uint64_t* leaves, uint64_t count, uint64_t* table_keys, uint32_t lg,
+                            uint32_t* slot_of, unsigned long long* occupied, unsigned long long* err);
+
+// Chunk roots of a round: its levels 0 .. top_levels - 1 lead to the chunk
+// slots (8^top_levels of them, Morton order).
+constexpr int kMaxResidentTop = kTailTopLevel + 1;
+struct ResidentRootArgs {
+    int top_levels;
+    bool has_root;                               // the round has a root node
+    const uint8_t* masks[kMaxResidentTop];       // round levels 0 .. top_levels - 1
+    const uint32_t* children[kMaxResidentTop];
+    const uint32_t* root_slot_of;                // round rank at level top_levels -> resident slot
+    const uint64_t* root_table_keys;             // resident table of level top_levels
+    uint32_t* root_slot;                         // per chunk slot: resident root slot (kEmptyChild: none)
+    uint64_t* root_hash;                         // out: per chunk slot, its root's hash if added now, else 0
+    unsigned long long* err;
+};
+void launch_resident_roots(hipStream_t s, const ResidentRootArgs& a);
+// Filled chunk slots in slot order: codes[i] = slot index, list_slots[i] = root slot, *count.
+void launch_resident_root_list(hipStream_t s, const uint32_t* root_slot, uint32_t slots, uint64_t* codes,
+                               uint32_t* list_slots, uint64_t* count);
 
 }  // namespace svdag

@@ -1,162 +1,145 @@
-"""Served mode: chunk-build requests with continuous admission.
+"""Served mode: chunk-compression requests with continuous batching into a
+global DAG resident on the device.
 
-Requests are chunks of voxel keys (host arrays, chunk-local Morton codes) that
-arrive over time, each for a distinct sub-volume of a 8^k chunk grid (the
-reference's per-chunk loop, src/DAG_Compression_Fast.cpp:266-330, fed by a
-request stream).
+Each request is one chunk (a 2^chunk_depth sub-volume at its own slot of an
+8^slot_levels grid of slots) given as unsorted chunk-local Morton keys, either
+already in device memory (a DeviceBuffer, as the reference's voxelizer leaves
+its fragments in a GPU buffer, src/DAG_Compression_Fast.cpp:283-302) or as a
+host array uploaded on admission.  The server replays the arrivals against
+the wall clock:
 
-- **Slots.** Each slot is a host thread with its own build context on its own
-  HIP stream and its own device key buffer. A free slot takes the oldest
-  arrived request, so the device work of admitted requests overlaps.
-- **Admission.** A request is admitted only while the device bytes of the
-  requests in flight (keys + build scratch, from svdag_scratch_bytes) stay
-  within the HBM budget.
-- **Results.** Each request's result is its chunk DAG, kept on the device; its
-  latency runs from arrival to that DAG being complete (upload included). The
-  global DAG is the merge of all chunk DAGs (DAGCompression::MergeDAGs), which
-  equals the batched build of the same chunks.
+- **Continuous batching.**  Whenever the device is free, every request that
+  has arrived and fits the HBM budget joins the next round; requests that
+  arrive while a round is on the device wait for the following one.  A round
+  is one batched build over the slots of its requests (their work runs in the
+  same kernels), then one insertion into the resident DAG.
+- **Admission by HBM footprint.**  A round's footprint is its keys, the build's
+  scratch for them (svdag_scratch_bytes) and its result; requests are admitted
+  in arrival order while the round stays within the budget.  The resident DAG's
+  tables are sized once (max_nodes distinct nodes per level); a round that
+  could overflow them is refused before any device work.
+- **Resident state.**  The global DAG and its per-level dedup tables stay on
+  the device between rounds (svdag_resident_*); each round adds exactly the
+  nodes the union gains.  ``dag()`` returns the global DAG, equal to the
+  reference's MergeDAGs over every served chunk (src/DAGCompression.cu:1447-1548).
+- **Results.**  A request's result is its chunk's root hash in the global DAG
+  (the content identity of its chunk DAG: the root hash of the chunk built
+  alone), and its latency runs from arrival to the end of its round's
+  insertion (upload included for host keys).
 """
 import collections
-import ctypes
-import threading
 import time
 
 import numpy as np
 
 from . import _lib
-from . import Context, Dag, DeviceBuffer
+from . import Context, DeviceBuffer, ResidentDag, create_dag_batched_gather
 
-Request = collections.namedtuple("Request", "index chunk keys arrival")
+Request = collections.namedtuple("Request", "index slot keys arrival")
 
 
 class Result:
-    __slots__ = ("index", "chunk", "dag", "n", "arrival", "start", "done", "slot")
+    __slots__ = ("index", "slot", "root_hash", "n", "arrival", "start", "done", "round")
 
-    def __init__(self, req, dag, start, done, slot):
-        self.index, self.chunk, self.n, self.arrival = req.index, req.chunk, len(req.keys), req.arrival
-        self.dag, self.start, self.done, self.slot = dag, start, done, slot
+    def __init__(self, req, n, root_hash, start, done, rnd):
+        self.index, self.slot, self.n, self.arrival = req.index, req.slot, n, req.arrival
+        self.root_hash, self.start, self.done, self.round = root_hash, start, done, rnd
 
     @property
     def latency(self):
         return self.done - self.arrival
 
 
-class ChunkServer:
-    """Continuous-admission server for chunk builds (see the module docstring)."""
+def _count(keys):
+    return int(keys.count) if isinstance(keys, DeviceBuffer) else len(keys)
 
-    def __init__(self, chunk_depth, slots=4, budget_bytes=None, max_chunk_keys=1 << 22):
+
+class ChunkServer:
+    """Continuous-batching server over a resident global DAG (module docstring)."""
+
+    def __init__(self, chunk_depth, slot_levels, max_nodes, budget_bytes=None, ctx=None, clock=time.perf_counter):
         self.chunk_depth = int(chunk_depth)
-        self.slots = int(slots)
-        self.max_chunk_keys = int(max_chunk_keys)
-        per_slot = self.footprint(self.max_chunk_keys)
-        self.budget = int(budget_bytes) if budget_bytes else per_slot * self.slots
-        H = _lib.hip()
-        self._streams = []
-        self._ctx = []
-        self._keys = []
-        for _ in range(self.slots):
-            s = ctypes.c_void_p()
-            _lib.hip_check(H.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
-            self._streams.append(s)
-            self._ctx.append(Context(s))
-            self._keys.append(DeviceBuffer(8 * self.max_chunk_keys))
+        self.slot_levels = int(slot_levels)
+        self.num_slots = 8 ** self.slot_levels
+        self.depth = self.chunk_depth + self.slot_levels
+        self.ctx = ctx or Context()
+        self._own_ctx = ctx is None
+        self.clock = clock
+        self.resident = ResidentDag(self.chunk_depth, self.slot_levels, max_nodes, ctx=self.ctx)
+        self.budget = int(budget_bytes) if budget_bytes else self.footprint(1 << 26)
+        self.rounds = 0
 
     def footprint(self, n):
-        """Device bytes one request holds while in flight: its keys and build scratch."""
-        return 8 * int(n) + int(_lib.lib().svdag_scratch_bytes(int(n), self.chunk_depth, 8))
+        """Device bytes a round of n keys holds while it runs: its keys (and
+        their gathered copy), the build's scratch and its result."""
+        L = _lib.lib()
+        n = int(n)
+        return 16 * n + int(L.svdag_scratch_bytes(n, self.depth, 8)) + int(L.svdag_result_bytes(n, self.depth))
+
+    def _round(self, batch):
+        """One round: the batched build of the requests' chunks at their slots,
+        then its insertion into the resident DAG.  Returns the root hashes."""
+        bufs = {}
+        staged = []
+        try:
+            for r in batch:
+                if isinstance(r.keys, DeviceBuffer):
+                    bufs[r.slot] = r.keys
+                elif len(r.keys):
+                    b = DeviceBuffer.from_numpy(np.ascontiguousarray(r.keys, dtype=np.uint64))
+                    staged.append(b)
+                    bufs[r.slot] = b
+            rd = create_dag_batched_gather(bufs, self.chunk_depth, ctx=self.ctx, num_chunks=self.num_slots)
+        finally:
+            for b in staged:
+                b.free()
+        try:
+            return self.resident.insert(rd)
+        finally:
+            rd.free()
 
     def run(self, requests):
-        """Serves `requests` (sorted by arrival, seconds from the call) and returns
-        the results in request order. Arrivals are replayed against the wall clock."""
+        """Serves `requests` (arrival times in seconds from the call) and returns
+        the results in request order."""
+        seen = set()
         for r in requests:
-            if len(r.keys) > self.max_chunk_keys:
-                raise ValueError(f"chunk of {len(r.keys)} keys exceeds the slot capacity {self.max_chunk_keys}")
-        lock = threading.Condition()
-        pending = collections.deque(requests)
-        state = {"in_flight": 0, "error": None}
+            if not 0 <= r.slot < self.num_slots or r.slot in seen:
+                raise ValueError(f"request {r.index}: slot {r.slot} out of range or used twice")
+            seen.add(r.slot)
+            if self.footprint(_count(r.keys)) > self.budget:
+                raise ValueError(f"request {r.index} ({_count(r.keys)} keys) exceeds the HBM budget")
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         results = [None] * len(requests)
-        t0 = time.perf_counter()
-        L, H = _lib.lib(), _lib.hip()
-
-        def take():
-            with lock:
-                while True:
-                    if state["error"] or not pending:
-                        return None
-                    head = pending[0]
-                    now = time.perf_counter() - t0
-                    if head.arrival > now:
-                        lock.wait(timeout=head.arrival - now)
-                        continue
-                    need = self.footprint(len(head.keys))
-                    if state["in_flight"] and state["in_flight"] + need > self.budget:
-                        lock.wait()
-                        continue
-                    pending.popleft()
-                    state["in_flight"] += need
-                    return head, need
-
-        def slot_loop(slot):
-            ctx, buf, stream = self._ctx[slot], self._keys[slot], self._streams[slot]
-            while True:
-                item = take()
-                if item is None:
-                    return
-                req, need = item
-                try:
-                    start = time.perf_counter() - t0
-                    keys = np.ascontiguousarray(req.keys, dtype=np.uint64)
-                    if keys.size:
-                        _lib.hip_check(H.hipMemcpyAsync(buf.ptr, keys.ctypes.data, keys.nbytes, _lib.HIP_H2D,
-                                                        stream), "hipMemcpyAsync")
-                    raw = _lib.SvdagDag()
-                    _lib.check(L.svdag_create_dag(ctx.handle, buf.ptr, keys.size, self.chunk_depth,
-                                                  ctypes.byref(raw)))
-                    done = time.perf_counter() - t0
-                    results[req.index] = Result(req, Dag(raw, ctx), start, done, slot)
-                except Exception as e:  # noqa: BLE001 -- reported by run()
-                    with lock:
-                        state["error"] = e
-                finally:
-                    with lock:
-                        state["in_flight"] -= need
-                        lock.notify_all()
-
-        threads = [threading.Thread(target=slot_loop, args=(i,), daemon=True) for i in range(self.slots)]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
-        if state["error"] is not None:
-            raise state["error"]
+        t0 = self.clock()
+        while pending:
+            now = self.clock() - t0
+            if pending[0].arrival > now:
+                # Nothing has arrived: wait for the next arrival.
+                time.sleep(min(pending[0].arrival - now, 0.0005))
+                continue
+            batch, keys = [], 0
+            while pending and pending[0].arrival <= now:
+                n = _count(pending[0].keys)
+                if batch and self.footprint(keys + n) > self.budget:
+                    break
+                batch.append(pending.popleft())
+                keys += n
+            start = self.clock() - t0
+            hashes = self._round(batch)
+            done = self.clock() - t0
+            for r in batch:
+                results[r.index] = Result(r, _count(r.keys), int(hashes[r.slot]), start, done, self.rounds)
+            self.rounds += 1
         return results
 
-    def merge(self, results, num_chunks):
-        """Global DAG of the served chunks (slots without a request are empty)."""
-        empty = _lib.SvdagDag()
-        empty.num_levels = self.chunk_depth - 2
-        empty.location = _lib.SVDAG_DEVICE
-        arr = (_lib.SvdagDag * num_chunks)()
-        for i in range(num_chunks):
-            arr[i] = empty
-        for r in results:
-            arr[r.chunk] = r.dag.raw
-        out = _lib.SvdagDag()
-        ctx = self._ctx[0]
-        for s in self._streams:
-            _lib.hip_check(_lib.hip().hipStreamSynchronize(s), "hipStreamSynchronize")
-        _lib.check(_lib.lib().svdag_merge_dags(ctx.handle, arr, num_chunks, ctypes.byref(out)))
-        return Dag(out, ctx)
+    def dag(self):
+        """The global DAG of every chunk served so far (levels in hash order)."""
+        return self.resident.dag()
 
     def close(self):
-        for r in self._keys:
-            r.free()
-        for c in self._ctx:
-            c.close()
-        H = _lib.hip()
-        for s in self._streams:
-            H.hipStreamDestroy(s)
-        self._streams, self._ctx, self._keys = [], [], []
+        self.resident.close()
+        if self._own_ctx:
+            self.ctx.close()
 
 
 def poisson_arrivals(rate, count, seed):

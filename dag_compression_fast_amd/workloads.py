@@ -13,7 +13,9 @@ library's generators (csrc/svdag_gen.cpp):
   (the reference's many-chunk flow); noise512 is one 512^3 cube of it with
   32-bit keys (depth 9);
 - chunks512: 512 sub-volumes of 512^3 (chunk depth 9, global depth 12) with
-  lognormal key counts, sphere-shell or triangle-surface content.
+  lognormal key counts, sphere-shell or triangle-surface content;
+- served: 2000 such chunks as requests at distinct slots of a 4096-slot grid
+  (global depth 13) with Poisson arrivals.
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -24,6 +26,11 @@ from . import gen_chunk, gen_chunk_plan, gen_value_noise
 CHUNKS = dict(desc="512 sub-volumes of 512^3 (chunk depth 9 -> depth 12), lognormal keys per chunk "
                    "(median 2e5, sigma 1, 1e3..4e6), sphere-shell or triangle-surface content, 20-30% duplicates",
               seed=0x5EED0003, num=512, depth=9, levels_above=3)
+
+SERVED = dict(desc="served 512^3 chunk requests (chunk depth 9) at distinct slots of a 16^3-slot grid (4096 slots, "
+                   "depth 13), key counts and content drawn as in the 512-chunk workload, Poisson arrivals",
+              seed=0x5EED0004, requests=2000, depth=9, slot_levels=4,
+              rates=(50.0, 100.0, 200.0, 500.0, 1000.0, 2000.0), min_seconds=30.0)
 
 NOISE = dict(desc="value noise 1024^3 (depth 10): lattice spacing 8, trilinear at voxel centres >= 0.5 in fixed "
                   "point, 8 Morton-aligned 512^3 batches (shuffled within a batch), each built at depth 9 and "

@@ -164,6 +164,16 @@ int svdag_create_dag_batched(svdag_context* ctx, const uint64_t* d_keys,
                              const uint64_t* offsets, uint64_t num_chunks,
                              int32_t chunk_depth, svdag_dag* out);
 
+/* The same batched build with each chunk's keys in its own device array (the
+ * reference's per-chunk flow, where every chunk's fragments sit in the
+ * voxelizer's GPU buffer, src/DAG_Compression_Fast.cpp:283-302):
+ * d_chunk_keys[i] (host array of device pointers) holds counts[i] keys of
+ * chunk i; chunks with no keys may have a null pointer.  The keys are gathered
+ * back to back on the device first (8 bytes per key of extra memory). */
+int svdag_create_dag_batched_gather(svdag_context* ctx, const uint64_t* const* d_chunk_keys,
+                                    const uint64_t* counts, uint64_t num_chunks,
+                                    int32_t chunk_depth, svdag_dag* out);
+
 /* DAGCompression::CreateFinalDAG (src/DAGCompression.cu:1550-1610): the flat
  * uint32 layout.  *d_words is a device allocation (svdag_free_words). */
 int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag,
@@ -180,6 +190,41 @@ int svdag_free_words(svdag_context* ctx, uint32_t* d_words);
 /* DAGCompression::FreeAll (src/DAGCompression.cu:1612-1615): release the
  * context's scratch arena. */
 int svdag_free_all(svdag_context* ctx);
+
+/* --- resident global DAG (served mode) ----------------------------------- */
+
+/* No single counterpart in the reference: the global DAG of the reference's
+ * chunk loop (src/DAG_Compression_Fast.cpp:283-320, merged by
+ * DAGCompression::MergeDAGs, src/DAGCompression.cu:1109-1320, 1447-1548), kept
+ * on the device while chunks are added to it.  The grid holds 8^slot_levels
+ * chunk slots (Morton order) of side 2^chunk_depth; the global depth is
+ * chunk_depth + slot_levels (1 <= slot_levels <= 4).  Each level keeps one
+ * dedup table for the object's life, sized for min(max_nodes, 8^l) distinct
+ * nodes at global level l (leaves: min(max_nodes, 8^(depth-2))); a level can
+ * never hold more nodes than the keys of all chunks added, and holds far fewer
+ * on surface-like data (about 1.5% of the keys at the 512-chunk workload).
+ * Device memory: about 90 bytes per node of capacity per level.  Not
+ * thread-safe; it runs on its context's stream. */
+typedef struct svdag_resident svdag_resident;
+int svdag_resident_create(svdag_context* ctx, int32_t chunk_depth, int32_t slot_levels,
+                          uint64_t max_nodes, svdag_resident** out);
+int svdag_resident_destroy(svdag_resident* r);
+/* Adds the chunks of `round`, a device DAG of depth chunk_depth + slot_levels
+ * (as svdag_create_dag_batched returns for the 8^slot_levels chunk slots, empty
+ * ones allowed).  Every chunk of the round must be at a slot not filled yet.
+ * root_hashes (host, 8^slot_levels entries, or NULL) receives, per slot, the
+ * root hash of the chunk this call added there (its content identity), else 0.
+ * A round whose nodes could overflow a table fails with SVDAG_ERR_MEMORY before
+ * any device work; a failure after device work has started leaves the object
+ * unusable. */
+int svdag_resident_insert(svdag_resident* r, const svdag_dag* round, uint64_t* root_hashes);
+/* Nodes per resident level (global levels slot_levels .. depth-3, then the
+ * leaves: chunk_depth - 1 entries) and the number of chunks added. */
+int svdag_resident_counts(svdag_resident* r, uint64_t* nodes, uint64_t* chunks);
+/* The global DAG of every chunk added so far, in the reference's form (each
+ * level in ascending hash order): a new device DAG owned by the caller, equal
+ * to svdag_merge_dags of the chunks' DAGs (empty slots empty). */
+int svdag_resident_dag(svdag_resident* r, svdag_dag* out);
 
 /* Message of the last failing call on this host thread. */
 const char* svdag_last_error(void);

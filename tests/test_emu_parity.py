@@ -77,6 +77,12 @@ def test_emu_long_runs_in_sorted_bits(emu, oracle):
     build_and_compare(emu, oracle, keys, 12)
 
 
+def test_emu_solid_regions(emu, oracle):
+    """Long runs of all-ones leaves: the leaf insert probes once per run."""
+    import test_gpu_parity as T
+    build_and_compare(emu, oracle, T.solid_keys(np.random.default_rng(6), 7, 2000), 7)
+
+
 def test_emu_u32(emu, oracle):
     rng = np.random.default_rng(4)
     keys = rng.integers(0, 8 ** 9, size=20000, dtype=np.uint64)

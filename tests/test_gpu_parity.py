@@ -240,3 +240,30 @@ def test_batched_segmented_digit_split(gpu, oracle, first_bits):
         batched.free()
     finally:
         ctx.close()
+
+
+def solid_keys(rng, depth, n_random):
+    """A solid box (runs of all-ones leaves in Morton order) with a cavity and
+    scattered voxels around it (distinct masks between the runs)."""
+    side = 1 << depth
+    lo, hi = side // 8, side // 8 + side // 4
+    ax = np.arange(lo, hi, dtype=np.uint64)
+    x, y, z = np.meshgrid(ax, ax, ax, indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    c = np.uint64(lo + side // 8)
+    keep = (x - c) * (x - c) + (y - c) * (y - c) + (z - c) * (z - c) > np.uint64((side // 16) ** 2)
+    x, y, z = x[keep], y[keep], z[keep]
+    code = np.zeros(len(x), dtype=np.uint64)
+    for b in range(depth):
+        for a, v in enumerate((x, y, z)):
+            code |= ((v >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + 2 - a)
+    extra = rng.integers(0, 1 << (3 * depth), size=n_random, dtype=np.uint64)
+    keys = np.concatenate([code, extra, code[: len(code) // 5]])
+    rng.shuffle(keys)
+    return keys
+
+
+@pytest.mark.parametrize("depth", [7, 9])
+def test_solid_regions(gpu, oracle, depth):
+    rng = np.random.default_rng(500 + depth)
+    check_case(gpu, oracle, solid_keys(rng, depth, 3000), depth, f"solid d{depth}")
