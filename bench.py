@@ -354,6 +354,23 @@ def bench_chunks(svd, _lib, args):
             f"{tuned['worst_ms']:.2f} ms) over {len(res['table'])} settings in {tuned['tune_s']:.1f} s")
     t_batch, w_batch = timed(batched, args.chunk_reps)
     log(f"chunks batched: {t_batch * 1e3:.2f} ms")
+    # The same build captured once as a HIP graph and replayed (offsets in
+    # device memory, no host planning per replay).
+    d_off = svd.DeviceBuffer.from_numpy(offsets)
+    graph = svd.BatchedGraph(d_keys, d_off, len(chunks), depth, total, ctx=ctx)
+
+    def replay():
+        graph.launch()
+        d = graph.result()
+        raw, d.raw = d.raw, None  # the arrays live in the graph; nothing to free
+        return raw
+
+    try:
+        t_graph, w_graph = timed(replay, args.chunk_reps)
+    finally:
+        graph.close()
+        d_off.free()
+    log(f"chunks batched, graph replay: {t_graph * 1e3:.2f} ms")
     if args.kernel_table:
         ctx.profile_enable(None)
         ctx.profile_reset()
@@ -362,7 +379,7 @@ def bench_chunks(svd, _lib, args):
         ctx.profile_enable([])
         for k, (ms, cnt) in sorted(table.items(), key=lambda kv: -kv[1][0])[:12]:
             log(f"  batched {k:20s} {ms:8.3f} ms  {cnt:4d} launches")
-    same = np.array_equal(w_plain, w_batch) and np.array_equal(w_union, w_batch)
+    same = np.array_equal(w_plain, w_batch) and np.array_equal(w_union, w_batch) and np.array_equal(w_graph, w_batch)
 
     served = bench_served(svd, _lib, args, d_keys, offsets, w_batch)
     same = same and served["saturated"]["bit_exact_vs_batched"] and all(
@@ -370,7 +387,7 @@ def bench_chunks(svd, _lib, args):
     d_keys.free()
     ctx.close()
 
-    check = "plain, union, batched and served final DAGs identical" if same else "MISMATCH"
+    check = "plain, union, batched, graph and served final DAGs identical" if same else "MISMATCH"
     cpu = None
     if not args.no_cpu_baseline and same:
         # The oracle on the union: the CPU baseline of this workload and the check.
@@ -379,7 +396,8 @@ def bench_chunks(svd, _lib, args):
         ora = oracle_build(union, global_depth, args.threads)
         t_cpu = time.perf_counter() - t0
         if np.array_equal(ora, w_batch):
-            check = "bit-exact vs oracle (union of the chunks at depth 12; plain, union, batched and served identical)"
+            check = ("bit-exact vs oracle (union of the chunks at depth 12; plain, union, batched, graph and served "
+                     "identical)")
         else:
             check = "MISMATCH vs oracle"
         cpu = {"value": round(total / t_cpu, 1), "unit": "voxels/s", "cores": args.threads, "kind": "port",
@@ -388,6 +406,7 @@ def bench_chunks(svd, _lib, args):
             "plain_ms": round(t_plain * 1e3, 2), "union_voxels_per_s": round(total / t_union, 1),
             "union_ms": round(t_union * 1e3, 3), "batched_voxels_per_s": round(total / t_batch, 1),
             "batched_ms": round(t_batch * 1e3, 3), "batched_vs_plain": round(t_plain / t_batch, 2),
+            "graph_ms": round(t_graph * 1e3, 3), "graph_voxels_per_s": round(total / t_graph, 1),
             "batched_vs_union": round(t_union / t_batch, 3),
             "served": served, "served_vs_batched": round(served["saturated"]["voxels_per_s"] * t_batch / total, 3),
             "autotune": tuned, "cpu_baseline": cpu, "check": check}

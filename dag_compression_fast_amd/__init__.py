@@ -22,7 +22,7 @@ from . import _lib
 from ._lib import SvdagError, SVDAG_DEVICE, SVDAG_HOST
 
 __all__ = [
-    "Context", "DeviceBuffer", "DeviceView", "Dag", "ResidentDag", "SvdagError", "create_dag", "create_dag_batched",
+    "BatchedGraph", "Context", "DeviceBuffer", "DeviceView", "Dag", "ResidentDag", "SvdagError", "create_dag", "create_dag_batched",
     "create_dag_batched_gather", "merge_dags",
     "create_final_dag", "free_all", "default_context", "morton_encode", "gen_sphere_shells",
     "gen_triangle_surfaces", "gen_value_noise", "gen_chunk_plan", "gen_chunk",
@@ -303,6 +303,39 @@ def create_dag_batched_gather(chunk_keys, chunk_depth, ctx=None, num_chunks=None
         ctx.handle, ptrs.ctypes.data_as(ctypes.POINTER(ctypes.c_void_p)),
         counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), num, int(chunk_depth), ctypes.byref(out)))
     return Dag(out, ctx)
+
+
+class BatchedGraph:
+    """The batched build captured once as a HIP graph (svdag_batched_graph_*)
+    over the device buffers `keys` (capacity max_keys keys) and `offsets`
+    (num_chunks + 1 uint64).  Write new chunk keys and offsets into those
+    buffers, then ``launch()`` and ``result()``; the result's arrays live in the
+    graph until its next launch."""
+
+    def __init__(self, keys, offsets, num_chunks, chunk_depth, max_keys, ctx=None):
+        self.ctx = ctx or default_context()
+        self.handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().svdag_batched_graph_create(self.ctx.handle, keys.ptr, offsets.ptr, int(num_chunks),
+                                                         int(chunk_depth), int(max_keys), ctypes.byref(self.handle)))
+
+    def launch(self):
+        _lib.check(_lib.lib().svdag_batched_graph_launch(self.handle))
+
+    def result(self):
+        out = _lib.SvdagDag()
+        _lib.check(_lib.lib().svdag_batched_graph_result(self.handle, ctypes.byref(out)))
+        return Dag(out, self.ctx)
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            _lib.lib().svdag_batched_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 -- interpreter shutdown
+            pass
 
 
 class ResidentDag:

@@ -129,6 +129,10 @@ def bind_library(L):
         "svdag_merge_dags": ([vp, dagp, u64, dagp], i32),
         "svdag_create_dag_batched": ([vp, vp, ctypes.POINTER(u64), u64, i32, dagp], i32),
         "svdag_create_dag_batched_gather": ([vp, ctypes.POINTER(vp), ctypes.POINTER(u64), u64, i32, dagp], i32),
+        "svdag_batched_graph_create": ([vp, vp, vp, u64, i32, u64, ctypes.POINTER(vp)], i32),
+        "svdag_batched_graph_launch": ([vp], i32),
+        "svdag_batched_graph_result": ([vp, dagp], i32),
+        "svdag_batched_graph_destroy": ([vp], i32),
         "svdag_resident_create": ([vp, i32, i32, u64, ctypes.POINTER(vp)], i32),
         "svdag_resident_destroy": ([vp], i32),
         "svdag_resident_insert": ([vp, dagp, ctypes.POINTER(u64)], i32),

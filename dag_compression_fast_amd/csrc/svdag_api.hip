@@ -133,9 +133,6 @@ struct svdag_context {
     int device = 0;
     char* arena = nullptr;
     uint64_t arena_cap = 0;
-    // Host tile table of the last batched build (kept until its copy is done).
-    std::vector<SegTile> seg_tiles;
-    std::vector<uint32_t> seg_chunk_tiles;
     // Source table of the last gathered build (kept until its copy is done).
     std::vector<GatherSource> gather_src;
     // Result blocks (DAG storage, final words): reused across calls, returned
@@ -472,10 +469,14 @@ void empty_dag(svdag_dag* out, int num_levels, int location)
     out->location = location;
 }
 
+// A batched build: num_chunks chunks back to back.  Their offsets come from
+// the host (h_offsets, uploaded by the build) or already sit in device memory
+// (d_offsets, a captured build's: checked and counted on the device).
 struct BatchInfo {
     const uint64_t* h_offsets;
     uint64_t num_chunks;
     int chunk_depth;
+    const uint64_t* d_offsets = nullptr;
 };
 
 // Returned by build() when a table sized from an estimate overflowed; the
@@ -486,12 +487,94 @@ constexpr int kRetryWithBounds = -100;
 // arena, the caller's buffer in place of a pooled result block.  Such a build
 // also keeps to the context's stream (no side stream, which is created on
 // first use) and times no kernels (timing events are created on demand).
+struct BuildResult;
 struct FixedMemory {
     char* arena;
     uint64_t arena_bytes;
     char* storage;
     uint64_t storage_bytes;
+    // Non-null: the build is being captured into a graph.  It enqueues its
+    // work and the counter read-back only, and leaves here what completing a
+    // replay needs (complete_build).
+    BuildResult* captured = nullptr;
 };
+
+// Where a build's result lives, for turning its counters into a svdag_dag.
+struct BuildResult {
+    int levels = 0;
+    char* O = nullptr;
+    DagStorage S;
+    uint64_t* leaves = nullptr;
+    uint64_t code_mask = 0;
+    bool batched = false;
+};
+
+// The result of a build whose work has finished, from its counters: fills
+// `out` (arrays in the result block) or returns the error the counters report
+// (kRetryWithBounds when a table sized from an estimate overflowed).
+int complete_build(const BuildResult& R, const BuildCounters& h, bool estimate, svdag_dag* out)
+{
+    if (!R.batched && (h.key_or & ~R.code_mask))
+        return set_error(SVDAG_ERR_ARGUMENT, "a key is >= 2^(3*depth) for depth %d", R.levels + 2);
+    if (h.error & kErrKeyRange) return set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)");
+    if (h.error & kErrOffsets)
+        return set_error(SVDAG_ERR_ARGUMENT, "chunk offsets do not start at 0, decrease, or exceed the key capacity");
+    if ((h.error & kErrTableOverflow) && estimate) return kRetryWithBounds;
+    if (h.error & kErrTableOverflow) return set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow");
+    if (h.error & kErrInternal) return set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow");
+    empty_dag(out, R.levels, SVDAG_DEVICE);
+    out->num_keys = h.n;
+    if (h.n == 0) return 0;
+    if (h.uniq[0] != 1) return set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]);
+    out->leaves = R.leaves;
+    out->num_leaves = h.unique_leaves;
+    out->leaf_positions = h.leaf_positions;
+    out->hash_collisions = h.collisions;
+    for (int l = 0; l < R.levels; ++l) {
+        out->levels[l].hashes = reinterpret_cast<uint64_t*>(R.O + R.S.hashes[l]);
+        out->levels[l].children = reinterpret_cast<uint32_t*>(R.O + R.S.children[l]);
+        out->levels[l].masks = reinterpret_cast<uint8_t*>(R.O + R.S.masks[l]);
+        out->levels[l].num_nodes = h.uniq[l];
+        out->level_words[l] = h.words[l];
+        out->pre_dedup[l] = h.pre[l];
+    }
+    return 0;
+}
+
+// Scratch of a build beyond plan_build's layout: a batched build's device
+// offsets and, when it sorts inside each chunk, the segmented sort's tables.
+struct BatchLayout {
+    uint64_t total = 0;  // bytes from BuildLayout::total on
+    uint64_t offsets = 0, seg_tiles = 0, seg_ct = 0, seg_counts = 0, seg_totals = 0, seg_starts = 0, seg_ntiles = 0;
+    bool segmented = false;
+    uint32_t seg_lo = 0, seg_chunk_bits = 0;
+};
+
+BatchLayout plan_batch(const BuildLayout& L, uint64_t n, const BatchInfo* batch)
+{
+    BatchLayout B;
+    if (!batch) return B;
+    Carver c;
+    c.off = L.total;
+    B.offsets = c.take(8 * (batch->num_chunks + 1));
+    // Batched builds whose chunk-local sort spans at most kSegMaxBits bits sort
+    // inside each chunk (the chunk order is the input order); the tiles are
+    // planned on the device from the offsets.
+    B.seg_lo = uint32_t(L.key_plan.shift[0]);
+    B.seg_chunk_bits = uint32_t(3 * batch->chunk_depth);
+    B.segmented = L.key_plan.passes > 0 && B.seg_chunk_bits > B.seg_lo && B.seg_chunk_bits - B.seg_lo <= kSegMaxBits;
+    if (B.segmented) {
+        const uint64_t tb = seg_tile_bound(n, batch->num_chunks);
+        B.seg_tiles = c.take(sizeof(SegTile) * tb);
+        B.seg_ct = c.take(4 * (batch->num_chunks + 1));
+        B.seg_counts = c.take(4 * (tb << kMaxSegDigitBits));
+        B.seg_totals = c.take(4 * (batch->num_chunks << kMaxSegDigitBits));
+        B.seg_starts = c.take(4 * (batch->num_chunks << kMaxSegDigitBits));
+        B.seg_ntiles = c.take(8);
+    }
+    B.total = c.off - L.total;
+    return B;
+}
 
 // Result storage of a build over n keys at `depth` (the layout of plan_storage).
 uint64_t result_bytes(uint64_t n, int depth)
@@ -515,35 +598,8 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
 
     const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits,
                                      ctx->sort_ipt);
-    const uint64_t offsets_bytes = batch ? align_up(8 * (batch->num_chunks + 1)) : 0;
-    // Batched builds whose chunk-local sort spans at most kSegMaxBits bits sort
-    // inside each chunk (the chunk order is the input order): tile table laid
-    // out here, chunk by chunk.
-    const uint32_t seg_lo = uint32_t(L.key_plan.shift[0]);
-    const uint32_t seg_chunk_bits = batch ? uint32_t(3 * batch->chunk_depth) : 0;
-    const bool segmented = batch && L.key_plan.passes > 0 && seg_chunk_bits > seg_lo &&
-                           seg_chunk_bits - seg_lo <= kSegMaxBits;
-    uint64_t seg_bytes = 0, o_seg_tiles = 0, o_seg_ct = 0, o_seg_counts = 0, o_seg_totals = 0, o_seg_starts = 0;
-    if (segmented) {
-        ctx->seg_tiles.clear();
-        ctx->seg_chunk_tiles.assign(batch->num_chunks + 1, 0);
-        for (uint64_t ch = 0; ch < batch->num_chunks; ++ch) {
-            ctx->seg_chunk_tiles[ch] = uint32_t(ctx->seg_tiles.size());
-            for (uint64_t s0 = batch->h_offsets[ch]; s0 < batch->h_offsets[ch + 1]; s0 += kSegTileItems)
-                ctx->seg_tiles.push_back(SegTile{
-                    s0, uint32_t(std::min<uint64_t>(kSegTileItems, batch->h_offsets[ch + 1] - s0)), uint32_t(ch)});
-        }
-        ctx->seg_chunk_tiles[batch->num_chunks] = uint32_t(ctx->seg_tiles.size());
-        Carver sc;
-        sc.off = L.total + offsets_bytes;
-        o_seg_tiles = sc.take(sizeof(SegTile) * ctx->seg_tiles.size());
-        o_seg_ct = sc.take(4 * (batch->num_chunks + 1));
-        o_seg_counts = sc.take(4 * (uint64_t(ctx->seg_tiles.size()) << kMaxSegDigitBits));
-        o_seg_totals = sc.take(4 * (batch->num_chunks << kMaxSegDigitBits));
-        o_seg_starts = sc.take(4 * (batch->num_chunks << kMaxSegDigitBits));
-        seg_bytes = sc.off - (L.total + offsets_bytes);
-    }
-    const uint64_t scratch_need = L.total + offsets_bytes + seg_bytes;
+    const BatchLayout B = plan_batch(L, n, batch);
+    const uint64_t scratch_need = L.total + B.total;
     if (fixed && scratch_need > fixed->arena_bytes)
         return set_error(SVDAG_ERR_MEMORY, "build needs %llu scratch bytes, the fixed scratch holds %llu",
                          (unsigned long long)scratch_need, (unsigned long long)fixed->arena_bytes);
@@ -568,6 +624,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         TRY(device_alloc(ctx, &storage, S.total));
     }
     char* O = static_cast<char*>(storage);
+    const bool capture = fixed && fixed->captured;
     bool forked = false;  // work is in flight on the side stream
     auto fail = [&](int code) {
         if (forked) (void)hipStreamSynchronize(ctx->side);
@@ -577,15 +634,25 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     };
 
     launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + L.zero_begin), (L.zero_end - L.zero_begin) / 8, 0);
-    launch_fill_u64(st, &c->n, 1, n);
+    // The key count: given, or (device offsets) checked and read on the device.
+    const uint64_t* d_off = nullptr;
+    if (batch && batch->d_offsets) {
+        d_off = batch->d_offsets;
+        launch_batch_count(st, d_off, batch->num_chunks, n, &c->n, err);
+    } else {
+        launch_fill_u64(st, &c->n, 1, n);
+    }
 
     // Sort keys on bits [0, 3*depth) (SortKeys, DAGCompression.cu:1024-1026).
     int where;
     const void* sorted;
     if (batch) {
-        uint64_t* d_off = reinterpret_cast<uint64_t*>(A + L.total);
-        if (hipMemcpyAsync(d_off, batch->h_offsets, 8 * (batch->num_chunks + 1), hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(set_error(SVDAG_ERR_HIP, "offset upload failed"));
+        if (!d_off) {
+            uint64_t* up = reinterpret_cast<uint64_t*>(A + B.offsets);
+            if (hipMemcpyAsync(up, batch->h_offsets, 8 * (batch->num_chunks + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+                return fail(set_error(SVDAG_ERR_HIP, "offset upload failed"));
+            d_off = up;
+        }
         // The sort's first reads put each chunk's index above its chunk-local keys.
         ChunkPrefix prefix;
         prefix.offsets = d_off;
@@ -594,19 +661,17 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         prefix.err = err;
         uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
         uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
-        if (segmented) {
-            SegTile* d_tiles = reinterpret_cast<SegTile*>(A + o_seg_tiles);
-            uint32_t* d_ct = reinterpret_cast<uint32_t*>(A + o_seg_ct);
-            if (hipMemcpyAsync(d_tiles, ctx->seg_tiles.data(), sizeof(SegTile) * ctx->seg_tiles.size(),
-                               hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(d_ct, ctx->seg_chunk_tiles.data(), 4 * ctx->seg_chunk_tiles.size(),
-                               hipMemcpyHostToDevice, st) != hipSuccess)
-                return fail(set_error(SVDAG_ERR_HIP, "tile table upload failed"));
-            where = launch_segmented_sort(st, reinterpret_cast<const uint64_t*>(d_keys), ka, kb, d_tiles,
-                                          ctx->seg_tiles.size(), d_ct, d_off, batch->num_chunks, seg_lo,
-                                          seg_chunk_bits, reinterpret_cast<uint32_t*>(A + o_seg_counts),
-                                          reinterpret_cast<uint32_t*>(A + o_seg_totals),
-                                          reinterpret_cast<uint32_t*>(A + o_seg_starts), err, ctx->seg_first_bits);
+        if (B.segmented) {
+            SegScratch sc;
+            sc.tiles = reinterpret_cast<SegTile*>(A + B.seg_tiles);
+            sc.chunk_tiles = reinterpret_cast<uint32_t*>(A + B.seg_ct);
+            sc.counts = reinterpret_cast<uint32_t*>(A + B.seg_counts);
+            sc.totals = reinterpret_cast<uint32_t*>(A + B.seg_totals);
+            sc.starts = reinterpret_cast<uint32_t*>(A + B.seg_starts);
+            sc.ntiles = reinterpret_cast<uint64_t*>(A + B.seg_ntiles);
+            where = launch_segmented_sort(st, reinterpret_cast<const uint64_t*>(d_keys), ka, kb, d_off,
+                                          batch->num_chunks, &c->n, n, sc, B.seg_lo, B.seg_chunk_bits, err,
+                                          ctx->seg_first_bits);
         } else {
             where = launch_sort<uint64_t>(st, L.key_plan, reinterpret_cast<const uint64_t*>(d_keys), nullptr, ka, kb,
                                           nullptr, nullptr, nullptr, nullptr, &c->n, n,
@@ -622,7 +687,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     }
     if (where < 0) return fail(set_error(SVDAG_ERR_INTERNAL, "sort plan failed"));
-    mark(ctx, "key sort");
+    if (!capture) mark(ctx, "key sort");
 
     uint64_t* lp_code = reinterpret_cast<uint64_t*>(A + L.lp_code);
     uint64_t* lp_mask = reinterpret_cast<uint64_t*>(A + L.lp_mask);
@@ -640,7 +705,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
                                      lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
                                      code_mask);
 
-    mark(ctx, "leaf reduce");
+    if (!capture) mark(ctx, "leaf reduce");
     // SortLevel(Leaves) (DAGCompression.cu:1070-1071): dedup table, then order.
     uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + L.leaf_table);
     uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + L.leaf_rank);
@@ -699,10 +764,10 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         joined = true;
         if (hipStreamWaitEvent(st, ctx->join_ev, 0) != hipSuccess)
             return fail(set_error(SVDAG_ERR_HIP, "stream wait failed"));
-        mark(ctx, "leaf order");
+        if (!capture) mark(ctx, "leaf order");
         return 0;
     };
-    mark(ctx, "leaf dedup");
+    if (!capture) mark(ctx, "leaf dedup");
 
     // Levels, bottom-up (DAGCompression.cu:1077-1092): hashed-table levels down
     // to tail_top + 1, then levels tail_top..0 (at most 4096 nodes each) in one
@@ -751,7 +816,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
         launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : nodes_bound(l + 1));
-        mark(ctx, "level build");
+        if (!capture) mark(ctx, "level build");
         TRY(order_leaves());
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
@@ -774,7 +839,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.w_count = &c->words[l];
         TRY(join());
         launch_level_extract(st, e, slots_bound(l));
-        mark(ctx, "level extract");
+        if (!capture) mark(ctx, "level extract");
         if (l - 1 > tail_top)
             launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[cur ^ 1]), &c->pre[l - 1],
                                &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, slots_bound(l - 1));
@@ -813,7 +878,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         t.check_collisions = ctx->check_collisions;
         TRY(join());
         launch_level_tail(st, t);
-        mark(ctx, "top levels");
+        if (!capture) mark(ctx, "top levels");
     }
     {
         const hipError_t e = hipGetLastError();
@@ -821,33 +886,24 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     }
     if (hipMemcpyAsync(ctx->h_counters, c, sizeof(BuildCounters), hipMemcpyDeviceToHost, st) != hipSuccess)
         return fail(set_error(SVDAG_ERR_HIP, "counter read-back failed: %s", hipGetErrorString(hipGetLastError())));
+    BuildResult R;
+    R.levels = levels;
+    R.O = O;
+    R.S = S;
+    R.leaves = leaves_out;
+    R.code_mask = code_mask;
+    R.batched = batch != nullptr;
+    if (capture) {
+        *fixed->captured = R;
+        return 0;
+    }
     mark(ctx, "counter read-back");
     // On a stall or an asynchronous error the result block goes back to the
     // pool (stream order keeps its later reuse safe).
     if (const int rc = finish(ctx, "create_dag")) return fail(rc);
-    const BuildCounters& h = *ctx->h_counters;
-    if (!batch && (h.key_or & ~code_mask))
-        return fail(set_error(SVDAG_ERR_ARGUMENT, "a key is >= 2^(3*depth) for depth %d", depth));
-    if (h.error & kErrKeyRange)
-        return fail(set_error(SVDAG_ERR_ARGUMENT, "a chunk-local key is >= 2^(3*chunk_depth)"));
-    if ((h.error & kErrTableOverflow) && estimate) return fail(kRetryWithBounds);
-    if (h.error & kErrTableOverflow) return fail(set_error(SVDAG_ERR_INTERNAL, "dedup table probe overflow"));
-    if (h.error & kErrInternal) return fail(set_error(SVDAG_ERR_INTERNAL, "leaf reduce region overflow"));
-    if (h.uniq[0] != 1) return fail(set_error(SVDAG_ERR_INTERNAL, "root level holds %llu nodes", (unsigned long long)h.uniq[0]));
-
+    if (const int rc = complete_build(R, *ctx->h_counters, estimate, out)) return fail(rc);
+    out->num_keys = n;
     out->storage = fixed ? nullptr : storage;  // the caller's buffer is not the library's to free
-    out->leaves = leaves_out;
-    out->num_leaves = h.unique_leaves;
-    out->leaf_positions = h.leaf_positions;
-    out->hash_collisions = h.collisions;
-    for (int l = 0; l < levels; ++l) {
-        out->levels[l].hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
-        out->levels[l].children = reinterpret_cast<uint32_t*>(O + S.children[l]);
-        out->levels[l].masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
-        out->levels[l].num_nodes = h.uniq[l];
-        out->level_words[l] = h.words[l];
-        out->pre_dedup[l] = h.pre[l];
-    }
     return 0;
 }
 
@@ -1864,6 +1920,129 @@ int svdag_profile_read(svdag_context* ctx, int32_t kernel, double* total_ms, uin
 const char* svdag_kernel_name(int32_t kernel)
 {
     return kernel >= 0 && kernel < kNumKernelIds ? kKernelNames[kernel] : nullptr;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched build captured as a HIP graph.  The build is enqueued once on the
+// graph's own stream under stream capture, sized for max_keys keys: the chunk
+// offsets are read on the device (checked there, and the segmented sort's
+// tiles planned from them), every kernel takes its real counts from device
+// memory, the leaf table is sized from its bound (no estimate to retry), and no
+// stream is forked.  A replay therefore needs no host work: the caller writes
+// new keys and offsets into the same device buffers and launches the graph.
+// ---------------------------------------------------------------------------
+struct svdag_batched_graph {
+    svdag_context* ctx = nullptr;  // the graph's own context, on its own stream
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    char* arena = nullptr;
+    char* storage = nullptr;
+    BuildResult R;
+    bool launched = false;
+};
+
+extern "C" {
+
+int svdag_batched_graph_destroy(svdag_batched_graph* g)
+{
+    if (!g) return 0;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->arena) (void)hipFree(g->arena);
+    if (g->storage) (void)hipFree(g->storage);
+    if (g->ctx) (void)svdag_context_destroy(g->ctx);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+int svdag_batched_graph_create(svdag_context* ctx, const uint64_t* d_keys, const uint64_t* d_offsets,
+                               uint64_t num_chunks, int32_t chunk_depth, uint64_t max_keys,
+                               svdag_batched_graph** out)
+{
+    TRY(check_ctx(ctx));
+    if (!out || !d_keys || !d_offsets) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    int extra = 0;
+    for (uint64_t c = num_chunks; c > 1; c /= 8) {
+        if (c % 8) return set_error(SVDAG_ERR_ARGUMENT, "chunk count %llu is not a power of 8", (unsigned long long)num_chunks);
+        ++extra;
+    }
+    if (num_chunks == 0) return set_error(SVDAG_ERR_ARGUMENT, "no chunks");
+    const int depth = chunk_depth + extra;
+    if (chunk_depth < 3 || depth > 21) return set_error(SVDAG_ERR_ARGUMENT, "bad chunk depth %d", chunk_depth);
+    if (max_keys == 0 || max_keys >= (1ull << 30))
+        return set_error(SVDAG_ERR_ARGUMENT, "max_keys must be in [1, 2^30)");
+    auto* g = new svdag_batched_graph();
+    auto fail = [&](int code) {
+        (void)svdag_batched_graph_destroy(g);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        g->stream = nullptr;
+        return fail(set_error(SVDAG_ERR_HIP, "cannot create the graph's stream"));
+    }
+    if (const int rc = svdag_context_create(&g->ctx, g->stream)) return fail(rc);
+    g->ctx->load_shift = ctx->load_shift;
+    g->ctx->radix_bits = ctx->radix_bits;
+    g->ctx->sort_ipt = ctx->sort_ipt;
+    g->ctx->seg_first_bits = ctx->seg_first_bits;
+    g->ctx->check_collisions = ctx->check_collisions;
+    g->ctx->wait_seconds = ctx->wait_seconds;
+    const BatchInfo batch{nullptr, num_chunks, chunk_depth, d_offsets};
+    const BuildLayout L = plan_build(max_keys, depth, 8, g->ctx->load_shift, true, g->ctx->radix_bits,
+                                     g->ctx->sort_ipt);
+    const uint64_t arena_bytes = L.total + plan_batch(L, max_keys, &batch).total;
+    const uint64_t storage_bytes = result_bytes(max_keys, depth);
+    if (ctx->budget && arena_bytes + storage_bytes > ctx->budget)
+        return fail(set_error(SVDAG_ERR_MEMORY, "graph needs %llu bytes, the budget is %llu",
+                              (unsigned long long)(arena_bytes + storage_bytes), (unsigned long long)ctx->budget));
+    if (hipMalloc(reinterpret_cast<void**>(&g->arena), arena_bytes) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&g->storage), storage_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes for the graph",
+                              (unsigned long long)(arena_bytes + storage_bytes)));
+    }
+    FixedMemory fm{g->arena, arena_bytes, g->storage, storage_bytes};
+    fm.captured = &g->R;
+    active_recorder() = nullptr;  // timing events are not part of the graph
+    if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "stream capture did not start"));
+    svdag_dag unused;
+    const int rc = build<uint64_t>(g->ctx, d_keys, max_keys, depth, &unused, &batch, false, &fm);
+    const hipError_t ec = hipStreamEndCapture(g->stream, &g->graph);
+    if (rc) return fail(rc);
+    if (ec != hipSuccess || !g->graph) return fail(set_error(SVDAG_ERR_HIP, "stream capture failed: %s", hipGetErrorString(ec)));
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        return fail(set_error(SVDAG_ERR_HIP, "graph instantiation failed"));
+    *out = g;
+    return 0;
+}
+
+int svdag_batched_graph_launch(svdag_batched_graph* g)
+{
+    if (!g) return set_error(SVDAG_ERR_ARGUMENT, "null graph");
+    TRY(check_ctx(g->ctx));
+    HIP_TRY(hipGraphLaunch(g->exec, g->stream));
+    g->launched = true;
+    return 0;
+}
+
+int svdag_batched_graph_result(svdag_batched_graph* g, svdag_dag* out)
+{
+    if (!g || !out) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
+    TRY(check_ctx(g->ctx));
+    if (!g->launched) return set_error(SVDAG_ERR_ARGUMENT, "the graph has not been launched");
+    TRY(finish(g->ctx, "batched graph"));
+    const int rc = complete_build(g->R, *g->ctx->h_counters, false, out);
+    if (rc) return rc;
+    out->storage = nullptr;  // the graph's storage, valid until its next launch
+    return 0;
 }
 
 }  // extern "C"

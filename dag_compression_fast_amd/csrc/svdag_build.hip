@@ -1313,6 +1313,27 @@ __global__ __launch_bounds__(kRootThreads) void k_resident_roots(ResidentRootArg
     }
 }
 
+// One workgroup checks the device chunk offsets of a batched build and sets
+// its key count (0 when they are invalid, so that no kernel reads past them).
+constexpr int kCountThreads = 1024;
+
+__global__ __launch_bounds__(kCountThreads) void k_batch_count(const uint64_t* __restrict__ offsets,
+                                                               uint64_t num_chunks, uint64_t n_bound,
+                                                               uint64_t* __restrict__ n_out,
+                                                               unsigned long long* __restrict__ err)
+{
+    __shared__ uint32_t s_bad;
+    if (threadIdx.x == 0) s_bad = offsets[0] != 0 || offsets[num_chunks] > n_bound;
+    __syncthreads();
+    for (uint64_t c = threadIdx.x; c < num_chunks; c += kCountThreads)
+        if (offsets[c + 1] < offsets[c]) s_bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *n_out = s_bad ? 0 : offsets[num_chunks];
+        if (s_bad) atomicOr(err, (unsigned long long)kErrOffsets);
+    }
+}
+
 // Keys of several device arrays, back to back: thread i finds its source by a
 // binary search over the destination starts.
 __global__ void k_gather_keys(const GatherSource* __restrict__ src, uint32_t nsrc, uint64_t total,
@@ -1595,6 +1616,13 @@ void launch_resident_level(hipStream_t s, const ResidentLevelArgs& a)
     TimedLaunch t(kKResidentInsert, s);
     hipLaunchKernelGGL(k_resident_level, dim3(grid_for((a.count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s,
                        a);
+}
+
+void launch_batch_count(hipStream_t s, const uint64_t* offsets, uint64_t num_chunks, uint64_t n_bound, uint64_t* n_out,
+                        unsigned long long* err)
+{
+    TimedLaunch t(kKFill, s);
+    hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(kCountThreads), 0, s, offsets, num_chunks, n_bound, n_out, err);
 }
 
 void launch_gather_keys(hipStream_t s, const GatherSource* src, uint32_t nsrc, uint64_t total, uint64_t* dst)

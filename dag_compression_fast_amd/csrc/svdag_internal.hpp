@@ -129,16 +129,14 @@ struct ChunkPrefix {
 };
 
 // Segmented key sort of a batched build (chunks back to back, in chunk order):
-// tiles of at most kSegTileItems keys, none crossing a chunk, laid out by the
-// host chunk by chunk; chunk_tiles[c] = first tile of chunk c (num_chunks + 1
-// entries), chunk_start[c] = its first key.  Sorts the chunk-local bits
-// [lo, chunk_bits) (at most kSegMaxBits of them, in one or two passes of at
-// most kMaxSegDigitBits) and puts each chunk's index above chunk_bits.
-// Scratch: counts (ntiles << kMaxSegDigitBits u32), totals and starts
-// (num_chunks << kMaxSegDigitBits u32 each).  Returns 0 (result in keys_a), 1
-// (keys_b), or -1 when the bit span does not fit.  first_bits: the first
-// pass's digit width (0: half the bits), clamped so both passes have at most
-// kMaxSegDigitBits.
+// tiles of at most kSegTileItems keys, none crossing a chunk, planned on the
+// device from the chunk offsets (chunk_start: num_chunks + 1 device entries;
+// *n_ptr keys in all, at most n_bound; *n_ptr == 0 plans nothing).  Sorts the
+// chunk-local bits [lo, chunk_bits) (at most kSegMaxBits of them, in one or
+// two passes of at most kMaxSegDigitBits) and puts each chunk's index above
+// chunk_bits.  Returns 0 (result in keys_a), 1 (keys_b), or -1 when the bit
+// span does not fit.  first_bits: the first pass's digit width (0: half the
+// bits), clamped so both passes have at most kMaxSegDigitBits.
 struct SegTile {
     uint64_t start;
     uint32_t count;
@@ -147,10 +145,21 @@ struct SegTile {
 constexpr uint32_t kSegTileItems = 4096;
 constexpr uint32_t kSegMaxBits = 12;
 constexpr uint32_t kMaxSegDigitBits = 8;
+inline uint64_t seg_tile_bound(uint64_t n_bound, uint64_t num_chunks) { return n_bound / kSegTileItems + num_chunks; }
+// Scratch of the segmented sort: tiles (seg_tile_bound entries), chunk_tiles
+// (num_chunks + 1), counts (tile bound << kMaxSegDigitBits), totals and starts
+// (num_chunks << kMaxSegDigitBits each), the tile count.
+struct SegScratch {
+    SegTile* tiles;
+    uint32_t* chunk_tiles;
+    uint32_t* counts;
+    uint32_t* totals;
+    uint32_t* starts;
+    uint64_t* ntiles;
+};
 int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
-                          const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
-                          const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
-                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err,
+                          const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
+                          const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits = 0);
 
 template <typename K>
@@ -182,6 +191,7 @@ constexpr uint64_t kErrTableOverflow = 1;
 constexpr uint64_t kErrKeyRange = 2;
 constexpr uint64_t kErrInternal = 4;  // a bound the kernels rely on did not hold
 constexpr uint64_t kErrSlotFilled = 8;  // a resident chunk slot received a second chunk
+constexpr uint64_t kErrOffsets = 16;    // device chunk offsets invalid or past the key capacity
 
 // Tile sizes of the compaction stages; a stage over at most n items needs
 // tiles(n, kXxxTile) u32 counts and u64 offsets of scratch.
@@ -197,6 +207,11 @@ inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
 int device_cu_count();
 
 void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value);
+
+// *n_out = offsets[num_chunks] when the device offsets start at 0, never
+// decrease and end at most at n_bound; else *n_out = 0 and kErrOffsets in *err.
+void launch_batch_count(hipStream_t s, const uint64_t* offsets, uint64_t num_chunks, uint64_t n_bound, uint64_t* n_out,
+                        unsigned long long* err);
 
 // One-workgroup exclusive scan of tile counts into offsets; the number of tiles
 // is ceil(*items / per_tile); the total goes to *total_out when non-null.

@@ -727,15 +727,67 @@ __device__ __forceinline__ uint64_t seg_count_index(const SegTile& tile, const u
 
 // Count the pass digit ((key >> shift) & dmask) of every key of each tile.
 // The first pass also range-checks the chunk-local keys.
+// Tile table of the segmented sort, planned on the device from the chunk
+// offsets (no host planning, so a captured build replays for any offsets
+// within its bounds): chunk c's tiles start at chunk_tiles[c] (exclusive scan
+// of ceil(keys / kSegTile)), *ntiles = their total.  *n_ptr == 0 (no keys, or
+// offsets a check rejected) plans no tiles.
+constexpr int kPlanThreads = 1024;
+
+__global__ __launch_bounds__(kPlanThreads) void k_seg_plan_chunks(const uint64_t* __restrict__ offsets,
+                                                                  uint64_t num_chunks,
+                                                                  const uint64_t* __restrict__ n_ptr,
+                                                                  uint32_t* __restrict__ chunk_tiles,
+                                                                  uint64_t* __restrict__ ntiles)
+{
+    __shared__ uint32_t s_scan[kPlanThreads / kWave];
+    const bool any = *n_ptr != 0;
+    uint32_t carry = 0;
+    for (uint64_t base = 0; base < num_chunks; base += kPlanThreads) {
+        const uint64_t c = base + threadIdx.x;
+        uint32_t t = 0;
+        if (any && c < num_chunks) t = uint32_t((offsets[c + 1] - offsets[c] + kSegTile - 1) / kSegTile);
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kPlanThreads>(t, s_scan, total);
+        if (c < num_chunks) chunk_tiles[c] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        chunk_tiles[num_chunks] = carry;
+        *ntiles = carry;
+    }
+}
+
+// Tile t: the chunk whose tile range holds it (last c with chunk_tiles[c] <= t).
+__global__ void k_seg_plan_tiles(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ chunk_tiles,
+                                 uint64_t num_chunks, const uint64_t* __restrict__ ntiles_ptr,
+                                 SegTile* __restrict__ tiles)
+{
+    const uint64_t ntiles = *ntiles_ptr;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < ntiles; t += stride) {
+        uint64_t lo = 0, hi = num_chunks;  // chunk_tiles[lo] <= t < chunk_tiles[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (chunk_tiles[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t start = offsets[lo] + (t - chunk_tiles[lo]) * kSegTile;
+        tiles[t] = SegTile{start, uint32_t(min(uint64_t(kSegTile), offsets[lo + 1] - start)), uint32_t(lo)};
+    }
+}
+
 template <bool FIRST>
 __global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint64_t* __restrict__ keys,
-                                                           const SegTile* __restrict__ tiles, uint64_t ntiles,
+                                                           const SegTile* __restrict__ tiles,
+                                                           const uint64_t* __restrict__ ntiles_ptr,
                                                            const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
                                                            uint32_t dmask, uint32_t chunk_bits,
                                                            uint32_t* __restrict__ counts,
                                                            unsigned long long* __restrict__ err)
 {
     __shared__ uint32_t s_cnt[256];
+    const uint64_t ntiles = *ntiles_ptr;
     const uint32_t R = dmask + 1;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < R; i += kSegThreads) s_cnt[i] = 0;
@@ -804,7 +856,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scan_digits(const uint64_t*
 template <bool FIRST>
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __restrict__ keys_in,
                                                              uint64_t* __restrict__ keys_out,
-                                                             const SegTile* __restrict__ tiles, uint64_t ntiles,
+                                                             const SegTile* __restrict__ tiles,
+                                                             const uint64_t* __restrict__ ntiles_ptr,
                                                              const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
                                                              uint32_t dmask, uint32_t prev_shift, uint32_t prev_bits,
                                                              uint32_t chunk_bits, const uint32_t* __restrict__ counts,
@@ -815,6 +868,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_scan[kSegThreads / kWave];
     __shared__ uint32_t s_span[2 * (kSegThreads / kWave)];
+    const uint64_t ntiles = *ntiles_ptr;
     const uint32_t R = dmask + 1;
     const uint32_t prev_mask = (1u << prev_bits) - 1u;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
@@ -918,12 +972,11 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
 }  // namespace
 
 int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
-                          const SegTile* tiles, uint64_t ntiles, const uint32_t* chunk_tiles,
-                          const uint64_t* chunk_start, uint64_t num_chunks, uint32_t lo, uint32_t chunk_bits,
-                          uint32_t* counts, uint32_t* totals, uint32_t* starts, unsigned long long* err,
+                          const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
+                          const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits)
 {
-    if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || ntiles == 0) return -1;
+    if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || n_bound == 0) return -1;
     const uint32_t B = chunk_bits - lo;
     // First pass digit width: half the bits unless asked otherwise, and never
     // leaving either pass wider than kMaxSegDigitBits.
@@ -932,35 +985,44 @@ int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t*
     const uint32_t b1 = first_bits ? std::min(hi_b1, std::max(lo_b1, first_bits))
                                    : (B > kMaxSegDigitBits ? (B + 1) / 2 : B);
     const uint32_t b2 = B - b1;                                  // second (0: one pass)
+    const uint64_t tile_bound = seg_tile_bound(n_bound, num_chunks);
     const int cus = device_cu_count();
-    const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(ntiles, uint64_t(cus) * 64)));
+    const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(tile_bound, uint64_t(cus) * 64)));
     const int chunk_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks, uint64_t(cus) * 8)));
+    {
+        TimedLaunch t(kKSortScan, stream);
+        hipLaunchKernelGGL(k_seg_plan_chunks, dim3(1), dim3(kPlanThreads), 0, stream, chunk_start, num_chunks, n_ptr,
+                           sc.chunk_tiles, sc.ntiles);
+        hipLaunchKernelGGL(k_seg_plan_tiles, dim3(int(std::max<uint64_t>(1, std::min<uint64_t>((tile_bound + 255) / 256,
+                                                                                           uint64_t(cus) * 8)))),
+                           dim3(256), 0, stream, chunk_start, sc.chunk_tiles, num_chunks, sc.ntiles, sc.tiles);
+    }
     auto pass = [&](bool first, const uint64_t* src, uint64_t* dst, uint32_t shift, uint32_t bits) {
         const uint32_t mask = (1u << bits) - 1u, R = mask + 1;
         const int row_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks * R, uint64_t(cus) * 64)));
         {
             TimedLaunch t(kKSortCount, stream);
             if (first)
-                hipLaunchKernelGGL(k_seg_count<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, tiles, ntiles,
-                                   chunk_tiles, shift, mask, chunk_bits, counts, err);
+                hipLaunchKernelGGL(k_seg_count<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, sc.tiles,
+                                   sc.ntiles, sc.chunk_tiles, shift, mask, chunk_bits, sc.counts, err);
             else
-                hipLaunchKernelGGL(k_seg_count<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, tiles, ntiles,
-                                   chunk_tiles, shift, mask, chunk_bits, counts, err);
+                hipLaunchKernelGGL(k_seg_count<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, sc.tiles,
+                                   sc.ntiles, sc.chunk_tiles, shift, mask, chunk_bits, sc.counts, err);
         }
         {
             TimedLaunch t(kKSortScan, stream);
-            hipLaunchKernelGGL(k_seg_scan_rows, dim3(row_grid), dim3(kWave), 0, stream, chunk_tiles, num_chunks, R,
-                               counts, totals);
+            hipLaunchKernelGGL(k_seg_scan_rows, dim3(row_grid), dim3(kWave), 0, stream, sc.chunk_tiles, num_chunks, R,
+                               sc.counts, sc.totals);
             hipLaunchKernelGGL(k_seg_scan_digits, dim3(chunk_grid), dim3(kSegThreads), 0, stream, chunk_start,
-                               num_chunks, R, totals, starts);
+                               num_chunks, R, sc.totals, sc.starts);
         }
         TimedLaunch t(kKSortScatterKeys, stream);
         if (first)
-            hipLaunchKernelGGL(k_seg_scatter<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, tiles, ntiles,
-                               chunk_tiles, shift, mask, 0u, 0u, chunk_bits, counts, starts);
+            hipLaunchKernelGGL(k_seg_scatter<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, sc.tiles,
+                               sc.ntiles, sc.chunk_tiles, shift, mask, 0u, 0u, chunk_bits, sc.counts, sc.starts);
         else
-            hipLaunchKernelGGL(k_seg_scatter<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, tiles, ntiles,
-                               chunk_tiles, shift, mask, lo, b1, chunk_bits, counts, starts);
+            hipLaunchKernelGGL(k_seg_scatter<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, sc.tiles,
+                               sc.ntiles, sc.chunk_tiles, shift, mask, lo, b1, chunk_bits, sc.counts, sc.starts);
     };
     pass(true, keys_in, keys_a, lo, b1);
     if (b2 == 0) return 0;

@@ -174,6 +174,26 @@ int svdag_create_dag_batched_gather(svdag_context* ctx, const uint64_t* const* d
                                     const uint64_t* counts, uint64_t num_chunks,
                                     int32_t chunk_depth, svdag_dag* out);
 
+/* The batched build captured once as a HIP graph and replayed.  No
+ * counterpart in the reference, whose batch loop synchronises the host per
+ * chunk and per level (src/DAG_Compression_Fast.cpp:283-302,
+ * src/DAGCompression.cu:252-268).  The graph reads the keys at d_keys and the
+ * num_chunks + 1 offsets at d_offsets (both device buffers the caller keeps and
+ * may rewrite between launches) and holds its scratch and result for up to
+ * max_keys keys (< 2^30).  Offsets that do not start at 0, decrease, or end
+ * past max_keys make that launch's result SVDAG_ERR_ARGUMENT.  The graph has
+ * its own stream: svdag_batched_graph_launch enqueues a replay without
+ * waiting; svdag_batched_graph_result waits for it and returns the DAG, whose
+ * arrays live in the graph's storage until its next launch (out->storage is
+ * NULL). */
+typedef struct svdag_batched_graph svdag_batched_graph;
+int svdag_batched_graph_create(svdag_context* ctx, const uint64_t* d_keys, const uint64_t* d_offsets,
+                               uint64_t num_chunks, int32_t chunk_depth, uint64_t max_keys,
+                               svdag_batched_graph** out);
+int svdag_batched_graph_launch(svdag_batched_graph* g);
+int svdag_batched_graph_result(svdag_batched_graph* g, svdag_dag* out);
+int svdag_batched_graph_destroy(svdag_batched_graph* g);
+
 /* DAGCompression::CreateFinalDAG (src/DAGCompression.cu:1550-1610): the flat
  * uint32 layout.  *d_words is a device allocation (svdag_free_words). */
 int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag,

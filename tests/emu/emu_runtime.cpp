@@ -280,7 +280,35 @@ bool valid_config(dim3 grid, dim3 block)
            grid.z <= 65535u && threads >= 1 && threads <= 1024 && block.z <= 1024;
 }
 
+// Stream capture: operations recorded instead of run (one capture at a time).
+struct GraphOp {
+    dim3 grid, block;
+    std::function<void()> body;  // a kernel launch, or
+    void* dst = nullptr;         // a copy
+    const void* src = nullptr;
+    size_t bytes = 0;
+};
+struct Graph {
+    std::vector<GraphOp> ops;
+};
+Graph* g_capture = nullptr;
+
+void run_launch(dim3 grid, dim3 block, std::function<void()>& body);
+
 void launch(dim3 grid, dim3 block, std::function<void()> body)
+{
+    if (g_capture) {
+        GraphOp op;
+        op.grid = grid;
+        op.block = block;
+        op.body = std::move(body);
+        g_capture->ops.push_back(std::move(op));
+        return;
+    }
+    run_launch(grid, block, body);
+}
+
+void run_launch(dim3 grid, dim3 block, std::function<void()>& body)
 {
     std::lock_guard<std::mutex> serial(g_launch_mutex);
     if (g_cur >= 0) {
@@ -345,7 +373,54 @@ hipError_t hipFreeAsync(void* p, hipStream_t) { return hipFree(p); }
 hipError_t hipHostMalloc(void** p, size_t n, unsigned) { return hipMalloc(p, n); }
 hipError_t hipHostFree(void* p) { return hipFree(p); }
 hipError_t hipMemcpy(void* d, const void* s, size_t n, int) { if (n) std::memmove(d, s, n); return hipSuccess; }
-hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, int k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, int k, hipStream_t)
+{
+    if (emu::g_capture) {
+        emu::GraphOp op;
+        op.dst = d;
+        op.src = s;
+        op.bytes = n;
+        emu::g_capture->ops.push_back(std::move(op));
+        return hipSuccess;
+    }
+    return hipMemcpy(d, s, n, k);
+}
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode)
+{
+    if (emu::g_capture) return hipErrorInvalidValue;
+    emu::g_capture = new emu::Graph();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g)
+{
+    if (!emu::g_capture) return hipErrorInvalidValue;
+    *g = emu::g_capture;
+    emu::g_capture = nullptr;
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t)
+{
+    *e = new emu::Graph(*static_cast<emu::Graph*>(g));
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t)
+{
+    for (emu::GraphOp& op : static_cast<emu::Graph*>(e)->ops) {
+        if (op.body) emu::run_launch(op.grid, op.block, op.body);
+        else if (op.bytes) std::memmove(op.dst, op.src, op.bytes);
+    }
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e)
+{
+    delete static_cast<emu::Graph*>(e);
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g)
+{
+    delete static_cast<emu::Graph*>(g);
+    return hipSuccess;
+}
 hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
