@@ -667,6 +667,35 @@ def profile_batched(svd, _lib, args):
     return 0
 
 
+def profile_served(svd, _lib, args):
+    """N saturated served runs of the 512-chunk workload (every chunk arrives at
+    once; rounds cut by the HBM budget), for a profiler; the first run warms
+    the shared context."""
+    from dag_compression_fast_amd import serve
+    from dag_compression_fast_amd import workloads as W
+    chunks = W.chunk_workload(args.threads)
+    offsets = W.chunk_offsets(chunks)
+    d_keys = svd.DeviceBuffer.from_numpy(np.concatenate(chunks))
+    views = [svd.DeviceView(d_keys, 8 * int(offsets[i]), len(chunks[i])) for i in range(len(chunks))]
+    ctx = svd.Context()
+    times, rounds = [], []
+    try:
+        for _ in range(args.profile_served):
+            server = serve.ChunkServer(W.CHUNKS["depth"], W.CHUNKS["levels_above"],
+                                       max_nodes=max(1 << 20, int(offsets[-1]) // 16), ctx=ctx)
+            try:
+                results = server.run([serve.Request(i, i, views[i], 0.0) for i in range(len(views))])
+                times.append(max(r.done for r in results))
+                rounds.append(server.rounds)
+            finally:
+                server.close()
+    finally:
+        d_keys.free()
+        ctx.close()
+    print(json.dumps({"served_runs": len(times), "rounds": rounds, "best_ms": round(min(times) * 1e3, 3)}), flush=True)
+    return 0
+
+
 def profile_noise(svd, _lib, args):
     """N rounds of the value-noise flow (8 batch builds + merge + flat layout)
     and nothing else, for a profiler: the dedup kernels at worst-case load."""
@@ -737,6 +766,8 @@ def main():
     ap.add_argument("--no-sort", action="store_true", help="skip the sort_keys timing")
     ap.add_argument("--profile-batched", type=int, default=0, metavar="N",
                     help="only run N batched builds of the chunk workload (for a profiler), then exit")
+    ap.add_argument("--profile-served", type=int, default=0, metavar="N",
+                    help="only run N saturated served runs of the chunk workload (for a profiler), then exit")
     ap.add_argument("--profile-noise", type=int, default=0, metavar="N",
                     help="only run N rounds of the value-noise flow (for a profiler), then exit")
     ap.add_argument("--chunk-reps", type=int, default=2)
@@ -758,6 +789,8 @@ def main():
         return profile_batched(svd, _lib, args)
     if args.profile_noise:
         return profile_noise(svd, _lib, args)
+    if args.profile_served:
+        return profile_served(svd, _lib, args)
     spec = WORKLOADS[args.workload]
     depth = spec["depth"]
     log(f"generating {args.workload} on {args.threads} host threads")

@@ -298,8 +298,15 @@ struct BuildLayout {
     uint64_t lp_code = 0, lp_mask = 0, lp_slot = 0, leaf_table = 0, leaf_rank = 0;
     uint64_t reg_code = 0, reg_mask = 0;  // leaf-reduce regions, over the later stages' buffers
     uint64_t list_keys[2] = {}, list_slots[2] = {}, sorted_slots = 0;
-    uint64_t codes[2] = {}, slots[2] = {}, hashes[2] = {}, masks = 0, children = 0, tail_hash = 0;
-    uint64_t tkeys[2] = {}, trep[2] = {}, trank[2] = {};
+    // Levels: build outputs (codes, slots, hashes) ping-pong between
+    // consecutive level builds; every hashed level keeps its own records
+    // (masks, children) and table until its extract, so that all level builds
+    // can run before the first extract (which waits for the leaf order).
+    uint64_t codes[2] = {}, slots[2] = {}, hashes[2] = {};
+    uint64_t masks[SVDAG_MAX_LEVELS] = {}, children[SVDAG_MAX_LEVELS] = {};
+    uint64_t tkeys[SVDAG_MAX_LEVELS] = {}, trep[SVDAG_MAX_LEVELS] = {}, trank[2] = {};
+    uint64_t level_nodes[SVDAG_MAX_LEVELS] = {}, level_slots_of[SVDAG_MAX_LEVELS] = {};
+    uint64_t tail_codes[2] = {}, tail_ranks[2] = {}, tail_masks = 0, tail_children = 0, tail_hash = 0;
     uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
     uint32_t group_bits = 0;
     SortPlan key_plan{}, leaf_plan{};
@@ -354,14 +361,27 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
         L.slots[i] = c.take(4 * L.level_bound);
         L.hashes[i] = c.take(8 * L.level_bound);
     }
-    L.masks = c.take(L.level_bound);
-    L.children = c.take(32 * L.level_bound);
-    L.tail_hash = c.take(8 * 8 * std::min<uint64_t>(L.level_bound, kTailMaxParents));
-    for (int i = 0; i < 2; ++i) {
-        L.tkeys[i] = c.take(8 * L.level_slots);
-        L.trep[i] = c.take(4 * L.level_slots);
-        L.trank[i] = c.take(4 * L.level_slots);
+    // Hashed levels tail_top + 1 .. depth - 3: at most min(level bound, 8^l)
+    // nodes each.
+    const int tail_top = std::min(depth - 3, kTailTopLevel);
+    uint64_t widest_slots = 1;
+    for (int l = tail_top + 1; l <= depth - 3; ++l) {
+        L.level_nodes[l] = std::min<uint64_t>(L.level_bound, pow8_capped(l, L.level_bound));
+        L.level_slots_of[l] = (1ull << table_lg(L.level_nodes[l], load_shift)) + kTableOverflow;
+        widest_slots = std::max(widest_slots, L.level_slots_of[l]);
+        L.masks[l] = c.take(L.level_nodes[l]);
+        L.children[l] = c.take(32 * L.level_nodes[l]);
+        L.tkeys[l] = c.take(8 * L.level_slots_of[l]);
+        L.trep[l] = c.take(4 * L.level_slots_of[l]);
     }
+    for (int i = 0; i < 2; ++i) L.trank[i] = c.take(4 * widest_slots);
+    for (int i = 0; i < 2; ++i) {
+        L.tail_codes[i] = c.take(8 * kTailMaxParents);
+        L.tail_ranks[i] = c.take(4 * kTailMaxParents);
+    }
+    L.tail_masks = c.take(kTailMaxParents);
+    L.tail_children = c.take(32 * kTailMaxParents);
+    L.tail_hash = c.take(8 * 8 * kTailMaxParents);
     L.total = c.off;
     // The leaf-reduce regions are read only by the compaction, before any of
     // the buffers from the leaf table on is written.
@@ -770,65 +790,61 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     if (!capture) mark(ctx, "leaf dedup");
 
     // Levels, bottom-up (DAGCompression.cu:1077-1092): hashed-table levels down
-    // to tail_top + 1, then levels tail_top..0 (at most 4096 nodes each) in one
-    // workgroup.
+    // to tail_top + 1, then levels tail_top..0 (at most 512 nodes each) in one
+    // workgroup.  A level build needs only the children's hashes (in child
+    // order) and table slots, not their ranks, so every level build is issued
+    // first; the extracts follow in order (each turns its children's slots
+    // into ranks of the level below), the first one after the leaf order.
     const int tail_top = std::min(depth - 3, kTailTopLevel);
-    // Host bounds of level l (at most 8^l nodes) size the launches of the small
-    // levels; the kernels walk the device-side counts.
-    auto nodes_bound = [&](int l) { return std::min<uint64_t>(L.level_bound, pow8_capped(l, L.level_bound)); };
-    auto slots_bound = [&](int l) {
-        return std::min<uint64_t>(L.level_slots, (1ull << table_lg(nodes_bound(l), ctx->load_shift)) + kTableOverflow);
-    };
-    if (depth - 3 > tail_top)
-        launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[0]), &c->pre[depth - 3],
-                           &c->table_log2[depth - 3], ctx->load_shift, kTableOverflow, L.level_slots);
     for (int l = depth - 3; l > tail_top; --l) {
-        const int step = depth - 3 - l;
-        const int cur = step & 1;
-        // Children: their hashes in child order (a leaf's hash is its mask; the
-        // level below wrote its nodes' hashes) and their table slots, which the
-        // extract turns into ranks.
+        const int cur = (depth - 3 - l) & 1;
         LevelBuildArgs a{};
-        const uint32_t* child_rank_of_slot;
         if (l == depth - 3) {
             a.child_codes = lp_code;
             a.child_slots = lp_slot;
-            a.child_hash_seq = lp_mask;
+            a.child_hash_seq = lp_mask;  // a leaf's hash is its mask
             a.child_count = &c->leaf_positions;
-            child_rank_of_slot = leaf_rank;
         } else {
             a.child_codes = reinterpret_cast<const uint64_t*>(A + L.codes[cur ^ 1]);
             a.child_slots = reinterpret_cast<const uint32_t*>(A + L.slots[cur ^ 1]);
             a.child_hash_seq = reinterpret_cast<const uint64_t*>(A + L.hashes[cur ^ 1]);
             a.child_count = &c->pre[l + 1];
-            child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur ^ 1]);
         }
         a.parent_hashes = reinterpret_cast<uint64_t*>(A + L.hashes[cur]);
         a.parent_codes = reinterpret_cast<uint64_t*>(A + L.codes[cur]);
         a.parent_slots = reinterpret_cast<uint32_t*>(A + L.slots[cur]);
-        a.parent_masks = reinterpret_cast<uint8_t*>(A + L.masks);
-        a.parent_children = reinterpret_cast<uint32_t*>(A + L.children);
-        a.table_keys = reinterpret_cast<uint64_t*>(A + L.tkeys[cur]);
-        a.table_rep = reinterpret_cast<uint32_t*>(A + L.trep[cur]);
+        a.parent_masks = reinterpret_cast<uint8_t*>(A + L.masks[l]);
+        a.parent_children = reinterpret_cast<uint32_t*>(A + L.children[l]);
+        a.table_keys = reinterpret_cast<uint64_t*>(A + L.tkeys[l]);
+        a.table_rep = reinterpret_cast<uint32_t*>(A + L.trep[l]);
         a.table_log2 = &c->table_log2[l];
         a.tile_counts = tile_counts;
         a.offsets = tile_offsets;
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
-        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : nodes_bound(l + 1));
+        // The table is sized from the level's node count (the build of the
+        // level below, or the leaf reduce, counted it).
+        launch_table_clear(st, a.table_keys, &c->pre[l], &c->table_log2[l], ctx->load_shift, kTableOverflow,
+                           L.level_slots_of[l]);
+        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_nodes[l + 1]);
         if (!capture) mark(ctx, "level build");
-        TRY(order_leaves());
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
-                               &c->collisions, nodes_bound(l));
+                               &c->collisions, L.level_nodes[l]);
+        if (l == depth - 3) TRY(order_leaves());
+    }
+    TRY(join());
+    const uint32_t* below = leaf_rank;  // slot -> rank map of the level below
+    for (int l = depth - 3; l > tail_top; --l) {
+        const int cur = (depth - 3 - l) & 1;
         LevelExtractArgs e{};
-        e.table_keys = a.table_keys;
-        e.table_rep = a.table_rep;
-        e.table_log2 = a.table_log2;
+        e.table_keys = reinterpret_cast<const uint64_t*>(A + L.tkeys[l]);
+        e.table_rep = reinterpret_cast<const uint32_t*>(A + L.trep[l]);
+        e.table_log2 = &c->table_log2[l];
         e.rank_of_slot = reinterpret_cast<uint32_t*>(A + L.trank[cur]);
-        e.parent_masks = a.parent_masks;
-        e.parent_children = a.parent_children;
-        e.child_rank_of_slot = child_rank_of_slot;
+        e.parent_masks = reinterpret_cast<const uint8_t*>(A + L.masks[l]);
+        e.parent_children = reinterpret_cast<const uint32_t*>(A + L.children[l]);
+        e.child_rank_of_slot = below;
         e.out_hashes = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
         e.out_masks = reinterpret_cast<uint8_t*>(O + S.masks[l]);
         e.out_children = reinterpret_cast<uint32_t*>(O + S.children[l]);
@@ -837,12 +853,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.slots_scratch = &c->table_slots;
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
-        TRY(join());
-        launch_level_extract(st, e, slots_bound(l));
+        launch_level_extract(st, e, L.level_slots_of[l]);
         if (!capture) mark(ctx, "level extract");
-        if (l - 1 > tail_top)
-            launch_table_clear(st, reinterpret_cast<uint64_t*>(A + L.tkeys[cur ^ 1]), &c->pre[l - 1],
-                               &c->table_log2[l - 1], ctx->load_shift, kTableOverflow, slots_bound(l - 1));
+        below = e.rank_of_slot;
     }
     {
         TailArgs t{};
@@ -850,24 +863,22 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         if (tail_top == depth - 3) {
             t.child_codes = lp_code;
             t.child_slots = lp_slot;
-            t.child_rank_of_slot = leaf_rank;
             t.child_hashes = leaves_out;
             t.child_count = &c->leaf_positions;
         } else {
-            const int cur = (depth - 3 - (tail_top + 1)) & 1;  // buffers of level tail_top + 1
+            const int cur = (depth - 3 - (tail_top + 1)) & 1;  // outputs of the build of level tail_top + 1
             t.child_codes = reinterpret_cast<const uint64_t*>(A + L.codes[cur]);
             t.child_slots = reinterpret_cast<const uint32_t*>(A + L.slots[cur]);
-            t.child_rank_of_slot = reinterpret_cast<const uint32_t*>(A + L.trank[cur]);
             t.child_hashes = reinterpret_cast<const uint64_t*>(O + S.hashes[tail_top + 1]);
             t.child_count = &c->pre[tail_top + 1];
         }
-        // The level tables' key and representative arrays are free by now.
+        t.child_rank_of_slot = below;
         for (int i = 0; i < 2; ++i) {
-            t.codes[i] = reinterpret_cast<uint64_t*>(A + L.tkeys[i]);
-            t.ranks[i] = reinterpret_cast<uint32_t*>(A + L.trep[i]);
+            t.codes[i] = reinterpret_cast<uint64_t*>(A + L.tail_codes[i]);
+            t.ranks[i] = reinterpret_cast<uint32_t*>(A + L.tail_ranks[i]);
         }
-        t.masks = reinterpret_cast<uint8_t*>(A + L.masks);
-        t.children = reinterpret_cast<uint32_t*>(A + L.children);
+        t.masks = reinterpret_cast<uint8_t*>(A + L.tail_masks);
+        t.children = reinterpret_cast<uint32_t*>(A + L.tail_children);
         t.child_hash = reinterpret_cast<uint64_t*>(A + L.tail_hash);
         for (int l = 0; l <= tail_top; ++l) {
             t.out_hashes[l] = reinterpret_cast<uint64_t*>(O + S.hashes[l]);
@@ -876,7 +887,6 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         }
         t.counters = c;
         t.check_collisions = ctx->check_collisions;
-        TRY(join());
         launch_level_tail(st, t);
         if (!capture) mark(ctx, "top levels");
     }

@@ -130,8 +130,8 @@ int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n,
  * process ends) and its result in the caller's device buffer d_storage, of
  * storage_bytes >= svdag_result_bytes(n, depth).  Calls no hipMalloc, hipFree
  * or allocating routine.  A build needing more scratch than the array holds
- * (svdag_scratch_bytes(n, depth, 8) at load shift 1; about 4M keys at depth 9,
- * 1M at depth 10 to 12) or more storage than given fails with SVDAG_ERR_MEMORY
+ * (svdag_scratch_bytes(n, depth, 8) at load shift 1; about 5M keys at depth 9,
+ * 1M at depth 10, 900k at depth 11, 650k at depth 12) or more storage than given fails with SVDAG_ERR_MEMORY
  * before any device work.  `out` points into d_storage and owns nothing
  * (out->storage is NULL; svdag_free_dag releases nothing).  Every call on a
  * GPU shares that GPU's array, which keeps the last call's contents (each call

@@ -59,7 +59,7 @@ def run_cases(m):
 def run_errors(m):
     from dag_compression_fast_amd import _lib
     L = _lib.lib()
-    # Past the fixed scratch: 2M keys at depth 12 need about 440 MiB; the call
+    # Past the fixed scratch: 2M keys at depth 12 need about 630 MiB; the call
     # fails on the host before any device work.
     n = 2_000_000
     d_keys = m.DeviceBuffer.from_numpy(np.zeros(n, dtype=np.uint64))
