@@ -482,6 +482,57 @@ void pool_trim(svdag_context* ctx)
     ctx->free_blocks.clear();
 }
 
+// Host storage of DAGs copied to the host: pinned blocks (copies run at the
+// link's speed, and their pages are resident), kept in a process-wide pool when
+// freed and reused for later copies; released by svdag_free_all.
+std::mutex g_host_mutex;
+std::multimap<uint64_t, void*> g_host_free;
+std::unordered_map<void*, uint64_t> g_host_size;
+
+void* host_alloc(uint64_t bytes)
+{
+    bytes = (std::max<uint64_t>(bytes, 1) + 4095) / 4096 * 4096;
+    {
+        std::lock_guard<std::mutex> lock(g_host_mutex);
+        auto it = g_host_free.lower_bound(bytes);
+        if (it != g_host_free.end() && it->first <= 2 * bytes) {
+            void* p = it->second;
+            g_host_free.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_host_mutex);
+    g_host_size[p] = bytes;
+    return p;
+}
+
+void host_release(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_host_mutex);
+    const auto it = g_host_size.find(p);
+    if (it == g_host_size.end()) {
+        std::free(p);  // not a pool block
+        return;
+    }
+    g_host_free.emplace(it->second, p);
+}
+
+void host_trim()
+{
+    std::lock_guard<std::mutex> lock(g_host_mutex);
+    for (auto& kv : g_host_free) {
+        (void)hipHostFree(kv.second);
+        g_host_size.erase(kv.second);
+    }
+    g_host_free.clear();
+}
+
 void empty_dag(svdag_dag* out, int num_levels, int location)
 {
     std::memset(out, 0, sizeof(*out));
@@ -759,9 +810,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         ls = ctx->side;
         forked = true;
     }
-    // The sort's launches are issued after the first level build's: the side
-    // stream already waits for the leaf dedup (fork_ev), and issuing the
-    // build first keeps the main stream busy while the host issues the sort's
+    // The sort's launches are issued after the first level builds' (below):
+    // the side stream already waits for the leaf dedup (fork_ev), and issuing
+    // builds first keeps the main stream busy while the host issues the sort's
     // two dozen small launches.
     bool ordered = false;
     auto order_leaves = [&]() -> int {
@@ -831,7 +882,11 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
                                &c->collisions, L.level_nodes[l]);
-        if (l == depth - 3) TRY(order_leaves());
+        // The unique-leaf sort's two dozen small launches go out once the two
+        // largest level builds are queued: the main stream then has enough
+        // work not to wait while the host issues them, and the sort starts
+        // early enough on the side stream to end beside the builds.
+        if (l == std::max(depth - 4, tail_top + 1)) TRY(order_leaves());
     }
     TRY(join());
     const uint32_t* below = leaf_rank;  // slot -> rank map of the level below
@@ -1397,6 +1452,19 @@ int svdag_create_dag_batched_gather(svdag_context* ctx, const uint64_t* const* d
         if (offsets[i + 1] >= (1ull << 30)) return set_error(SVDAG_ERR_ARGUMENT, "more than 2^30 - 1 keys");
     }
     const uint64_t n = offsets[num_chunks];
+    // Chunks that already sit back to back in chunk order need no copy.
+    uintptr_t base = 0;
+    bool have_base = false, contiguous = true;
+    for (uint64_t i = 0; i < num_chunks && contiguous; ++i) {
+        if (!counts[i]) continue;
+        const uintptr_t b = reinterpret_cast<uintptr_t>(d_chunk_keys[i]) - 8 * offsets[i];
+        if (!have_base) base = b;
+        have_base = true;
+        contiguous = b == base;
+    }
+    if (contiguous)
+        return svdag_create_dag_batched(ctx, have_base ? reinterpret_cast<const uint64_t*>(base) : nullptr,
+                                        offsets.data(), num_chunks, chunk_depth, out);
     void* keys = nullptr;
     if (n) {
         // The chunks' keys back to back, in chunk order, in a pooled block
@@ -1528,7 +1596,7 @@ int svdag_dag_to_host(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out)
     if (!dag || !out) return set_error(SVDAG_ERR_ARGUMENT, "null argument");
     uint64_t total = 8 * dag->num_leaves + 8;
     for (int l = 0; l < dag->num_levels; ++l) total += 41 * dag->levels[l].num_nodes + 32;
-    char* mem = static_cast<char*>(std::malloc(total));
+    char* mem = static_cast<char*>(host_alloc(total));
     if (!mem) return set_error(SVDAG_ERR_MEMORY, "host allocation failed");
     *out = *dag;
     out->location = SVDAG_HOST;
@@ -1550,7 +1618,7 @@ int svdag_dag_to_host(svdag_context* ctx, const svdag_dag* dag, svdag_dag* out)
     out->leaves = static_cast<uint64_t*>(copy(dag->leaves, 8 * dag->num_leaves));
     const hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        std::free(mem);
+        host_release(mem);
         out->storage = nullptr;
         return set_error(SVDAG_ERR_HIP, "copy to host failed: %s", hipGetErrorString(e));
     }
@@ -1624,7 +1692,7 @@ int svdag_free_dag(svdag_context* ctx, svdag_dag* dag)
             (void)hipSetDevice(ctx->device);
             device_release(ctx, dag->storage);
         } else {
-            std::free(dag->storage);
+            host_release(dag->storage);
         }
     }
     std::memset(dag, 0, sizeof(*dag));
@@ -1646,6 +1714,7 @@ int svdag_free_all(svdag_context* ctx)
     ctx->arena = nullptr;
     ctx->arena_cap = 0;
     pool_trim(ctx);
+    host_trim();
     return 0;
 }
 

@@ -365,6 +365,26 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
     return 0;
 }
 
+// Per-wave election among lanes holding equal 64-bit keys: every valid lane
+// writes its lane index into the wave's LDS bucket (kElectBuckets words) for
+// its key, one write wins; a lane whose bucket winner holds the same key is a
+// follower of that winner and returns it, other lanes return themselves.
+// Every lane of the wave must call it.
+constexpr uint32_t kElectBuckets = 128;
+
+__device__ __forceinline__ uint32_t elect_leader(uint64_t key, bool valid, uint32_t* buckets, bool& follower)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t bucket = uint32_t(fmix64(key)) & (kElectBuckets - 1);
+    if (valid) buckets[bucket] = lane;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t winner = valid ? buckets[bucket] : lane;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t winner_key = __shfl(key, int(winner), kWave);
+    follower = valid && winner != lane && winner_key == key;
+    return follower ? winner : lane;
+}
+
 // Every leaf position's mask (never 0) enters the table and records its slot.
 // A mask shared by many leaves (all-ones inside a solid, the same few patterns
 // along a flat surface) would otherwise turn into a stream of probes and
@@ -375,7 +395,6 @@ __device__ __forceinline__ uint32_t hashed_insert(uint64_t* __restrict__ tk, uin
 // bucket winner holds the same mask takes the winner's slot instead of probing.
 // Lanes that lose their bucket to another mask probe themselves.
 constexpr int kInsertIpt = 4;
-constexpr uint32_t kElectBuckets = 128;
 
 __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __restrict__ lp_mask,
                                                           const uint64_t* __restrict__ count_ptr,
@@ -399,19 +418,14 @@ __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __rest
             const uint64_t q = r0 + uint64_t(j) * kThreads + lane;
             const bool valid = q < n;
             const uint64_t mask = valid ? lp_mask[q] : 0ull;
-            const uint32_t bucket = uint32_t(fmix64(mask)) & (kElectBuckets - 1);
-            if (valid) elect[bucket] = lane;
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t winner = valid ? elect[bucket] : lane;
-            __builtin_amdgcn_wave_barrier();
-            const uint64_t winner_mask = __shfl(mask, int(winner), kWave);
-            const bool follower = valid && winner != lane && winner_mask == mask;
+            bool follower;
+            const uint32_t leader = elect_leader(mask, valid, elect, follower);
             uint32_t slot = 0;
             if (valid && !follower) {
                 bool won;
                 slot = hashed_insert(tk, lg, mask, won, err);
             }
-            slot = uint32_t(__shfl(int(slot), int(follower ? winner : lane), kWave));
+            slot = uint32_t(__shfl(int(slot), int(leader), kWave));
             if (valid) lp_slot[q] = slot;
         }
     }
@@ -683,6 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
     __shared__ uint64_t s_hash[kBuildTile];
+    __shared__ uint32_t s_elect[kThreads / kWave][kElectBuckets];
     const uint64_t n = *g.child_count;
     const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
     const uint32_t lg = *g.table_log2;
@@ -749,9 +764,19 @@ __global__ __launch_bounds__(kThreads) void k_level_build(LevelBuildArgs g)
             s_hash[q] = acc_emit(acc, p0 + q, g);
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < tile_parents; k += kThreads) {
-            const uint64_t p = p0 + k;
-            g.parent_slots[p] = ordered_insert(g.table_keys, g.table_rep, lg, limit, s_hash[k], uint32_t(p), g.counters);
+        // Equal parents (identical subtrees) often sit side by side: one lane
+        // per distinct hash in a wave probes the table, the others take its slot.
+        for (uint32_t k0 = 0; k0 < tile_parents; k0 += kThreads) {
+            const uint32_t k = k0 + threadIdx.x;
+            const bool valid = k < tile_parents;
+            const uint64_t h = valid ? s_hash[k] : 0ull;
+            bool follower;
+            const uint32_t leader = elect_leader(h, valid, s_elect[threadIdx.x / kWave], follower);
+            uint32_t slot = 0;
+            if (valid && !follower)
+                slot = ordered_insert(g.table_keys, g.table_rep, lg, limit, h, uint32_t(p0 + k), g.counters);
+            slot = uint32_t(__shfl(int(slot), int(leader), kWave));
+            if (valid) g.parent_slots[p0 + k] = slot;
         }
         __syncthreads();  // s_hash is rewritten by the next tile
     }
@@ -1334,20 +1359,39 @@ __global__ __launch_bounds__(kCountThreads) void k_batch_count(const uint64_t* _
     }
 }
 
-// Keys of several device arrays, back to back: thread i finds its source by a
-// binary search over the destination starts.
-__global__ void k_gather_keys(const GatherSource* __restrict__ src, uint32_t nsrc, uint64_t total,
-                              uint64_t* __restrict__ dst)
+// Keys of several device arrays, back to back.  A workgroup copies a tile of
+// kGatherTile destination keys: one binary search finds the tile's first
+// source, then the sources overlapping the tile are copied in turn, each as a
+// contiguous, coalesced range.
+constexpr uint32_t kGatherTile = 4096;
+
+__global__ __launch_bounds__(kThreads) void k_gather_keys(const GatherSource* __restrict__ src, uint32_t nsrc,
+                                                          uint64_t total, uint64_t* __restrict__ dst)
 {
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-        uint32_t lo = 0, hi = nsrc;  // src[lo].start <= i < src[hi].start
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (src[mid].start <= i) lo = mid;
-            else hi = mid;
+    __shared__ uint32_t s_first;
+    const uint64_t ntiles = (total + kGatherTile - 1) / kGatherTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t t0 = tile * kGatherTile;
+        const uint64_t t1 = min(t0 + kGatherTile, total);
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = nsrc;  // src[lo].start <= t0 < src[hi].start
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (src[mid].start <= t0) lo = mid;
+                else hi = mid;
+            }
+            s_first = lo;
         }
-        dst[i] = __builtin_nontemporal_load(src[lo].keys + (i - src[lo].start));
+        __syncthreads();
+        uint32_t sidx = s_first;
+        for (uint64_t pos = t0; pos < t1; ++sidx) {
+            const GatherSource g = src[sidx];
+            const uint64_t end = min(t1, sidx + 1 < nsrc ? src[sidx + 1].start : total);
+            for (uint64_t i = pos + threadIdx.x; i < end; i += kThreads)
+                dst[i] = __builtin_nontemporal_load(g.keys + (i - g.start));
+            pos = end;
+        }
+        __syncthreads();  // s_first is rewritten by the next tile
     }
 }
 
@@ -1629,8 +1673,8 @@ void launch_gather_keys(hipStream_t s, const GatherSource* src, uint32_t nsrc, u
 {
     if (total == 0 || nsrc == 0) return;
     TimedLaunch t(kKFill, s);
-    hipLaunchKernelGGL(k_gather_keys, dim3(grid_for((total + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, src,
-                       nsrc, total, dst);
+    hipLaunchKernelGGL(k_gather_keys, dim3(grid_for((total + kGatherTile - 1) / kGatherTile, 8)), dim3(kThreads), 0, s,
+                       src, nsrc, total, dst);
 }
 
 void launch_resident_roots(hipStream_t s, const ResidentRootArgs& a)

@@ -66,7 +66,7 @@ class ChunkServer:
         self._own_ctx = ctx is None
         self.clock = clock
         self.resident = ResidentDag(self.chunk_depth, self.slot_levels, max_nodes, ctx=self.ctx)
-        self.budget = int(budget_bytes) if budget_bytes else self.footprint(1 << 26)
+        self.budget = int(budget_bytes) if budget_bytes else self.footprint(1 << 27)
         self.rounds = 0
 
     def footprint(self, n):

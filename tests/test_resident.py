@@ -106,14 +106,32 @@ def run_gather(m, oracle):
     chunks = [T.clustered_keys(rng, CHUNK_DEPTH, int(rng.integers(0, 3000))) if i % 5 else np.zeros(0, np.uint64)
               for i in range(8)]
     bufs = [m.DeviceBuffer.from_numpy(c) if len(c) else None for c in chunks]
+    exp = oracle.merge_dags([oracle.create_dag(c, CHUNK_DEPTH) for c in chunks])
     try:
         got = m.create_dag_batched_gather(bufs, CHUNK_DEPTH)
-        exp = oracle.merge_dags([oracle.create_dag(c, CHUNK_DEPTH) for c in chunks])
         assert_dag_equal(got.to_numpy(), exp, "gathered batched build")
     finally:
         for b in bufs:
             if b is not None:
                 b.free()
+    # Chunks already back to back in one buffer (no copy), and out of order
+    # in it (copied into chunk order).
+    whole = m.DeviceBuffer.from_numpy(np.concatenate(chunks))
+    try:
+        starts = np.concatenate([[0], np.cumsum([len(c) for c in chunks])])
+        views = [m.DeviceView(whole, 8 * int(starts[i]), len(c)) if len(c) else None for i, c in enumerate(chunks)]
+        assert_dag_equal(m.create_dag_batched_gather(views, CHUNK_DEPTH).to_numpy(), exp, "contiguous chunks")
+        rev = np.concatenate(chunks[::-1])
+        rstarts = np.concatenate([[0], np.cumsum([len(c) for c in chunks[::-1]])])
+        whole.free()
+        whole = m.DeviceBuffer.from_numpy(rev)
+        views = [None] * len(chunks)
+        for j, c in enumerate(chunks[::-1]):
+            if len(c):
+                views[len(chunks) - 1 - j] = m.DeviceView(whole, 8 * int(rstarts[j]), len(c))
+        assert_dag_equal(m.create_dag_batched_gather(views, CHUNK_DEPTH).to_numpy(), exp, "chunks out of order")
+    finally:
+        whole.free()
 
 
 @pytest.fixture(scope="module")
