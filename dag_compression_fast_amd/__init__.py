@@ -107,6 +107,13 @@ class Context:
     def set_option(self, option, value):
         _lib.check(_lib.lib().svdag_set_option(self.handle, option, int(value)))
 
+    def memory_usage(self):
+        """Device bytes held: {"scratch", "results" (owned by live DAGs),
+        "cached" (freed result blocks kept for reuse)}."""
+        v = [ctypes.c_uint64() for _ in range(3)]
+        _lib.check(_lib.lib().svdag_memory_usage(self.handle, *(ctypes.byref(x) for x in v)))
+        return {"scratch": v[0].value, "results": v[1].value, "cached": v[2].value}
+
     def free_all(self):
         _lib.check(_lib.lib().svdag_free_all(self.handle))
 

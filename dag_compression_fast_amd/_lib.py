@@ -144,6 +144,7 @@ def bind_library(L):
         "svdag_free_dag": ([vp, dagp], i32),
         "svdag_free_words": ([vp, vp], i32),
         "svdag_free_all": ([vp], i32),
+        "svdag_memory_usage": ([vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)], i32),
         "svdag_last_error": ([], ctypes.c_char_p),
         "svdag_profile_enable": ([vp, u64], i32),
         "svdag_profile_reset": ([vp], i32),

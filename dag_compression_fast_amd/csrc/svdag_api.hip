@@ -414,6 +414,13 @@ DagStorage plan_storage(int num_levels, const uint64_t* caps, uint64_t leaf_cap)
     return S;
 }
 
+void pool_trim(svdag_context* ctx);
+
+// Device memory runs out, or the context's budget would be passed: like the
+// reference's allocator (src/Memory.cpp:56-70, with the pool's fallback at
+// src/DAGCompression.cu:85-89, 178-189), the cached free result blocks are
+// released and the allocation is tried once more; if that fails too, the call
+// returns SVDAG_ERR_MEMORY (the reference aborts).
 int ensure_arena(svdag_context* ctx, uint64_t bytes)
 {
     if (ctx->budget && bytes > ctx->budget)
@@ -424,14 +431,23 @@ int ensure_arena(svdag_context* ctx, uint64_t bytes)
     if (ctx->arena) HIP_TRY(hipFree(ctx->arena));
     ctx->arena = nullptr;
     ctx->arena_cap = 0;
-    const uint64_t cap = align_up(bytes + bytes / 8);
-    if (hipMalloc(reinterpret_cast<void**>(&ctx->arena), cap) != hipSuccess) {
+    // Some headroom for the next, slightly larger call, within the budget.
+    uint64_t cap = align_up(bytes + bytes / 8);
+    if (ctx->budget) cap = std::max(align_up(bytes), std::min(cap, ctx->budget / kAlign * kAlign));
+    if (ctx->budget && cap + ctx->pool_bytes > ctx->budget) pool_trim(ctx);
+    if (ctx->budget && cap + ctx->pool_bytes > ctx->budget)
+        return set_error(SVDAG_ERR_MEMORY, "%llu bytes of scratch and %llu of results exceed the budget of %llu",
+                         (unsigned long long)cap, (unsigned long long)ctx->pool_bytes, (unsigned long long)ctx->budget);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (hipMalloc(reinterpret_cast<void**>(&ctx->arena), cap) == hipSuccess) {
+            ctx->arena_cap = cap;
+            return 0;
+        }
         (void)hipGetLastError();
         ctx->arena = nullptr;
-        return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device scratch", (unsigned long long)cap);
+        if (attempt == 0) pool_trim(ctx);
     }
-    ctx->arena_cap = cap;
-    return 0;
+    return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device scratch", (unsigned long long)cap);
 }
 
 // Result allocation from the context's block pool: a free block of at least
@@ -446,16 +462,20 @@ int device_alloc(svdag_context* ctx, void** p, uint64_t bytes)
         ctx->free_blocks.erase(it);
         return 0;
     }
+    if (ctx->budget && ctx->arena_cap + ctx->pool_bytes + bytes > ctx->budget) pool_trim(ctx);
     if (ctx->budget && ctx->arena_cap + ctx->pool_bytes + bytes > ctx->budget)
         return set_error(SVDAG_ERR_MEMORY, "result block of %llu bytes exceeds the memory budget", (unsigned long long)bytes);
-    if (hipMalloc(p, bytes) != hipSuccess) {
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (hipMalloc(p, bytes) == hipSuccess) {
+            ctx->block_size[*p] = bytes;
+            ctx->pool_bytes += bytes;
+            return 0;
+        }
         (void)hipGetLastError();
         *p = nullptr;
-        return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device memory", (unsigned long long)bytes);
+        if (attempt == 0) pool_trim(ctx);
     }
-    ctx->block_size[*p] = bytes;
-    ctx->pool_bytes += bytes;
-    return 0;
+    return set_error(SVDAG_ERR_MEMORY, "cannot allocate %llu bytes of device memory", (unsigned long long)bytes);
 }
 
 // Returns a result block to the pool (stream order keeps later reuse safe).
@@ -472,8 +492,13 @@ void device_release(svdag_context* ctx, void* p)
     ctx->free_blocks.emplace(it->second, p);
 }
 
+// Frees the cached free result blocks (work in flight may still use them
+// until the context's streams drain).
 void pool_trim(svdag_context* ctx)
 {
+    if (ctx->free_blocks.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto& kv : ctx->free_blocks) {
         (void)hipFree(kv.second);
         ctx->pool_bytes -= ctx->block_size[kv.second];
@@ -1703,6 +1728,17 @@ int svdag_free_words(svdag_context* ctx, uint32_t* d_words)
 {
     TRY(check_ctx(ctx));
     device_release(ctx, d_words);
+    return 0;
+}
+
+int svdag_memory_usage(svdag_context* ctx, uint64_t* scratch_bytes, uint64_t* result_bytes, uint64_t* cached_bytes)
+{
+    if (!ctx) return set_error(SVDAG_ERR_ARGUMENT, "null context");
+    uint64_t cached = 0;
+    for (const auto& kv : ctx->free_blocks) cached += kv.first;
+    if (scratch_bytes) *scratch_bytes = ctx->arena_cap;
+    if (result_bytes) *result_bytes = ctx->pool_bytes - cached;
+    if (cached_bytes) *cached_bytes = cached;
     return 0;
 }
 

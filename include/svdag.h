@@ -208,8 +208,16 @@ int svdag_free_dag(svdag_context* ctx, svdag_dag* dag);
 int svdag_free_words(svdag_context* ctx, uint32_t* d_words);
 
 /* DAGCompression::FreeAll (src/DAGCompression.cu:1612-1615): release the
- * context's scratch arena. */
+ * context's scratch arena and its cached result blocks. */
 int svdag_free_all(svdag_context* ctx);
+
+/* Device memory a context holds: its scratch arena, result blocks owned by
+ * live DAGs and word arrays, and freed result blocks kept for reuse (these
+ * are released, and the allocation retried once, when an allocation fails or
+ * would pass SVDAG_OPT_MEMORY_BUDGET: the reference's out-of-memory fallback,
+ * src/Memory.cpp:56-70). */
+int svdag_memory_usage(svdag_context* ctx, uint64_t* scratch_bytes, uint64_t* result_bytes,
+                       uint64_t* cached_bytes);
 
 /* --- resident global DAG (served mode) ----------------------------------- */
 

@@ -3,6 +3,9 @@
 #include "hip/hip_runtime.h"
 
 #include <signal.h>
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/asan_interface.h>
+#endif
 
 #include <atomic>
 #include <cstdint>
@@ -136,6 +139,11 @@ Wave& my_wave() { return g_waves[flat_tid() / 64]; }
 // stack aligned as after a call.
 void* fresh_stack(char* base)
 {
+#if defined(__SANITIZE_ADDRESS__)
+    // A fiber that ended never returned from its frames: under AddressSanitizer
+    // their stack redzones are still poisoned when the stack is reused.
+    __asan_unpoison_memory_region(base, kStack);
+#endif
     uintptr_t top = (reinterpret_cast<uintptr_t>(base) + kStack) & ~uintptr_t(15);
     void** sp = reinterpret_cast<void**>(top - 64);
     for (int i = 0; i < 6; ++i) sp[i] = nullptr;  // r15 r14 r13 r12 rbx rbp
@@ -161,6 +169,9 @@ void on_fault(int sig, siginfo_t* info, void*)
 
 void install_fault_reporter()
 {
+#if defined(__SANITIZE_ADDRESS__)
+    return;  // AddressSanitizer reports faults itself and owns the signal stack
+#endif
     thread_local bool done = false;
     if (done) return;
     done = true;

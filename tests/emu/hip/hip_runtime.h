@@ -19,7 +19,7 @@
 
 #define __global__
 #define __device__
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define HIP_SYMBOL(x) (&(x))
 #define __host__
 #define __forceinline__ inline
