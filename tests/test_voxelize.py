@@ -6,9 +6,9 @@ floating-point steps in the same order, no fused multiply-adds.  Checked as
 sets (integer output: tolerance 0) on seeded soups, on hand-made edge cases
 (axis-aligned, degenerate, outside or straddling the grid), and at the full
 2048^3 workload on the GPU, where the DAG built from the device keys must equal
-the one built from the host keys.  The host rule itself is pinned by the
-decode checks of the workloads it produces (every emitted voxel is a grid
-voxel; the triangle-surface workload equals this rule over its triangles).
+the one built from the host keys.  The host rule itself is pinned against its
+geometric definition, exhaustively in exact arithmetic and by closed forms, in
+tests/test_voxel_rule.py.
 """
 import numpy as np
 import pytest
