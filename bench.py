@@ -758,6 +758,8 @@ def main():
     ap.add_argument("--kernel-table", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--radix-bits", type=int, default=None, help="widest key-sort digit (library default if unset)")
     ap.add_argument("--sort-tile", type=int, default=None, help="keys per key-sort tile, 4096 or 8192")
+    ap.add_argument("--top-digit-first", type=int, default=None, choices=(0, 1, 2),
+                    help="key sort route of plain 64-bit builds: 0 LSD, 1 auto (library default), 2 top digit first")
     ap.add_argument("--no-chunks", action="store_true", help="skip the 512-chunk batched/served comparison")
     ap.add_argument("--no-noise", action="store_true", help="skip the value-noise workloads")
     ap.add_argument("--no-voxelize", action="store_true", help="skip the device voxeliser timing")
@@ -803,6 +805,8 @@ def main():
         ctx.set_option(_lib.OPT_RADIX_BITS, args.radix_bits)
     if args.sort_tile is not None:
         ctx.set_option(_lib.OPT_SORT_TILE, args.sort_tile)
+    if args.top_digit_first is not None:
+        ctx.set_option(_lib.OPT_TOP_DIGIT_FIRST, args.top_digit_first)
     L = _lib.lib()
     d_keys = svd.DeviceBuffer.from_numpy(keys)
     H = _lib.hip()

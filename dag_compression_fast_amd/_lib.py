@@ -28,6 +28,7 @@ OPT_SORT_TILE = 7
 OPT_TABLE_ESTIMATE = 8
 OPT_VOXEL_STAGE_BYTES = 9
 OPT_SEG_FIRST_BITS = 10
+OPT_TOP_DIGIT_FIRST = 11
 
 STATUS_NAMES = {0: "OK", 1: "ARGUMENT", 2: "HIP", 3: "MEMORY", 4: "INTERNAL", 5: "CAPACITY"}
 

@@ -1,8 +1,9 @@
 """Autotuner for the build's speed options.
 
 The key-sort digit width (``SVDAG_OPT_RADIX_BITS``), the key-sort tile
-(``SVDAG_OPT_SORT_TILE``), the digit split of the batched build's per-chunk
-sort (``SVDAG_OPT_SEG_FIRST_BITS``) and the dedup table load factor
+(``SVDAG_OPT_SORT_TILE``), the key sort's route (``SVDAG_OPT_TOP_DIGIT_FIRST``),
+the digit split of the batched build's per-chunk sort
+(``SVDAG_OPT_SEG_FIRST_BITS``) and the dedup table load factor
 (``SVDAG_OPT_TABLE_LOAD_SHIFT``) change how fast a build runs, never what it
 produces.  ``autotune`` times a caller-supplied build under every combination
 of a search space on one context, checks that every combination gives the same
@@ -12,7 +13,7 @@ and workload, so a later run starts from the stored choice.
 
 Which options matter depends on the build: a plain build (and a batched build
 whose chunk-local bits above the leaf groups exceed 12, chunk depth >= 10)
-runs the global key sort, whose digit width and tile are tuned; a batched build
+runs the global key sort, whose digit width, tile and route are tuned; a batched build
 of chunk depth <= 9 sorts inside each chunk, where only the digit split of that
 sort and the table load factor have an effect (``space_for``).
 
@@ -31,6 +32,7 @@ from . import _lib
 SPACE = {
     "radix_bits": (7, 8, 9),
     "sort_tile": (4096, 8192),
+    "top_digit_first": (0, 1),
     "table_load_shift": (1, 2),
 }
 
@@ -46,9 +48,10 @@ _OPTION_IDS = {
     "sort_tile": _lib.OPT_SORT_TILE,
     "table_load_shift": _lib.OPT_TABLE_LOAD_SHIFT,
     "seg_first_bits": _lib.OPT_SEG_FIRST_BITS,
+    "top_digit_first": _lib.OPT_TOP_DIGIT_FIRST,
 }
 
-DEFAULTS = {"radix_bits": 9, "sort_tile": 4096, "table_load_shift": 1, "seg_first_bits": 0}
+DEFAULTS = {"radix_bits": 9, "sort_tile": 4096, "table_load_shift": 1, "seg_first_bits": 0, "top_digit_first": 1}
 
 # The key sort skips a leaf's 6 voxel bits and its 9-bit place in its block of
 # 512 leaves; the batched build sorts inside each chunk when at most this many

@@ -147,6 +147,7 @@ struct svdag_context {
     int radix_bits = 9;
     int sort_ipt = kSortIptSmall;
     uint32_t seg_first_bits = 0;
+    int top_first = kTopFirstAuto;
     bool table_estimate = true;
     uint64_t vox_stage_max = kVoxStageMaxBytes;
     int host_threads = 1;
@@ -310,10 +311,18 @@ struct BuildLayout {
     uint64_t n = 0, leaf_bound = 0, level_bound = 0, leaf_slots = 0, level_slots = 0;
     uint32_t group_bits = 0;
     SortPlan key_plan{}, leaf_plan{};
+    // Plain builds of 64-bit keys may sort top digit first
+    // (launch_top_digit_sort): the 32-bit bucket-local keys over keys_b, the
+    // bucket starts and the segmented sort's tables over the buffers from
+    // lp_code on (nothing there is live before the leaf reduce).
+    bool top_first = false;
+    uint32_t top_shift = 0, top_bits = 0;
+    uint64_t bucket_start = 0, seg_tiles = 0, seg_ct = 0, seg_counts = 0, seg_totals = 0, seg_starts = 0,
+             seg_ntiles = 0;
 };
 
 BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift, bool batched, int radix_bits,
-                       int sort_ipt)
+                       int sort_ipt, int top_first = kTopFirstNever)
 {
     BuildLayout L;
     L.n = n;
@@ -389,6 +398,28 @@ BuildLayout plan_build(uint64_t n, int depth, int key_bytes, uint32_t load_shift
     L.reg_code = L.leaf_table;
     L.reg_mask = L.reg_code + region_bytes;
     L.total = std::max(L.total, L.reg_mask + region_bytes);
+    // Top digit first: a digit of radix_bits at the top of the code, whose
+    // bucket-local keys fit 32 bits, with at most kSegMaxBits sorted below it.
+    const uint32_t lo = L.key_plan.shift[0];
+    const uint32_t top_shift = uint32_t(3 * depth - radix_bits);
+    L.top_first = top_first != kTopFirstNever && !batched && key_bytes == 8 && L.key_plan.passes >= 2 &&
+                  (top_first == kTopFirstAlways || n >= kTopFirstMinKeys) && top_shift <= 32 && top_shift > lo &&
+                  top_shift - lo <= kSegMaxBits;
+    if (L.top_first) {
+        L.top_shift = top_shift;
+        L.top_bits = uint32_t(radix_bits);
+        Carver t;
+        t.off = L.lp_code;
+        const uint64_t tb = seg_tile_bound(n, kRadix);
+        L.bucket_start = t.take(8 * (kRadix + 1));
+        L.seg_tiles = t.take(sizeof(SegTile) * tb);
+        L.seg_ct = t.take(4 * (kRadix + 1));
+        L.seg_counts = t.take(4 * (tb << kMaxSegDigitBits));
+        L.seg_totals = t.take(4 * (uint64_t(kRadix) << kMaxSegDigitBits));
+        L.seg_starts = t.take(4 * (uint64_t(kRadix) << kMaxSegDigitBits));
+        L.seg_ntiles = t.take(8);
+        L.total = std::max(L.total, t.off);
+    }
     return L;
 }
 
@@ -693,7 +724,7 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     if (n == 0) return 0;  // DAGCompression.cu:1010-1013
 
     const BuildLayout L = plan_build(n, depth, int(sizeof(K)), ctx->load_shift, batch != nullptr, ctx->radix_bits,
-                                     ctx->sort_ipt);
+                                     ctx->sort_ipt, ctx->top_first);
     const BatchLayout B = plan_batch(L, n, batch);
     const uint64_t scratch_need = L.total + B.total;
     if (fixed && scratch_need > fixed->arena_bytes)
@@ -778,8 +809,25 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     } else {
         K* ka = reinterpret_cast<K*>(A + L.keys_a);
         K* kb = reinterpret_cast<K*>(A + L.keys_b);
-        where = launch_sort<K>(st, L.key_plan, d_keys, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n, n,
-                               reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or);
+        where = -1;
+        if constexpr (sizeof(K) == 8) {
+            if (L.top_first) {
+                SegScratch sc;
+                sc.tiles = reinterpret_cast<SegTile*>(A + L.seg_tiles);
+                sc.chunk_tiles = reinterpret_cast<uint32_t*>(A + L.seg_ct);
+                sc.counts = reinterpret_cast<uint32_t*>(A + L.seg_counts);
+                sc.totals = reinterpret_cast<uint32_t*>(A + L.seg_totals);
+                sc.starts = reinterpret_cast<uint32_t*>(A + L.seg_starts);
+                sc.ntiles = reinterpret_cast<uint64_t*>(A + L.seg_ntiles);
+                where = launch_top_digit_sort(st, d_keys, reinterpret_cast<uint32_t*>(kb), ka, kb, L.key_plan.ipt,
+                                              L.top_shift, L.top_bits, L.key_plan.shift[0], &c->n, n,
+                                              reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts,
+                                              &c->key_or, reinterpret_cast<uint64_t*>(A + L.bucket_start), sc, err);
+            }
+        }
+        if (!L.top_first)
+            where = launch_sort<K>(st, L.key_plan, d_keys, nullptr, ka, kb, nullptr, nullptr, nullptr, nullptr, &c->n,
+                                   n, reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts, &c->key_or);
         sorted = where == 0 ? static_cast<const void*>(ka) : static_cast<const void*>(kb);
     }
     if (where < 0) return fail(set_error(SVDAG_ERR_INTERNAL, "sort plan failed"));
@@ -1354,6 +1402,11 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
         if (value != 4096 && value != 8192) return set_error(SVDAG_ERR_ARGUMENT, "sort tile must be 4096 or 8192 keys");
         ctx->sort_ipt = value == 8192 ? kSortIptLarge : kSortIptSmall;
         return 0;
+    case SVDAG_OPT_TOP_DIGIT_FIRST:
+        if (value < kTopFirstNever || value > kTopFirstAlways)
+            return set_error(SVDAG_ERR_ARGUMENT, "top digit first must be 0 (never), 1 (auto) or 2 (always)");
+        ctx->top_first = int(value);
+        return 0;
     default: return set_error(SVDAG_ERR_ARGUMENT, "unknown option %d", option);
     }
 }
@@ -1361,8 +1414,13 @@ int svdag_set_option(svdag_context* ctx, int32_t option, int64_t value)
 uint64_t svdag_scratch_bytes(uint64_t n, int32_t depth, int32_t key_bytes)
 {
     if (depth < 3 || depth > 21 || n == 0) return 0;
-    // Narrowest digits: the most passes and the largest histogram region.
-    return plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false, 4, kSortIptSmall).total;
+    // The largest layout over the digit widths (narrow digits: more passes and
+    // histograms; wide ones: the top-digit-first tables).
+    uint64_t most = 0;
+    for (int bits = 4; bits <= kMaxRadixBits; ++bits)
+        most = std::max(most, plan_build(n, depth, key_bytes == 4 ? 4 : 8, 1, false, bits, kSortIptSmall,
+                                         kTopFirstAlways).total);
+    return most;
 }
 
 int svdag_create_dag(svdag_context* ctx, const uint64_t* d_keys, uint64_t n, int32_t depth, svdag_dag* out)

@@ -144,8 +144,13 @@ struct SegTile {
 };
 constexpr uint32_t kSegTileItems = 4096;
 constexpr uint32_t kSegMaxBits = 12;
-constexpr uint32_t kMaxSegDigitBits = 8;
+constexpr uint32_t kMaxSegDigitBits = 9;
 inline uint64_t seg_tile_bound(uint64_t n_bound, uint64_t num_chunks) { return n_bound / kSegTileItems + num_chunks; }
+// When plain builds of 64-bit keys sort top digit first (option
+// SVDAG_OPT_TOP_DIGIT_FIRST): never, from kTopFirstMinKeys keys on, always
+// (where the bits fit).
+constexpr int kTopFirstNever = 0, kTopFirstAuto = 1, kTopFirstAlways = 2;
+constexpr uint64_t kTopFirstMinKeys = 1ull << 20;
 // Scratch of the segmented sort: tiles (seg_tile_bound entries), chunk_tiles
 // (num_chunks + 1), counts (tile bound << kMaxSegDigitBits), totals and starts
 // (num_chunks << kMaxSegDigitBits each), the tile count.
@@ -157,10 +162,25 @@ struct SegScratch {
     uint32_t* starts;
     uint64_t* ntiles;
 };
-int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+template <typename KI>
+int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_a, uint64_t* keys_b,
                           const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
                           const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits = 0);
+
+// Key sort of a plain build of 64-bit keys on bits [lo, top_shift + top_bits),
+// top digit first: a pass on the top digit writes each key's bits below
+// top_shift (< 2^32) into `local`, grouped by top digit, with the groups'
+// starts in bucket_start (2^top_bits + 1 entries); the segmented sort then
+// sorts [lo, top_shift) inside each group and restores the top digit.  zero:
+// kRadix u32, zero at launch; counts: sort_count_words(n_bound) u32.  Returns
+// as launch_segmented_sort (0: keys_a, 1: keys_b), -1 when the bits do not fit
+// (top_shift > 32, or more than kSegMaxBits below it).
+int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint64_t* keys_a,
+                          uint64_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
+                          const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
+                          unsigned long long* key_or, uint64_t* bucket_start, const SegScratch& sc,
+                          unsigned long long* err);
 
 template <typename K>
 int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, const uint32_t* vals_in,

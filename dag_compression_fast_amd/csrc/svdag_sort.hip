@@ -304,8 +304,10 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict
 // stably.
 enum RankMode { kRankUnstable = 0, kRankStable = 1, kRankAdaptive = 2 };
 
-template <typename K, bool PAIRS, int MODE, int IPT, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+// KO narrower than K (the top-digit pass of a 64-bit key sort): each key is
+// written without its digit and the bits above it, as a key local to its digit.
+template <typename K, bool PAIRS, int MODE, int IPT, int THREADS, typename KO = K>
+__global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ keys_in, KO* __restrict__ keys_out,
                                                                const uint32_t* __restrict__ vals_in,
                                                                uint32_t* __restrict__ vals_out,
                                                                const uint64_t* __restrict__ n_ptr, uint32_t shift,
@@ -527,7 +529,10 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
             const K k = s_keys[j];
             const uint32_t d = uint32_t(k >> shift) & digit_mask;
             const uint32_t o = s_base[d] + j;
-            keys_out[o] = k;
+            if constexpr (sizeof(KO) < sizeof(K))
+                keys_out[o] = KO(k & ((K(1) << shift) - 1));
+            else
+                keys_out[o] = k;
             if constexpr (PAIRS) vals_out[o] = s_vals[j];
         };
         if (tile_n == kTile) {
@@ -711,22 +716,21 @@ constexpr int kSegIpt = 16;
 constexpr uint32_t kSegTile = uint32_t(kSegThreads) * kSegIpt;
 constexpr uint32_t kSegMaxBins = 1u << kSegMaxBits;  // combined bins of the second pass
 constexpr int kSegBinsPerThread = kSegMaxBins / kSegThreads;
+constexpr uint32_t kSegMaxDigits = 1u << kMaxSegDigitBits;
+constexpr int kSegDigitsPerThread = int(kSegMaxDigits / kSegThreads);
+static_assert(kSegDigitsPerThread * kSegThreads == int(kSegMaxDigits), "digits spread evenly over the threads");
 static_assert(kSegTile == kSegTileItems, "tile size shared with the planner");
 static_assert(kSegTile <= 65536 && kSegMaxBins <= 65536, "bin and rank packed in 16 bits each");
 static_assert(kSegMaxBins == kSegThreads * kSegBinsPerThread && kSegBinsPerThread * kWave * (kSegThreads / kWave) ==
                   kSegMaxBins, "bins covered by the row-wise scan");
 
-// Counts of tile t (local index lt in chunk c with nt tiles):
-// counts[chunk_tiles[c] * R + d * nt + lt].
-__device__ __forceinline__ uint64_t seg_count_index(const SegTile& tile, const uint32_t* chunk_tiles, uint64_t t,
-                                                    uint32_t R, uint32_t d)
+// Counts of tile t, tile-major: counts[t * R + d] (one contiguous run per
+// tile, written and read whole).
+__device__ __forceinline__ uint64_t seg_count_index(uint64_t t, uint32_t R, uint32_t d)
 {
-    const uint32_t t0 = chunk_tiles[tile.chunk], nt = chunk_tiles[tile.chunk + 1] - t0;
-    return uint64_t(t0) * R + uint64_t(d) * nt + (t - t0);
+    return t * R + d;
 }
 
-// Count the pass digit ((key >> shift) & dmask) of every key of each tile.
-// The first pass also range-checks the chunk-local keys.
 // Tile table of the segmented sort, planned on the device from the chunk
 // offsets (no host planning, so a captured build replays for any offsets
 // within its bounds): chunk c's tiles start at chunk_tiles[c] (exclusive scan
@@ -777,57 +781,80 @@ __global__ void k_seg_plan_tiles(const uint64_t* __restrict__ offsets, const uin
     }
 }
 
-template <bool FIRST>
-__global__ __launch_bounds__(kSegThreads) void k_seg_count(const uint64_t* __restrict__ keys,
+// Count the pass digit ((key >> shift) & dmask) of every key of each tile.
+// The first pass also range-checks the chunk-local keys.
+template <bool FIRST, typename KI>
+__global__ __launch_bounds__(kSegThreads) void k_seg_count(const KI* __restrict__ keys,
                                                            const SegTile* __restrict__ tiles,
-                                                           const uint64_t* __restrict__ ntiles_ptr,
-                                                           const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
+                                                           const uint64_t* __restrict__ ntiles_ptr, uint32_t shift,
                                                            uint32_t dmask, uint32_t chunk_bits,
                                                            uint32_t* __restrict__ counts,
                                                            unsigned long long* __restrict__ err)
 {
-    __shared__ uint32_t s_cnt[256];
+    __shared__ uint32_t s_cnt[kSegMaxDigits];
     const uint64_t ntiles = *ntiles_ptr;
     const uint32_t R = dmask + 1;
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    bool bad = false;
+    // Tiles XCD by XCD: a chunk's neighbouring tiles have adjacent counts in
+    // every digit row, which then combine in one L2.
+    const TileWalk walk = xcd_tiles(ntiles);
+    for (uint64_t t = walk.tile; t < walk.end; t += walk.step) {
+        const SegTile tile = tiles[t];
+        // The whole tile's keys are in flight before the counters are cleared.
+        KI key[kSegIpt];
+#pragma unroll
+        for (int i = 0; i < kSegIpt; ++i) {
+            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
+            key[i] = j < tile.count ? __builtin_nontemporal_load(keys + tile.start + j) : KI(0);
+        }
         for (uint32_t i = threadIdx.x; i < R; i += kSegThreads) s_cnt[i] = 0;
         __syncthreads();
-        const SegTile tile = tiles[t];
-        bool bad = false;
-        for (uint32_t j = threadIdx.x; j < tile.count; j += kSegThreads) {
-            const uint64_t k = __builtin_nontemporal_load(keys + tile.start + j);
-            if constexpr (FIRST) bad |= (k >> chunk_bits) != 0;
-            atomicAdd(&s_cnt[uint32_t(k >> shift) & dmask], 1u);
+#pragma unroll
+        for (int i = 0; i < kSegIpt; ++i) {
+            if (uint32_t(i) * kSegThreads + threadIdx.x < tile.count) {
+                if constexpr (FIRST) bad |= chunk_bits < 8 * sizeof(KI) && (uint64_t(key[i]) >> chunk_bits) != 0;
+                atomicAdd(&s_cnt[uint32_t(uint64_t(key[i]) >> shift) & dmask], 1u);
+            }
         }
-        if (FIRST && bad) atomicOr(err, (unsigned long long)kErrKeyRange);
         __syncthreads();
         for (uint32_t d = threadIdx.x; d < R; d += kSegThreads)
-            counts[seg_count_index(tile, chunk_tiles, t, R, d)] = s_cnt[d];
+            counts[seg_count_index(t, R, d)] = s_cnt[d];
         __syncthreads();
     }
+    if (FIRST && bad) atomicOr(err, (unsigned long long)kErrKeyRange);
 }
 
-// One wave per (chunk, digit) row: exclusive scan of the row over the chunk's
-// tiles, in place; the row total to totals[c * R + d].
-__global__ __launch_bounds__(kWave) void k_seg_scan_rows(const uint32_t* __restrict__ chunk_tiles, uint64_t num_chunks,
+// One wave per (chunk, 64 digits): lane d walks digit d's counts down the
+// chunk's tiles (each step reads and writes 64 adjacent counts of one tile),
+// replacing them by their exclusive prefix; the digit's total goes to
+// totals[c * R + d].  Loads run kScanAhead tiles ahead of the running sum.
+constexpr uint32_t kScanAhead = 8;
+
+__global__ __launch_bounds__(kWave) void k_seg_scan_cols(const uint32_t* __restrict__ chunk_tiles, uint64_t num_chunks,
                                                          uint32_t R, uint32_t* __restrict__ counts,
                                                          uint32_t* __restrict__ totals)
 {
     const uint32_t lane = lane_id();
-    for (uint64_t row = blockIdx.x; row < num_chunks * R; row += gridDim.x) {
-        const uint64_t c = row / R;
-        const uint32_t d = uint32_t(row % R);
+    const uint32_t groups = (R + kWave - 1) / kWave;
+    for (uint64_t job = blockIdx.x; job < num_chunks * groups; job += gridDim.x) {
+        const uint64_t c = job / groups;
+        const uint32_t d = uint32_t(job % groups) * kWave + lane;
         const uint32_t t0 = chunk_tiles[c], nt = chunk_tiles[c + 1] - t0;
-        uint32_t* r = counts + uint64_t(t0) * R + uint64_t(d) * nt;
-        uint32_t carry = 0;
-        for (uint32_t i0 = 0; i0 < nt; i0 += kWave) {
-            const uint32_t i = i0 + lane;
-            const uint32_t v = i < nt ? r[i] : 0u;
-            const uint32_t inc = wave_inclusive_scan(v);
-            if (i < nt) r[i] = carry + inc - v;
-            carry += uint32_t(__builtin_amdgcn_readlane(int(inc), kWave - 1));
+        uint32_t* col = counts + uint64_t(t0) * R + d;
+        uint32_t run = 0;
+        if (d < R) {
+            for (uint32_t i0 = 0; i0 < nt; i0 += kScanAhead) {
+                uint32_t v[kScanAhead];
+#pragma unroll
+                for (uint32_t k = 0; k < kScanAhead; ++k) v[k] = i0 + k < nt ? col[uint64_t(i0 + k) * R] : 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < kScanAhead; ++k) {
+                    if (i0 + k < nt) col[uint64_t(i0 + k) * R] = run;
+                    run += v[k];
+                }
+            }
+            totals[c * R + d] = run;
         }
-        if (lane == 0) totals[c * R + d] = carry;
     }
 }
 
@@ -840,48 +867,69 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scan_digits(const uint64_t*
 {
     __shared__ uint32_t s_scan[kSegThreads / kWave];
     for (uint64_t c = blockIdx.x; c < num_chunks; c += gridDim.x) {
-        const uint32_t d = threadIdx.x;
+        // kSegDigitsPerThread consecutive digits per thread.
+        uint32_t v[kSegDigitsPerThread];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSegDigitsPerThread; ++k) {
+            const uint32_t d = threadIdx.x * kSegDigitsPerThread + uint32_t(k);
+            v[k] = d < R ? totals[c * R + d] : 0u;
+            sum += v[k];
+        }
         uint32_t unused;
-        const uint32_t ex = block_exclusive_scan<kSegThreads>(d < R ? totals[c * R + d] : 0u, s_scan, unused);
-        if (d < R) starts[c * R + d] = uint32_t(chunk_start[c]) + ex;
+        uint32_t ex = uint32_t(chunk_start[c]) + block_exclusive_scan<kSegThreads>(sum, s_scan, unused);
+#pragma unroll
+        for (int k = 0; k < kSegDigitsPerThread; ++k) {
+            const uint32_t d = threadIdx.x * kSegDigitsPerThread + uint32_t(k);
+            if (d < R) starts[c * R + d] = ex;
+            ex += v[k];
+        }
     }
 }
 
 // The scatter of one pass.  FIRST: bins = the pass digit, and the keys get
-// their chunk index above chunk_bits.  Else: bins = digit * span + (previous
-// digit - the tile's smallest previous digit), span = the tile's range of
-// previous digits (the input is sorted on them, so a tile usually holds one or
-// two).  Output offset of tile t's digit d: starts[c * R + d] plus the row
-// prefix in counts.
-template <bool FIRST>
-__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __restrict__ keys_in,
+// their chunk index above chunk_bits as they are written.  Else: bins = digit *
+// span + (previous digit - the tile's smallest previous digit), span = the
+// tile's range of previous digits (the input is sorted on them, so a tile
+// usually holds one or two).  Output offset of tile t's digit d: starts[c * R +
+// d] plus the row prefix in counts.  The first pass stages the keys at their
+// input width and needs only R <= kSegMaxDigits bins, which keeps its LDS (and
+// so its workgroups per CU) within reach of the later passes' double.  Tiles
+// are walked XCD by XCD (the grid is a multiple of kXcds), so neighbouring
+// tiles, whose runs of one digit are adjacent in the output, are written
+// through one L2.
+template <bool FIRST, typename KI>
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const KI* __restrict__ keys_in,
                                                              uint64_t* __restrict__ keys_out,
                                                              const SegTile* __restrict__ tiles,
-                                                             const uint64_t* __restrict__ ntiles_ptr,
-                                                             const uint32_t* __restrict__ chunk_tiles, uint32_t shift,
+                                                             const uint64_t* __restrict__ ntiles_ptr, uint32_t shift,
                                                              uint32_t dmask, uint32_t prev_shift, uint32_t prev_bits,
                                                              uint32_t chunk_bits, const uint32_t* __restrict__ counts,
                                                              const uint32_t* __restrict__ starts)
 {
-    __shared__ uint64_t s_keys[kSegTile];
-    __shared__ uint32_t s_bin[kSegMaxBins];
-    __shared__ uint32_t s_base[256];
+    using KS = std::conditional_t<FIRST, KI, uint64_t>;  // staged key
+    constexpr uint32_t kBins = FIRST ? kSegMaxDigits : kSegMaxBins;
+    constexpr int kBinsPerThread = int(kBins / kSegThreads);
+    static_assert(kBinsPerThread * kSegThreads == int(kBins), "bins covered by the row-wise scan");
+    __shared__ KS s_keys[kSegTile];
+    __shared__ uint32_t s_bin[kBins];
+    __shared__ uint32_t s_base[kSegMaxDigits];
     __shared__ uint32_t s_scan[kSegThreads / kWave];
     __shared__ uint32_t s_span[2 * (kSegThreads / kWave)];
     const uint64_t ntiles = *ntiles_ptr;
     const uint32_t R = dmask + 1;
     const uint32_t prev_mask = (1u << prev_bits) - 1u;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const TileWalk walk = xcd_tiles(ntiles);
+    for (uint64_t t = walk.tile; t < walk.end; t += walk.step) {
         const SegTile tile = tiles[t];
-        uint64_t key[kSegIpt];
+        KS key[kSegIpt];
         uint32_t br[kSegIpt];  // bin << 16 | rank within the bin
         uint32_t lo = 0xFFFFFFFFu, hi = 0;
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
             const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
-            key[i] = j < tile.count ? __builtin_nontemporal_load(keys_in + tile.start + j) : 0ull;
-            if (FIRST) key[i] |= uint64_t(tile.chunk) << chunk_bits;
+            key[i] = j < tile.count ? KS(__builtin_nontemporal_load(keys_in + tile.start + j)) : KS(0);
             if (!FIRST && j < tile.count) {
                 const uint32_t p = uint32_t(key[i] >> prev_shift) & prev_mask;
                 lo = min(lo, p);
@@ -910,27 +958,27 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
             pmin = lo;
             span = hi >= lo ? hi - lo + 1 : 1;
         }
-        const uint32_t nbins = R * span;  // at most R << prev_bits <= kSegMaxBins
+        const uint32_t nbins = R * span;  // FIRST: R; else at most R << prev_bits <= kSegMaxBins
         for (uint32_t i = threadIdx.x; i < nbins; i += kSegThreads) s_bin[i] = 0;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
             const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
-            uint32_t bin = uint32_t(key[i] >> shift) & dmask;
+            uint32_t bin = uint32_t(uint64_t(key[i]) >> shift) & dmask;
             if (!FIRST) bin = bin * span + ((uint32_t(key[i] >> prev_shift) & prev_mask) - pmin);
             const uint32_t r = j < tile.count ? atomicAdd(&s_bin[bin], 1u) : 0u;
             br[i] = bin << 16 | r;
         }
         __syncthreads();
-        // Exclusive scan of the bins.  Wave w takes bins [w * 64 * kSegBinsPerThread, ...)
+        // Exclusive scan of the bins.  Wave w takes bins [w * 64 * kBinsPerThread, ...)
         // as rows of 64 consecutive bins (one per lane, so the LDS accesses
         // are conflict-free), with a carry from row to row and wave to wave;
         // rows past the tile's bins are skipped.
-        uint32_t ex[kSegBinsPerThread];
+        uint32_t ex[kBinsPerThread];
         uint32_t carry = 0;
 #pragma unroll
-        for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t row0 = (w * kSegBinsPerThread + uint32_t(q)) * kWave;
+        for (int q = 0; q < kBinsPerThread; ++q) {
+            const uint32_t row0 = (w * kBinsPerThread + uint32_t(q)) * kWave;
             if (row0 >= nbins) break;  // wave-uniform
             const uint32_t b = row0 + lane;
             const uint32_t v = b < nbins ? s_bin[b] : 0u;
@@ -944,8 +992,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
 #pragma unroll
         for (int i = 0; i < kSegThreads / kWave; ++i) base += uint32_t(i) < w ? s_scan[i] : 0u;
 #pragma unroll
-        for (int q = 0; q < kSegBinsPerThread; ++q) {
-            const uint32_t row0 = (w * kSegBinsPerThread + uint32_t(q)) * kWave;
+        for (int q = 0; q < kBinsPerThread; ++q) {
+            const uint32_t row0 = (w * kBinsPerThread + uint32_t(q)) * kWave;
             if (row0 >= nbins) break;
             const uint32_t b = row0 + lane;
             if (b < nbins) s_bin[b] = base + ex[q];
@@ -953,7 +1001,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
         __syncthreads();
         // Digit d's keys start at its first bin inside the tile.
         for (uint32_t d = threadIdx.x; d < R; d += kSegThreads)
-            s_base[d] = starts[uint64_t(tile.chunk) * R + d] + counts[seg_count_index(tile, chunk_tiles, t, R, d)] -
+            s_base[d] = starts[uint64_t(tile.chunk) * R + d] + counts[seg_count_index(t, R, d)] -
                         s_bin[d * span];
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
@@ -961,9 +1009,10 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
             if (j < tile.count) s_keys[s_bin[br[i] >> 16] + (br[i] & 0xFFFFu)] = key[i];
         }
         __syncthreads();
+        const uint64_t prefix = FIRST ? uint64_t(tile.chunk) << chunk_bits : 0ull;
         for (uint32_t j = threadIdx.x; j < tile.count; j += kSegThreads) {
-            const uint64_t k = s_keys[j];
-            keys_out[s_base[uint32_t(k >> shift) & dmask] + j] = k;
+            const KS k = s_keys[j];
+            keys_out[s_base[uint32_t(uint64_t(k) >> shift) & dmask] + j] = uint64_t(k) | prefix;
         }
         __syncthreads();
     }
@@ -971,7 +1020,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint64_t* __r
 
 }  // namespace
 
-int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+template <typename KI>
+int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_a, uint64_t* keys_b,
                           const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
                           const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits)
@@ -987,7 +1037,9 @@ int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t*
     const uint32_t b2 = B - b1;                                  // second (0: one pass)
     const uint64_t tile_bound = seg_tile_bound(n_bound, num_chunks);
     const int cus = device_cu_count();
-    const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(tile_bound, uint64_t(cus) * 64)));
+    // A multiple of kXcds, as the scatter's tile walk assumes.
+    const int grid = int(std::max<uint64_t>(
+        kXcds, std::min<uint64_t>((tile_bound + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 64) / kXcds * kXcds));
     const int chunk_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks, uint64_t(cus) * 8)));
     {
         TimedLaunch t(kKSortScan, stream);
@@ -997,37 +1049,124 @@ int launch_segmented_sort(hipStream_t stream, const uint64_t* keys_in, uint64_t*
                                                                                            uint64_t(cus) * 8)))),
                            dim3(256), 0, stream, chunk_start, sc.chunk_tiles, num_chunks, sc.ntiles, sc.tiles);
     }
-    auto pass = [&](bool first, const uint64_t* src, uint64_t* dst, uint32_t shift, uint32_t bits) {
+    // The first pass reads the caller's keys (KI), the second the first's output.
+    auto pass = [&](bool first, const void* src, uint64_t* dst, uint32_t shift, uint32_t bits) {
         const uint32_t mask = (1u << bits) - 1u, R = mask + 1;
-        const int row_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(num_chunks * R, uint64_t(cus) * 64)));
+        const uint64_t col_jobs = num_chunks * ((R + kWave - 1) / kWave);
+        const int col_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(col_jobs, uint64_t(cus) * 64)));
         {
             TimedLaunch t(kKSortCount, stream);
             if (first)
-                hipLaunchKernelGGL(k_seg_count<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, sc.tiles,
-                                   sc.ntiles, sc.chunk_tiles, shift, mask, chunk_bits, sc.counts, err);
+                hipLaunchKernelGGL((k_seg_count<true, KI>), dim3(grid), dim3(kSegThreads), 0, stream,
+                                   static_cast<const KI*>(src), sc.tiles, sc.ntiles, shift, mask,
+                                   chunk_bits, sc.counts, err);
             else
-                hipLaunchKernelGGL(k_seg_count<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, sc.tiles,
-                                   sc.ntiles, sc.chunk_tiles, shift, mask, chunk_bits, sc.counts, err);
+                hipLaunchKernelGGL((k_seg_count<false, uint64_t>), dim3(grid), dim3(kSegThreads), 0, stream,
+                                   static_cast<const uint64_t*>(src), sc.tiles, sc.ntiles, shift,
+                                   mask, chunk_bits, sc.counts, err);
         }
         {
             TimedLaunch t(kKSortScan, stream);
-            hipLaunchKernelGGL(k_seg_scan_rows, dim3(row_grid), dim3(kWave), 0, stream, sc.chunk_tiles, num_chunks, R,
+            hipLaunchKernelGGL(k_seg_scan_cols, dim3(col_grid), dim3(kWave), 0, stream, sc.chunk_tiles, num_chunks, R,
                                sc.counts, sc.totals);
             hipLaunchKernelGGL(k_seg_scan_digits, dim3(chunk_grid), dim3(kSegThreads), 0, stream, chunk_start,
                                num_chunks, R, sc.totals, sc.starts);
         }
         TimedLaunch t(kKSortScatterKeys, stream);
         if (first)
-            hipLaunchKernelGGL(k_seg_scatter<true>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, sc.tiles,
-                               sc.ntiles, sc.chunk_tiles, shift, mask, 0u, 0u, chunk_bits, sc.counts, sc.starts);
+            hipLaunchKernelGGL((k_seg_scatter<true, KI>), dim3(grid), dim3(kSegThreads), 0, stream,
+                               static_cast<const KI*>(src), dst, sc.tiles, sc.ntiles, shift, mask, 0u,
+                               0u, chunk_bits, sc.counts, sc.starts);
         else
-            hipLaunchKernelGGL(k_seg_scatter<false>, dim3(grid), dim3(kSegThreads), 0, stream, src, dst, sc.tiles,
-                               sc.ntiles, sc.chunk_tiles, shift, mask, lo, b1, chunk_bits, sc.counts, sc.starts);
+            hipLaunchKernelGGL((k_seg_scatter<false, uint64_t>), dim3(grid), dim3(kSegThreads), 0, stream,
+                               static_cast<const uint64_t*>(src), dst, sc.tiles, sc.ntiles, shift,
+                               mask, lo, b1, chunk_bits, sc.counts, sc.starts);
     };
     pass(true, keys_in, keys_a, lo, b1);
     if (b2 == 0) return 0;
     pass(false, keys_a, keys_b, lo + b1, b2);
     return 1;
+}
+
+template int launch_segmented_sort<uint64_t>(hipStream_t, const uint64_t*, uint64_t*, uint64_t*, const uint64_t*,
+                                             uint64_t, const uint64_t*, uint64_t, const SegScratch&, uint32_t,
+                                             uint32_t, unsigned long long*, uint32_t);
+template int launch_segmented_sort<uint32_t>(hipStream_t, const uint32_t*, uint64_t*, uint64_t*, const uint64_t*,
+                                             uint64_t, const uint64_t*, uint64_t, const SegScratch&, uint32_t,
+                                             uint32_t, unsigned long long*, uint32_t);
+
+// ---------------------------------------------------------------------------
+// Top digit first (plain builds of 64-bit keys).  One pass on the top digit
+// [top_shift, top_shift + top_bits) partitions the keys into 2^top_bits
+// Morton-ordered buckets and writes each key as a 32-bit key local to its
+// bucket (its bits below top_shift); the segmented sort then sorts the bits
+// [lo, top_shift) inside each bucket, reading those 32-bit keys, and puts the
+// bucket index back above them.  Against an LSD sort of the same bits it moves
+// half the bytes in the top pass's writes and in the reads after it.
+// ---------------------------------------------------------------------------
+namespace {
+
+// starts[d] = keys with a smaller top digit (d < R); starts[R] = n.
+__global__ __launch_bounds__(kRadix) void k_digit_starts(const uint32_t* __restrict__ hist, uint32_t R,
+                                                         const uint64_t* __restrict__ n_ptr,
+                                                         uint64_t* __restrict__ starts)
+{
+    __shared__ uint32_t s_scan[kRadix / kWave];
+    const uint32_t d = threadIdx.x;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<kRadix>(d < R ? hist[d] : 0u, s_scan, total);
+    if (d < R) starts[d] = ex;
+    if (d == 0) starts[R] = *n_ptr;
+}
+
+}  // namespace
+
+int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint64_t* keys_a,
+                          uint64_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
+                          const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
+                          unsigned long long* key_or, uint64_t* bucket_start, const SegScratch& sc,
+                          unsigned long long* err)
+{
+    if (top_bits < 1 || top_bits > uint32_t(kMaxRadixBits) || top_shift > 32 || top_shift <= lo ||
+        top_shift - lo > kSegMaxBits || n_bound == 0)
+        return -1;
+    const uint32_t dmask = (1u << top_bits) - 1u, R = dmask + 1;
+    HistPlan hp{};
+    hp.passes = 1;
+    hp.shift[0] = top_shift;
+    hp.mask[0] = dmask;
+    auto run = [&](auto ipt_tag) {
+        constexpr int I = decltype(ipt_tag)::value;
+        const uint64_t stride = sort_tiles(n_bound, I);
+        const int cus = device_cu_count();
+        const uint32_t grid = uint32_t(std::max<uint64_t>(
+            kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
+        const uint32_t scatter_grid = uint32_t(std::max<uint64_t>(
+            kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 64) / kXcds * kXcds));
+        {
+            TimedLaunch t(kKHistogram, stream);
+            hipLaunchKernelGGL((k_histogram<uint64_t, I>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in, n_ptr,
+                               hp, zero, key_or, counts, stride, ChunkPrefix{});
+        }
+        {
+            TimedLaunch t(kKSortScan, stream);
+            if (stride <= kSmallScanTiles)
+                hipLaunchKernelGGL(k_sort_scan<kWave>, dim3(R), dim3(kWave), 0, stream, counts, zero, n_ptr, stride,
+                                   uint64_t(kSortThreads) * I);
+            else
+                hipLaunchKernelGGL(k_sort_scan<kScanThreads>, dim3(R), dim3(kScanThreads), 0, stream, counts, zero,
+                                   n_ptr, stride, uint64_t(kSortThreads) * I);
+            hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(kRadix), 0, stream, zero, R, n_ptr, bucket_start);
+        }
+        TimedLaunch t(kKSortScatterKeys, stream);
+        hipLaunchKernelGGL((k_sort_scatter<uint64_t, false, kRankUnstable, I, kSortThreads, uint32_t>),
+                           dim3(scatter_grid), dim3(kSortThreads), kUnstableExtraLds, stream, keys_in, local, nullptr,
+                           nullptr, n_ptr, top_shift, dmask, counts, stride, ChunkPrefix{}, 0u, 0u, nullptr, 64u);
+    };
+    if (ipt == kSortIptLarge) run(std::integral_constant<int, kSortIptLarge>{});
+    else run(std::integral_constant<int, kSortIptSmall>{});
+    return launch_segmented_sort<uint32_t>(stream, local, keys_a, keys_b, bucket_start, R, n_ptr, n_bound, sc, lo,
+                                           top_shift, err, 0);
 }
 
 }  // namespace svdag

@@ -92,10 +92,15 @@ typedef enum svdag_option {
                                          (16 bytes per voxel column, default 4 GiB): within it
                                          the columns are evaluated once, past it twice (count,
                                          then write); 0 always takes two passes */
-    SVDAG_OPT_SEG_FIRST_BITS = 10     /* batched builds that sort inside each chunk (chunk
+    SVDAG_OPT_SEG_FIRST_BITS = 10,    /* batched builds that sort inside each chunk (chunk
                                          depth <= 9): digit width of that sort's first pass,
                                          0 = half the chunk-local bits (default); clamped so
-                                         that neither pass is wider than 8 bits */
+                                         that neither pass is wider than 9 bits */
+    SVDAG_OPT_TOP_DIGIT_FIRST = 11    /* plain builds of 64-bit keys: sort the top digit first
+                                         and the rest inside each top-digit bucket as 32-bit
+                                         keys; 0 = never, 1 = from 2^20 keys on (default),
+                                         2 = always.  Used where the bits below the top digit
+                                         fit 32 and at most 12 of them need sorting. */
 } svdag_option;
 
 /* --- context ------------------------------------------------------------ */

@@ -23,9 +23,9 @@ def setting_of(ctx):
 
 
 def fake_cost(setting):
-    # Fastest at radix 8, tile 4096, load shift 2 (made-up costs).
+    # Fastest at radix 8, tile 4096, top digit first, load shift 2 (made-up costs).
     return (abs(setting["radix_bits"] - 8) + (setting["sort_tile"] == 8192) * 2
-            + (setting["table_load_shift"] == 1) * 0.5 + 1.0)
+            + (setting["table_load_shift"] == 1) * 0.5 + (setting["top_digit_first"] == 0) * 0.25 + 1.0)
 
 
 def make_run(ctx, words_for=lambda s: np.arange(10, dtype=np.uint32)):
@@ -47,9 +47,9 @@ def test_picks_fastest_and_applies_it():
     ctx = FakeCtx()
     build, fetch, clock = make_run(ctx)
     res = autotune.autotune(ctx, build, fetch, reps=2, clock=clock)
-    assert res["best"] == {"radix_bits": 8, "sort_tile": 4096, "table_load_shift": 2}
+    assert res["best"] == {"radix_bits": 8, "sort_tile": 4096, "table_load_shift": 2, "top_digit_first": 1}
     assert setting_of(ctx) == res["best"]
-    assert len(res["table"]) == 3 * 2 * 2
+    assert len(res["table"]) == 3 * 2 * 2 * 2
     assert res["best_ms"] == pytest.approx(1.0)
 
 
@@ -83,7 +83,8 @@ def test_option_ids_match_the_header():
     for name, opt in (("RADIX_BITS", _lib.OPT_RADIX_BITS), ("SORT_TILE", _lib.OPT_SORT_TILE),
                       ("TABLE_LOAD_SHIFT", _lib.OPT_TABLE_LOAD_SHIFT)):
         assert f"SVDAG_OPT_{name} = {opt}," in text
-    assert f"SVDAG_OPT_SEG_FIRST_BITS = {_lib.OPT_SEG_FIRST_BITS} " in text
+    assert f"SVDAG_OPT_SEG_FIRST_BITS = {_lib.OPT_SEG_FIRST_BITS}," in text
+    assert f"SVDAG_OPT_TOP_DIGIT_FIRST = {_lib.OPT_TOP_DIGIT_FIRST} " in text
 
 
 def test_space_follows_the_build_kind():
