@@ -773,6 +773,8 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     // Sort keys on bits [0, 3*depth) (SortKeys, DAGCompression.cu:1024-1026).
     int where;
     const void* sorted;
+    // Segmented sorts leave 32-bit keys local to these buckets.
+    ChunkPrefix buckets;
     if (batch) {
         if (!d_off) {
             uint64_t* up = reinterpret_cast<uint64_t*>(A + B.offsets);
@@ -789,6 +791,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         uint64_t* ka = reinterpret_cast<uint64_t*>(A + L.keys_a);
         uint64_t* kb = reinterpret_cast<uint64_t*>(A + L.keys_b);
         if (B.segmented) {
+            buckets.offsets = d_off;
+            buckets.num_chunks = batch->num_chunks;
+            buckets.chunk_bits = B.seg_chunk_bits;
             SegScratch sc;
             sc.tiles = reinterpret_cast<SegTile*>(A + B.seg_tiles);
             sc.chunk_tiles = reinterpret_cast<uint32_t*>(A + B.seg_ct);
@@ -796,7 +801,8 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
             sc.totals = reinterpret_cast<uint32_t*>(A + B.seg_totals);
             sc.starts = reinterpret_cast<uint32_t*>(A + B.seg_starts);
             sc.ntiles = reinterpret_cast<uint64_t*>(A + B.seg_ntiles);
-            where = launch_segmented_sort(st, reinterpret_cast<const uint64_t*>(d_keys), ka, kb, d_off,
+            where = launch_segmented_sort(st, reinterpret_cast<const uint64_t*>(d_keys),
+                                          reinterpret_cast<uint32_t*>(ka), reinterpret_cast<uint32_t*>(kb), d_off,
                                           batch->num_chunks, &c->n, n, sc, B.seg_lo, B.seg_chunk_bits, err,
                                           ctx->seg_first_bits);
         } else {
@@ -819,10 +825,14 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
                 sc.totals = reinterpret_cast<uint32_t*>(A + L.seg_totals);
                 sc.starts = reinterpret_cast<uint32_t*>(A + L.seg_starts);
                 sc.ntiles = reinterpret_cast<uint64_t*>(A + L.seg_ntiles);
-                where = launch_top_digit_sort(st, d_keys, reinterpret_cast<uint32_t*>(kb), ka, kb, L.key_plan.ipt,
-                                              L.top_shift, L.top_bits, L.key_plan.shift[0], &c->n, n,
+                where = launch_top_digit_sort(st, d_keys, reinterpret_cast<uint32_t*>(kb),
+                                              reinterpret_cast<uint32_t*>(ka), reinterpret_cast<uint32_t*>(kb),
+                                              L.key_plan.ipt, L.top_shift, L.top_bits, L.key_plan.shift[0], &c->n, n,
                                               reinterpret_cast<uint32_t*>(A + L.key_sort_zero), sort_counts,
                                               &c->key_or, reinterpret_cast<uint64_t*>(A + L.bucket_start), sc, err);
+                buckets.offsets = reinterpret_cast<const uint64_t*>(A + L.bucket_start);
+                buckets.num_chunks = 1ull << L.top_bits;
+                buckets.chunk_bits = L.top_shift;
             }
         }
         if (!L.top_first)
@@ -840,12 +850,12 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
     // Leaf ReduceByKey (DAGCompression.cu:1044-1068).
     uint64_t* reg_code = reinterpret_cast<uint64_t*>(A + L.reg_code);
     uint64_t* reg_mask = reinterpret_cast<uint64_t*>(A + L.reg_mask);
-    if (batch || sizeof(K) == 8)
-        launch_leaf_reduce<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, L.group_bits, reg_code, reg_mask,
-                                     lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
-                                     code_mask);
-    else
+    if (buckets.offsets || (!batch && sizeof(K) == 4))
         launch_leaf_reduce<uint32_t>(st, static_cast<const uint32_t*>(sorted), c, L.group_bits, reg_code, reg_mask,
+                                     lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
+                                     code_mask, buckets);
+    else
+        launch_leaf_reduce<uint64_t>(st, static_cast<const uint64_t*>(sorted), c, L.group_bits, reg_code, reg_mask,
                                      lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
                                      code_mask);
 

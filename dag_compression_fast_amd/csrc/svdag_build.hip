@@ -97,7 +97,9 @@ constexpr uint32_t kReduceChunk = uint32_t(kPackTileItems);
 constexpr uint32_t kReduceTable = 1u << kLeafGroupBits;
 constexpr uint32_t kReduceRegion = uint32_t(kLeafRegionItems);
 static_assert(kReduceRegion == kReduceChunk + kReduceTable, "region bound");
-constexpr int kReduceGroups = 4;  // groups of 64 keys loaded together
+// Groups of 64 keys loaded together: 2 KiB of keys per wave in flight.
+template <typename K>
+constexpr int kReduceGroups = int(2048 / (kWave * sizeof(K)));
 constexpr int kReduceWaves = kThreads / kWave;
 static_assert(kReduceTable == 8 * kWave, "one lane holds the 8 leaves of one parent");
 
@@ -109,30 +111,60 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
     return uint64_t(hi) << 32 | lo;
 }
 
-// kReduceGroups groups of 64 raw keys from `base`; indices past the end read
+// G groups of 64 raw keys from `base`; indices past the end read
 // the last key (n >= 1), and the caller ignores those lanes.  No use of the
 // values here, so the loads stay in flight until the keys are needed.
-template <typename K>
+template <typename K, int G>
 __device__ __forceinline__ void load_groups(const K* __restrict__ sorted, uint64_t base, uint64_t n,
-                                            uint64_t (&k)[kReduceGroups])
+                                            uint64_t (&k)[G])
 {
     const uint32_t lane = lane_id();
 #pragma unroll
-    for (int g = 0; g < kReduceGroups; ++g)  // read once: non-temporal
+    for (int g = 0; g < G; ++g)  // read once: non-temporal
         k[g] = uint64_t(__builtin_nontemporal_load(sorted + min(base + g * kWave + lane, n - 1)));
 }
 
-template <typename K>
+// Bucket of position p among Morton-ordered buckets with starts start(0..num)
+// (start(num) > p): the last b >= lo with start(b) <= p (empty buckets share
+// their start with the next one).
+template <typename Start>
+__device__ __forceinline__ uint64_t bucket_of(const Start& start, uint64_t lo, uint64_t num, uint64_t p)
+{
+    uint64_t hi = num;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (start(mid) <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint32_t kLdsBuckets = 512;  // bucket tables up to this size are staged in LDS
+
+// BUCKETED: 32-bit bucket-local keys (see launch_leaf_reduce); each wave keeps
+// the bucket of its current position and its end, so a batch inside one
+// bucket takes the prefix from a register; batches that cross a bucket end
+// (a few per bucket) search per lane.
+template <typename K, bool BUCKETED>
 __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ sorted,
                                                           const uint64_t* __restrict__ n_ptr, uint64_t code_mask,
                                                           uint32_t r, uint64_t* __restrict__ reg_code,
                                                           uint64_t* __restrict__ reg_mask,
                                                           uint32_t* __restrict__ counts,
                                                           uint64_t* __restrict__ next_count,
-                                                          unsigned long long* __restrict__ err)
+                                                          unsigned long long* __restrict__ err, ChunkPrefix bk)
 {
     __shared__ uint64_t s_tab[kReduceWaves][kReduceTable];
     __shared__ uint64_t s_parents[kReduceWaves];
+    constexpr int G = kReduceGroups<K>;
+    // Bucket starts (positions < 2^30) staged in LDS when the table is small.
+    __shared__ uint32_t s_bstart[BUCKETED ? kLdsBuckets + 1 : 1];
+    const bool staged = BUCKETED && bk.num_chunks <= kLdsBuckets;  // the same for the whole block
+    if (staged) {
+        for (uint32_t i = threadIdx.x; i <= bk.num_chunks; i += kThreads) s_bstart[i] = uint32_t(bk.offsets[i]);
+        __syncthreads();
+    }
+    auto start_of = [&](uint64_t b) -> uint64_t { return staged ? uint64_t(s_bstart[b]) : bk.offsets[b]; };
     const uint32_t lane = lane_id();
     // The wave index as a scalar, so that the chunk loop and the group
     // bookkeeping below compile to scalar control flow.
@@ -155,8 +187,37 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
         uint64_t* const oc = reg_code + chunk * kReduceRegion;
         uint64_t* const om = reg_mask + chunk * kReduceRegion;
         uint32_t written = 0;
+        // Bucket of the chunk's current batch and that bucket's end.
+        uint64_t cur = 0, cur_end = ~0ull;
+        if constexpr (BUCKETED) {
+            cur = bucket_of(start_of, 0, bk.num_chunks, c0 > 0 ? c0 - 1 : 0);
+            cur_end = start_of(cur + 1);
+        }
+        // Key at position p of a batch starting at bb, with its bucket.
+        auto add_bucket = [&](uint64_t bb, uint64_t (&v)[G]) {
+            if constexpr (BUCKETED) {
+                const uint64_t last = min(bb + uint64_t(G) * kWave - 1, n - 1);
+                if (last < cur_end) {
+#pragma unroll
+                    for (int g = 0; g < G; ++g) v[g] |= cur << bk.chunk_bits;
+                } else {
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const uint64_t p = min(bb + uint64_t(g) * kWave + lane, n - 1);
+                        v[g] |= bucket_of(start_of, cur, bk.num_chunks, p) << bk.chunk_bits;
+                    }
+                    cur = bucket_of(start_of, cur, bk.num_chunks, last);
+                    cur_end = start_of(cur + 1);
+                }
+            }
+        };
         // Group of the key before the chunk (~0 never equals a group index).
-        uint64_t prev_b = c0 > 0 ? (uint64_t(sorted[c0 - 1]) & code_mask) >> sh : ~0ull;
+        uint64_t prev_b = ~0ull;
+        if (c0 > 0) {
+            uint64_t pk = uint64_t(sorted[c0 - 1]);
+            if constexpr (BUCKETED) pk |= cur << bk.chunk_bits;  // cur: the bucket of c0 - 1
+            prev_b = (pk & code_mask) >> sh;
+        }
         bool open = false;  // a group owned by this wave is being accumulated
         uint64_t cur_b = 0;
         bool done = false;
@@ -202,15 +263,16 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             open = false;
         };
 
-        uint64_t k[kReduceGroups], nxt[kReduceGroups];
+        uint64_t k[G], nxt[G];
         load_groups(sorted, c0, n, k);
         // No load may be pending at the loop head (see the wait at its end).
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        add_bucket(c0, k);
 #pragma unroll
-        for (int g = 0; g < kReduceGroups; ++g) k[g] &= code_mask;
-        for (uint64_t base = c0; !done; base += uint64_t(kReduceGroups) * kWave) {
+        for (int g = 0; g < G; ++g) k[g] &= code_mask;
+        for (uint64_t base = c0; !done; base += uint64_t(G) * kWave) {
             // The next batch of the chunk is in flight while this one is reduced.
-            const uint64_t next = base + uint64_t(kReduceGroups) * kWave;
+            const uint64_t next = base + uint64_t(G) * kWave;
             const bool prefetch = next < c1;
             if (prefetch) load_groups(sorted, next, n, nxt);
             // Group starts of the whole batch first.  Keys are sorted by group, so
@@ -218,9 +280,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             // it (lane 0 may still start one); only the others need the lane
             // shuffle.  Lanes past the end repeat the last key: they never start
             // a group, and OR-ing their bit again is harmless.
-            uint64_t H[kReduceGroups];
+            uint64_t H[G];
 #pragma unroll
-            for (int g = 0; g < kReduceGroups; ++g) {
+            for (int g = 0; g < G; ++g) {
                 const uint64_t b = k[g] >> sh;
                 const uint64_t b0 = readlane64(b, 0), b63 = readlane64(b, kWave - 1);
                 if (b0 == b63) {
@@ -233,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
                 prev_b = b63;
             }
 #pragma unroll
-            for (int g = 0; g < kReduceGroups; ++g) {
+            for (int g = 0; g < G; ++g) {
                 if (done) break;
                 const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
                 const uint64_t key = k[g];
@@ -282,8 +344,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             // next batch's keys are then complete at the loop head, so the
             // batch's first uses do not wait behind the prefetch issued there.
             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+            add_bucket(next, nxt);
 #pragma unroll
-            for (int g = 0; g < kReduceGroups; ++g) k[g] = nxt[g] & code_mask;
+            for (int g = 0; g < G; ++g) k[g] = nxt[g] & code_mask;
         }
         if (lane == 0) counts[chunk] = written;
     }
@@ -413,11 +476,16 @@ __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __rest
     // shuffles below.
     for (uint64_t r0 = uint64_t(blockIdx.x) * per_round + (threadIdx.x & ~uint32_t(kWave - 1)); r0 < n;
          r0 += uint64_t(gridDim.x) * per_round) {
+        // The round's masks are all in flight before the first insert (indices
+        // past the end read the last mask and are not inserted).
+        uint64_t masks[kInsertIpt];
+#pragma unroll
+        for (int j = 0; j < kInsertIpt; ++j) masks[j] = lp_mask[min(r0 + uint64_t(j) * kThreads + lane, n - 1)];
 #pragma unroll
         for (int j = 0; j < kInsertIpt; ++j) {
             const uint64_t q = r0 + uint64_t(j) * kThreads + lane;
             const bool valid = q < n;
-            const uint64_t mask = valid ? lp_mask[q] : 0ull;
+            const uint64_t mask = valid ? masks[j] : 0ull;
             bool follower;
             const uint32_t leader = elect_leader(mask, valid, elect, follower);
             uint32_t slot = 0;
@@ -446,10 +514,15 @@ __global__ __launch_bounds__(kThreads) void k_slot_count(const uint64_t* __restr
     if (blockIdx.x == 0 && threadIdx.x == 0) *slots_out = total;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint32_t occ = 0;
+        // Unconditional loads (indices past the end read the last slot and are
+        // not counted), so that all are in flight at once.
+        uint64_t v[kListIpt];
+#pragma unroll
+        for (int j = 0; j < kListIpt; ++j) v[j] = tk[min(tile * kListTile + uint64_t(j) * kThreads + threadIdx.x, total - 1)];
 #pragma unroll
         for (int j = 0; j < kListIpt; ++j) {
             const uint64_t i = tile * kListTile + uint64_t(j) * kThreads + threadIdx.x;
-            occ += (i < total && tk[i] != 0) ? 1u : 0u;
+            occ += (i < total && v[j] != 0) ? 1u : 0u;
         }
         const uint64_t cnt = block_reduce_sum<kThreads>(occ, s_red);
         if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
@@ -1441,15 +1514,25 @@ template <typename K>
 void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32_t group_bits, uint64_t* reg_code,
                         uint64_t* reg_mask, uint64_t* lp_code, uint64_t* lp_mask, uint64_t lp_capacity,
                         uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
-                        uint64_t code_mask)
+                        uint64_t code_mask, const ChunkPrefix& buckets)
 {
     const uint64_t chunks = (n_bound + kReduceChunk - 1) / kReduceChunk;
     const int grid = grid_for((chunks + kReduceWaves - 1) / kReduceWaves, 8);
     {
         TimedLaunch t(kKLeafReduce, s);
-        hipLaunchKernelGGL(k_leaf_reduce<K>, dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask, group_bits,
-                           reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
-                           reinterpret_cast<unsigned long long*>(&c->error));
+        bool bucketed = false;
+        if constexpr (sizeof(K) == 4) {
+            if (buckets.offsets) {
+                bucketed = true;
+                hipLaunchKernelGGL((k_leaf_reduce<K, true>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n,
+                                   code_mask, group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
+                                   reinterpret_cast<unsigned long long*>(&c->error), buckets);
+            }
+        }
+        if (!bucketed)
+            hipLaunchKernelGGL((k_leaf_reduce<K, false>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask,
+                               group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
+                               reinterpret_cast<unsigned long long*>(&c->error), ChunkPrefix{});
     }
     launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kReduceChunk, &c->leaf_positions);
     {
@@ -1461,10 +1544,10 @@ void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32
 }
 template void launch_leaf_reduce<uint64_t>(hipStream_t, const uint64_t*, BuildCounters*, uint32_t, uint64_t*,
                                            uint64_t*, uint64_t*, uint64_t*, uint64_t, uint32_t*, uint64_t*, uint64_t,
-                                           int, uint64_t);
+                                           int, uint64_t, const ChunkPrefix&);
 template void launch_leaf_reduce<uint32_t>(hipStream_t, const uint32_t*, BuildCounters*, uint32_t, uint64_t*,
                                            uint64_t*, uint64_t*, uint64_t*, uint64_t, uint32_t*, uint64_t*, uint64_t,
-                                           int, uint64_t);
+                                           int, uint64_t, const ChunkPrefix&);
 
 void launch_table_clear(hipStream_t s, uint64_t* keys, const uint64_t* count_ptr, uint32_t* log2_out,
                         uint32_t load_shift, uint32_t overflow, uint64_t slot_bound)

@@ -133,9 +133,10 @@ struct ChunkPrefix {
 // device from the chunk offsets (chunk_start: num_chunks + 1 device entries;
 // *n_ptr keys in all, at most n_bound; *n_ptr == 0 plans nothing).  Sorts the
 // chunk-local bits [lo, chunk_bits) (at most kSegMaxBits of them, in one or
-// two passes of at most kMaxSegDigitBits) and puts each chunk's index above
-// chunk_bits.  Returns 0 (result in keys_a), 1 (keys_b), or -1 when the bit
-// span does not fit.  first_bits: the first pass's digit width (0: half the
+// two passes of at most kMaxSegDigitBits) and writes the keys as 32-bit
+// chunk-local keys (chunk_bits <= 32); a reader restores key i's chunk index
+// from chunk_start (ChunkPrefix).  Returns 0 (result in keys_a), 1 (keys_b),
+// or -1 when the bit span does not fit.  first_bits: the first pass's digit width (0: half the
 // bits), clamped so both passes have at most kMaxSegDigitBits.
 struct SegTile {
     uint64_t start;
@@ -163,7 +164,7 @@ struct SegScratch {
     uint64_t* ntiles;
 };
 template <typename KI>
-int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint32_t* keys_a, uint32_t* keys_b,
                           const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
                           const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits = 0);
@@ -172,12 +173,12 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_
 // top digit first: a pass on the top digit writes each key's bits below
 // top_shift (< 2^32) into `local`, grouped by top digit, with the groups'
 // starts in bucket_start (2^top_bits + 1 entries); the segmented sort then
-// sorts [lo, top_shift) inside each group and restores the top digit.  zero:
+// sorts [lo, top_shift) inside each group, still as 32-bit bucket-local keys.  zero:
 // kRadix u32, zero at launch; counts: sort_count_words(n_bound) u32.  Returns
 // as launch_segmented_sort (0: keys_a, 1: keys_b), -1 when the bits do not fit
 // (top_shift > 32, or more than kSegMaxBits below it).
-int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint64_t* keys_a,
-                          uint64_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
+int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint32_t* keys_a,
+                          uint32_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
                           const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
                           unsigned long long* key_or, uint64_t* bucket_start, const SegScratch& sc,
                           unsigned long long* err);
@@ -244,11 +245,14 @@ void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets
 // kPackTileItems keys), then compacted into lp_code / lp_mask (lp_capacity
 // entries); sets
 // c->leaf_positions and adds the leaf level's parent count to c->pre[leaf_level].
+// buckets.offsets non-null (32-bit keys only): the keys are local to the
+// buckets of a segmented sort, key i's code is (b << buckets.chunk_bits) | key
+// for the bucket b holding position i.
 template <typename K>
 void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32_t group_bits, uint64_t* reg_code,
                         uint64_t* reg_mask, uint64_t* lp_code, uint64_t* lp_mask, uint64_t lp_capacity,
                         uint32_t* tile_counts, uint64_t* tile_offsets, uint64_t n_bound, int leaf_level,
-                        uint64_t code_mask);
+                        uint64_t code_mask, const ChunkPrefix& buckets = ChunkPrefix{});
 
 // Clears an ordered (load_shift, overflow) or hashed table sized from *count_ptr
 // and records its log2 size.

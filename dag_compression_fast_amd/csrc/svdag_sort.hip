@@ -800,12 +800,14 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(const KI* __restrict_
     const TileWalk walk = xcd_tiles(ntiles);
     for (uint64_t t = walk.tile; t < walk.end; t += walk.step) {
         const SegTile tile = tiles[t];
-        // The whole tile's keys are in flight before the counters are cleared.
+        // The whole tile's keys are in flight before the counters are cleared:
+        // positions past the tile's end read its last key (a tile holds at
+        // least one), so no load sits behind a branch, and are not counted.
         KI key[kSegIpt];
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
-            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
-            key[i] = j < tile.count ? __builtin_nontemporal_load(keys + tile.start + j) : KI(0);
+            const uint32_t j = min(uint32_t(i) * kSegThreads + threadIdx.x, tile.count - 1);
+            key[i] = __builtin_nontemporal_load(keys + tile.start + j);
         }
         for (uint32_t i = threadIdx.x; i < R; i += kSegThreads) s_cnt[i] = 0;
         __syncthreads();
@@ -824,36 +826,62 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(const KI* __restrict_
     if (FIRST && bad) atomicOr(err, (unsigned long long)kErrKeyRange);
 }
 
-// One wave per (chunk, 64 digits): lane d walks digit d's counts down the
-// chunk's tiles (each step reads and writes 64 adjacent counts of one tile),
-// replacing them by their exclusive prefix; the digit's total goes to
-// totals[c * R + d].  Loads run kScanAhead tiles ahead of the running sum.
+// One block per (chunk, 64 digits): lane l of every wave owns digit d = group
+// * 64 + l; the chunk's tiles are split into kColWaves consecutive slices, one
+// per wave.  Each wave sums its slice (reading 64 adjacent counts of one tile
+// per step, kScanAhead tiles in flight), the slice sums are combined in LDS,
+// and each wave rewrites its slice with exclusive prefixes; the digit's total
+// goes to totals[c * R + d].
 constexpr uint32_t kScanAhead = 8;
+constexpr int kColWaves = 8;
+constexpr int kColThreads = kColWaves * kWave;
 
-__global__ __launch_bounds__(kWave) void k_seg_scan_cols(const uint32_t* __restrict__ chunk_tiles, uint64_t num_chunks,
-                                                         uint32_t R, uint32_t* __restrict__ counts,
-                                                         uint32_t* __restrict__ totals)
+__global__ __launch_bounds__(kColThreads) void k_seg_scan_cols(const uint32_t* __restrict__ chunk_tiles,
+                                                               uint64_t num_chunks, uint32_t R,
+                                                               uint32_t* __restrict__ counts,
+                                                               uint32_t* __restrict__ totals)
 {
-    const uint32_t lane = lane_id();
+    __shared__ uint32_t s_part[kColWaves][kWave];
+    const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
     const uint32_t groups = (R + kWave - 1) / kWave;
     for (uint64_t job = blockIdx.x; job < num_chunks * groups; job += gridDim.x) {
         const uint64_t c = job / groups;
         const uint32_t d = uint32_t(job % groups) * kWave + lane;
         const uint32_t t0 = chunk_tiles[c], nt = chunk_tiles[c + 1] - t0;
+        const uint32_t per = (nt + kColWaves - 1) / kColWaves;
+        const uint32_t b = min(nt, w * per), e = min(nt, b + per);
         uint32_t* col = counts + uint64_t(t0) * R + d;
-        uint32_t run = 0;
-        if (d < R) {
-            for (uint32_t i0 = 0; i0 < nt; i0 += kScanAhead) {
+        const bool mine = d < R;
+        uint32_t sum = 0;
+        for (uint32_t i0 = b; i0 < e; i0 += kScanAhead) {
+            uint32_t v[kScanAhead];
+#pragma unroll
+            for (uint32_t k = 0; k < kScanAhead; ++k) v[k] = mine && i0 + k < e ? col[uint64_t(i0 + k) * R] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kScanAhead; ++k) sum += v[k];
+        }
+        s_part[w][lane] = sum;
+        __syncthreads();
+        uint32_t run = 0, total = 0;
+#pragma unroll
+        for (int i = 0; i < kColWaves; ++i) {
+            const uint32_t x = s_part[i][lane];
+            run += uint32_t(i) < w ? x : 0u;
+            total += x;
+        }
+        __syncthreads();  // s_part is rewritten by the next job
+        if (mine) {
+            for (uint32_t i0 = b; i0 < e; i0 += kScanAhead) {
                 uint32_t v[kScanAhead];
 #pragma unroll
-                for (uint32_t k = 0; k < kScanAhead; ++k) v[k] = i0 + k < nt ? col[uint64_t(i0 + k) * R] : 0u;
+                for (uint32_t k = 0; k < kScanAhead; ++k) v[k] = i0 + k < e ? col[uint64_t(i0 + k) * R] : 0u;
 #pragma unroll
                 for (uint32_t k = 0; k < kScanAhead; ++k) {
-                    if (i0 + k < nt) col[uint64_t(i0 + k) * R] = run;
+                    if (i0 + k < e) col[uint64_t(i0 + k) * R] = run;
                     run += v[k];
                 }
             }
-            totals[c * R + d] = run;
+            if (w == 0) totals[c * R + d] = total;
         }
     }
 }
@@ -887,27 +915,26 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scan_digits(const uint64_t*
     }
 }
 
-// The scatter of one pass.  FIRST: bins = the pass digit, and the keys get
-// their chunk index above chunk_bits as they are written.  Else: bins = digit *
-// span + (previous digit - the tile's smallest previous digit), span = the
-// tile's range of previous digits (the input is sorted on them, so a tile
-// usually holds one or two).  Output offset of tile t's digit d: starts[c * R +
-// d] plus the row prefix in counts.  The first pass stages the keys at their
-// input width and needs only R <= kSegMaxDigits bins, which keeps its LDS (and
-// so its workgroups per CU) within reach of the later passes' double.  Tiles
-// are walked XCD by XCD (the grid is a multiple of kXcds), so neighbouring
-// tiles, whose runs of one digit are adjacent in the output, are written
-// through one L2.
+// The scatter of one pass; keys are written as 32-bit chunk-local keys.
+// FIRST: bins = the pass digit.  Else: bins = digit * span + (previous digit -
+// the tile's smallest previous digit), span = the tile's range of previous
+// digits (the input is sorted on them, so a tile usually holds one or two).
+// Output offset of tile t's digit d: starts[c * R + d] plus the row prefix in
+// counts.  The first pass needs only R <= kSegMaxDigits bins, which keeps its
+// LDS (and so its workgroups per CU) within reach of the later passes' double.
+// Tiles are walked XCD by XCD (the grid is a multiple of kXcds), so
+// neighbouring tiles, whose runs of one digit are adjacent in the output, are
+// written through one L2.
 template <bool FIRST, typename KI>
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const KI* __restrict__ keys_in,
-                                                             uint64_t* __restrict__ keys_out,
+                                                             uint32_t* __restrict__ keys_out,
                                                              const SegTile* __restrict__ tiles,
                                                              const uint64_t* __restrict__ ntiles_ptr, uint32_t shift,
                                                              uint32_t dmask, uint32_t prev_shift, uint32_t prev_bits,
-                                                             uint32_t chunk_bits, const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ counts,
                                                              const uint32_t* __restrict__ starts)
 {
-    using KS = std::conditional_t<FIRST, KI, uint64_t>;  // staged key
+    using KS = KI;  // staged key
     constexpr uint32_t kBins = FIRST ? kSegMaxDigits : kSegMaxBins;
     constexpr int kBinsPerThread = int(kBins / kSegThreads);
     static_assert(kBinsPerThread * kSegThreads == int(kBins), "bins covered by the row-wise scan");
@@ -926,14 +953,22 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const KI* __restric
         KS key[kSegIpt];
         uint32_t br[kSegIpt];  // bin << 16 | rank within the bin
         uint32_t lo = 0xFFFFFFFFu, hi = 0;
+        // Every load is issued before any key is used; positions past the
+        // tile's end read its last key (no load behind a branch) and are
+        // skipped below.
 #pragma unroll
         for (int i = 0; i < kSegIpt; ++i) {
-            const uint32_t j = uint32_t(i) * kSegThreads + threadIdx.x;
-            key[i] = j < tile.count ? KS(__builtin_nontemporal_load(keys_in + tile.start + j)) : KS(0);
-            if (!FIRST && j < tile.count) {
-                const uint32_t p = uint32_t(key[i] >> prev_shift) & prev_mask;
-                lo = min(lo, p);
-                hi = max(hi, p);
+            const uint32_t j = min(uint32_t(i) * kSegThreads + threadIdx.x, tile.count - 1);
+            key[i] = KS(__builtin_nontemporal_load(keys_in + tile.start + j));
+        }
+        if constexpr (!FIRST) {
+#pragma unroll
+            for (int i = 0; i < kSegIpt; ++i) {
+                if (uint32_t(i) * kSegThreads + threadIdx.x < tile.count) {
+                    const uint32_t p = uint32_t(key[i] >> prev_shift) & prev_mask;
+                    lo = min(lo, p);
+                    hi = max(hi, p);
+                }
             }
         }
         uint32_t pmin = 0, span = 1;
@@ -1009,10 +1044,9 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const KI* __restric
             if (j < tile.count) s_keys[s_bin[br[i] >> 16] + (br[i] & 0xFFFFu)] = key[i];
         }
         __syncthreads();
-        const uint64_t prefix = FIRST ? uint64_t(tile.chunk) << chunk_bits : 0ull;
         for (uint32_t j = threadIdx.x; j < tile.count; j += kSegThreads) {
             const KS k = s_keys[j];
-            keys_out[s_base[uint32_t(uint64_t(k) >> shift) & dmask] + j] = uint64_t(k) | prefix;
+            keys_out[s_base[uint32_t(uint64_t(k) >> shift) & dmask] + j] = uint32_t(k);
         }
         __syncthreads();
     }
@@ -1021,12 +1055,12 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const KI* __restric
 }  // namespace
 
 template <typename KI>
-int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_a, uint64_t* keys_b,
+int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint32_t* keys_a, uint32_t* keys_b,
                           const uint64_t* chunk_start, uint64_t num_chunks, const uint64_t* n_ptr, uint64_t n_bound,
                           const SegScratch& sc, uint32_t lo, uint32_t chunk_bits, unsigned long long* err,
                           uint32_t first_bits)
 {
-    if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || n_bound == 0) return -1;
+    if (chunk_bits <= lo || chunk_bits - lo > kSegMaxBits || chunk_bits > 32 || n_bound == 0) return -1;
     const uint32_t B = chunk_bits - lo;
     // First pass digit width: half the bits unless asked otherwise, and never
     // leaving either pass wider than kMaxSegDigitBits.
@@ -1050,10 +1084,10 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_
                            dim3(256), 0, stream, chunk_start, sc.chunk_tiles, num_chunks, sc.ntiles, sc.tiles);
     }
     // The first pass reads the caller's keys (KI), the second the first's output.
-    auto pass = [&](bool first, const void* src, uint64_t* dst, uint32_t shift, uint32_t bits) {
+    auto pass = [&](bool first, const void* src, uint32_t* dst, uint32_t shift, uint32_t bits) {
         const uint32_t mask = (1u << bits) - 1u, R = mask + 1;
         const uint64_t col_jobs = num_chunks * ((R + kWave - 1) / kWave);
-        const int col_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(col_jobs, uint64_t(cus) * 64)));
+        const int col_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(col_jobs, uint64_t(cus) * 16)));
         {
             TimedLaunch t(kKSortCount, stream);
             if (first)
@@ -1061,13 +1095,14 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_
                                    static_cast<const KI*>(src), sc.tiles, sc.ntiles, shift, mask,
                                    chunk_bits, sc.counts, err);
             else
-                hipLaunchKernelGGL((k_seg_count<false, uint64_t>), dim3(grid), dim3(kSegThreads), 0, stream,
-                                   static_cast<const uint64_t*>(src), sc.tiles, sc.ntiles, shift,
+                hipLaunchKernelGGL((k_seg_count<false, uint32_t>), dim3(grid), dim3(kSegThreads), 0, stream,
+                                   static_cast<const uint32_t*>(src), sc.tiles, sc.ntiles, shift,
                                    mask, chunk_bits, sc.counts, err);
         }
         {
             TimedLaunch t(kKSortScan, stream);
-            hipLaunchKernelGGL(k_seg_scan_cols, dim3(col_grid), dim3(kWave), 0, stream, sc.chunk_tiles, num_chunks, R,
+            hipLaunchKernelGGL(k_seg_scan_cols, dim3(col_grid), dim3(kColThreads), 0, stream, sc.chunk_tiles,
+                               num_chunks, R,
                                sc.counts, sc.totals);
             hipLaunchKernelGGL(k_seg_scan_digits, dim3(chunk_grid), dim3(kSegThreads), 0, stream, chunk_start,
                                num_chunks, R, sc.totals, sc.starts);
@@ -1075,12 +1110,12 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_
         TimedLaunch t(kKSortScatterKeys, stream);
         if (first)
             hipLaunchKernelGGL((k_seg_scatter<true, KI>), dim3(grid), dim3(kSegThreads), 0, stream,
-                               static_cast<const KI*>(src), dst, sc.tiles, sc.ntiles, shift, mask, 0u,
-                               0u, chunk_bits, sc.counts, sc.starts);
+                               static_cast<const KI*>(src), dst, sc.tiles, sc.ntiles, shift, mask, 0u, 0u, sc.counts,
+                               sc.starts);
         else
-            hipLaunchKernelGGL((k_seg_scatter<false, uint64_t>), dim3(grid), dim3(kSegThreads), 0, stream,
-                               static_cast<const uint64_t*>(src), dst, sc.tiles, sc.ntiles, shift,
-                               mask, lo, b1, chunk_bits, sc.counts, sc.starts);
+            hipLaunchKernelGGL((k_seg_scatter<false, uint32_t>), dim3(grid), dim3(kSegThreads), 0, stream,
+                               static_cast<const uint32_t*>(src), dst, sc.tiles, sc.ntiles, shift, mask, lo, b1,
+                               sc.counts, sc.starts);
     };
     pass(true, keys_in, keys_a, lo, b1);
     if (b2 == 0) return 0;
@@ -1088,10 +1123,10 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint64_t* keys_
     return 1;
 }
 
-template int launch_segmented_sort<uint64_t>(hipStream_t, const uint64_t*, uint64_t*, uint64_t*, const uint64_t*,
+template int launch_segmented_sort<uint64_t>(hipStream_t, const uint64_t*, uint32_t*, uint32_t*, const uint64_t*,
                                              uint64_t, const uint64_t*, uint64_t, const SegScratch&, uint32_t,
                                              uint32_t, unsigned long long*, uint32_t);
-template int launch_segmented_sort<uint32_t>(hipStream_t, const uint32_t*, uint64_t*, uint64_t*, const uint64_t*,
+template int launch_segmented_sort<uint32_t>(hipStream_t, const uint32_t*, uint32_t*, uint32_t*, const uint64_t*,
                                              uint64_t, const uint64_t*, uint64_t, const SegScratch&, uint32_t,
                                              uint32_t, unsigned long long*, uint32_t);
 
@@ -1100,9 +1135,10 @@ template int launch_segmented_sort<uint32_t>(hipStream_t, const uint32_t*, uint6
 // [top_shift, top_shift + top_bits) partitions the keys into 2^top_bits
 // Morton-ordered buckets and writes each key as a 32-bit key local to its
 // bucket (its bits below top_shift); the segmented sort then sorts the bits
-// [lo, top_shift) inside each bucket, reading those 32-bit keys, and puts the
-// bucket index back above them.  Against an LSD sort of the same bits it moves
-// half the bytes in the top pass's writes and in the reads after it.
+// [lo, top_shift) inside each bucket, reading and writing those 32-bit keys
+// (the leaf reduce restores the bucket index from the bucket starts).  Against
+// an LSD sort of the same bits it moves half the bytes in the top pass's writes
+// and in every read and write after it.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -1121,8 +1157,8 @@ __global__ __launch_bounds__(kRadix) void k_digit_starts(const uint32_t* __restr
 
 }  // namespace
 
-int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint64_t* keys_a,
-                          uint64_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
+int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t* local, uint32_t* keys_a,
+                          uint32_t* keys_b, int ipt, uint32_t top_shift, uint32_t top_bits, uint32_t lo,
                           const uint64_t* n_ptr, uint64_t n_bound, uint32_t* zero, uint32_t* counts,
                           unsigned long long* key_or, uint64_t* bucket_start, const SegScratch& sc,
                           unsigned long long* err)

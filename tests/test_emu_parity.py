@@ -98,6 +98,7 @@ def test_emu_merge_and_batched(emu, oracle):
         T.test_batched_chunk_depths(emu, oracle, chunk_depth)
     for first_bits in (4, 8):
         T.test_batched_segmented_digit_split(emu, oracle, first_bits)
+    T.test_batched_many_sparse_chunks(emu, oracle)
 
 
 def test_emu_errors(emu):

@@ -200,6 +200,25 @@ def test_batched_chunk_depths(gpu, oracle, chunk_depth):
     assert_dag_equal(batched.to_numpy(), exp, f"batched d{chunk_depth}")
 
 
+def test_batched_many_sparse_chunks(gpu, oracle):
+    """4096 chunks, few of them filled: the segmented sort's 32-bit
+    chunk-local keys get their chunk back from a chunk table too large for the
+    leaf reduce to stage (more than 512 chunks), across long runs of empty
+    chunks."""
+    rng = np.random.default_rng(4096)
+    chunk_depth, num = 6, 4096
+    filled = set(int(c) for c in rng.choice(num, size=40, replace=False)) | {0, 1, num - 1}
+    chunks = [clustered_keys(rng, chunk_depth, int(rng.integers(1, 9000))) if c in filled
+              else np.zeros(0, dtype=np.uint64) for c in range(num)]
+    offsets = np.zeros(num + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(c) for c in chunks])
+    print(f"[gpu-parity] batched {num} chunks d{chunk_depth}", flush=True)
+    batched = gpu.create_dag_batched(np.concatenate(chunks), offsets, chunk_depth)
+    union = np.concatenate([(np.uint64(i) << np.uint64(3 * chunk_depth)) | c for i, c in enumerate(chunks)])
+    exp = oracle.create_dag(union, chunk_depth + 4)
+    assert_dag_equal(batched.to_numpy(), exp, "batched 4096 sparse chunks")
+
+
 @pytest.mark.parametrize("bits,tile", [(4, 4096), (6, 8192), (7, 4096), (9, 4096), (9, 8192)])
 def test_radix_digit_widths(gpu, oracle, bits, tile):
     from dag_compression_fast_amd import _lib
