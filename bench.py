@@ -50,23 +50,36 @@ def make_keys(svd, spec, threads):
     return svd.gen_sphere_shells(seed, spec["depth"], count, a, b, dup, threads=threads)
 
 
-def algorithmic_bytes(kernel, st, key_bytes, launches):
+def algorithmic_bytes(kernel, st, key_bytes, launches, top_first=False):
     """Algorithmic HBM bytes one step moves through `kernel` (None if not modelled).
 
     N keys, L0 leaf positions, L unique leaves, M_l / Q_l nodes of level l before /
     after dedup, C_l children feeding level l, W words of the flat layout;
-    `launches` = launches of the kernel per step (sort passes)."""
+    `launches` = launches of the kernel per step (sort passes).  top_first: the
+    key sort ran top digit first (its top pass writes 32-bit bucket-local keys,
+    which the segmented passes and the leaf reduce read)."""
     n, l0, lu = st["n"], st["leaf_positions"], st["num_leaves"]
     pre, uniq, words = st["pre"], st["uniq"], st["words"]
     levels = len(pre)
     child = [pre[l + 1] if l + 1 < levels else l0 for l in range(levels)]
+    sorted_bytes = 4 if top_first else key_bytes  # width of the keys the leaf reduce reads
     table = {
-        # Every key-sort pass reads and writes each key once.
+        # The first read of the keys (digit counts, range check).
+        "sort_histogram": key_bytes * n,
+        # LSD passes after the first count the keys they then scatter.
+        "sort_count": launches * key_bytes * n,
+        # Segmented counts read the 32-bit keys once per pass.
+        "seg_count": launches * 4 * n if top_first else None,
+        # Every LSD key-sort pass reads and writes each key once.
         "sort_scatter_keys": launches * 2 * key_bytes * n,
+        # The top-digit pass reads each key and writes it as a 32-bit key.
+        "sort_scatter_top": launches * (key_bytes + 4) * n,
+        # Segmented passes after a top-digit pass read and write 32-bit keys.
+        "seg_scatter": launches * 8 * n if top_first else None,
         # The unique-leaf sort moves (u64 mask, u32 slot) pairs.
         "sort_scatter_pairs": launches * 2 * 12 * lu,
         # grouped keys read once; (code, mask) per leaf into the chunk regions
-        "leaf_reduce": key_bytes * n + 16 * l0,
+        "leaf_reduce": sorted_bytes * n + 16 * l0,
         "leaf_compact": 32 * l0,
         # mask read and slot write per leaf position, table key per unique mask
         "leaf_insert": 12 * l0 + 8 * lu,
@@ -85,6 +98,16 @@ def read_traffic(kernel, workload):
     if not os.path.exists(path):
         return None
     m = re.search(r"HBM bytes per launch \(scaled\):\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def read_step_traffic(workload):
+    """HBM bytes of one whole step (every kernel of create_dag + create_final_dag)
+    from the committed rocprofv3 PMC summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"pmc_step_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"HBM bytes per step \(scaled\):\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -863,7 +886,8 @@ def main():
     ctx.profile_enable([])
 
     launches_per_step = launches / args.steps
-    alg = algorithmic_bytes(dominant, stats, 8, launches_per_step)
+    top_first = "sort_scatter_top" in table
+    alg = algorithmic_bytes(dominant, stats, 8, launches_per_step, top_first)
     roofline = None
     if alg is not None:
         per_launch = alg / launches_per_step
@@ -873,6 +897,21 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
                     "bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2)}
+    # The whole step against the HBM roofline: algorithmic bytes of every
+    # modelled kernel (one warm-up step's launches), and the counter-measured
+    # bytes of a step where a committed PMC summary holds them.
+    step_alg = 0.0
+    for k, (_ms, cnt) in table.items():
+        b = algorithmic_bytes(k, stats, 8, cnt, top_first)
+        if b is not None:
+            step_alg += b
+    step_traffic = read_step_traffic(args.workload)
+    step_s = ms_per_step / 1e3
+    whole_step = {"algorithmic_bytes": int(step_alg), "frac": round(step_alg / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                  "traffic": step_traffic,
+                  "traffic_frac": round(step_traffic / step_s / 1e9 / HBM_PEAK_GBS, 4) if step_traffic else None}
+    if roofline is not None:
+        roofline["whole_step"] = whole_step
 
     cpu = None
     check = "not checked"

@@ -141,11 +141,15 @@ __device__ __forceinline__ uint64_t bucket_of(const Start& start, uint64_t lo, u
 
 constexpr uint32_t kLdsBuckets = 512;  // bucket tables up to this size are staged in LDS
 
-// BUCKETED: 32-bit bucket-local keys (see launch_leaf_reduce); each wave keeps
-// the bucket of its current position and its end, so a batch inside one
-// bucket takes the prefix from a register; batches that cross a bucket end
+// Where the leaf reduce finds bucket starts: no buckets (keys are codes), a
+// copy in LDS (small tables), or the table in global memory.
+enum BucketMode { kBucketsNone = 0, kBucketsLds = 1, kBucketsGlobal = 2 };
+
+// BM != kBucketsNone: 32-bit bucket-local keys (see launch_leaf_reduce); each
+// wave keeps the bucket of its current position and its end, so a batch inside
+// one bucket takes the prefix from a register; batches that cross a bucket end
 // (a few per bucket) search per lane.
-template <typename K, bool BUCKETED>
+template <typename K, int BM>
 __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ sorted,
                                                           const uint64_t* __restrict__ n_ptr, uint64_t code_mask,
                                                           uint32_t r, uint64_t* __restrict__ reg_code,
@@ -157,14 +161,17 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
     __shared__ uint64_t s_tab[kReduceWaves][kReduceTable];
     __shared__ uint64_t s_parents[kReduceWaves];
     constexpr int G = kReduceGroups<K>;
+    constexpr bool BUCKETED = BM != kBucketsNone;
     // Bucket starts (positions < 2^30) staged in LDS when the table is small.
-    __shared__ uint32_t s_bstart[BUCKETED ? kLdsBuckets + 1 : 1];
-    const bool staged = BUCKETED && bk.num_chunks <= kLdsBuckets;  // the same for the whole block
-    if (staged) {
+    __shared__ uint32_t s_bstart[BM == kBucketsLds ? kLdsBuckets + 1 : 1];
+    if constexpr (BM == kBucketsLds) {
         for (uint32_t i = threadIdx.x; i <= bk.num_chunks; i += kThreads) s_bstart[i] = uint32_t(bk.offsets[i]);
         __syncthreads();
     }
-    auto start_of = [&](uint64_t b) -> uint64_t { return staged ? uint64_t(s_bstart[b]) : bk.offsets[b]; };
+    auto start_of = [&](uint64_t b) -> uint64_t {
+        if constexpr (BM == kBucketsLds) return s_bstart[b];
+        else return bk.offsets[b];
+    };
     const uint32_t lane = lane_id();
     // The wave index as a scalar, so that the chunk loop and the group
     // bookkeeping below compile to scalar control flow.
@@ -1524,14 +1531,19 @@ void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32
         if constexpr (sizeof(K) == 4) {
             if (buckets.offsets) {
                 bucketed = true;
-                hipLaunchKernelGGL((k_leaf_reduce<K, true>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n,
-                                   code_mask, group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
-                                   reinterpret_cast<unsigned long long*>(&c->error), buckets);
+                if (buckets.num_chunks <= kLdsBuckets)
+                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsLds>), dim3(grid), dim3(kThreads), 0, s, sorted,
+                                       &c->n, code_mask, group_bits, reg_code, reg_mask, tile_counts,
+                                       &c->pre[leaf_level], reinterpret_cast<unsigned long long*>(&c->error), buckets);
+                else
+                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsGlobal>), dim3(grid), dim3(kThreads), 0, s, sorted,
+                                       &c->n, code_mask, group_bits, reg_code, reg_mask, tile_counts,
+                                       &c->pre[leaf_level], reinterpret_cast<unsigned long long*>(&c->error), buckets);
             }
         }
         if (!bucketed)
-            hipLaunchKernelGGL((k_leaf_reduce<K, false>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n, code_mask,
-                               group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
+            hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsNone>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n,
+                               code_mask, group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
                                reinterpret_cast<unsigned long long*>(&c->error), ChunkPrefix{});
     }
     launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kReduceChunk, &c->leaf_positions);

@@ -47,6 +47,9 @@ enum KernelId : int {
     kKVoxWrite,
     kKTracePaths,
     kKResidentInsert,
+    kKSortScatterTop,  // top-digit pass of a top-digit-first key sort (64-bit in, 32-bit out)
+    kKSegCount,        // segmented sort: per-tile counts
+    kKSegScatter,      // segmented sort: scatter passes
     kNumKernelIds
 };
 

@@ -1089,7 +1089,7 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint32_t* keys_
         const uint64_t col_jobs = num_chunks * ((R + kWave - 1) / kWave);
         const int col_grid = int(std::max<uint64_t>(1, std::min<uint64_t>(col_jobs, uint64_t(cus) * 16)));
         {
-            TimedLaunch t(kKSortCount, stream);
+            TimedLaunch t(kKSegCount, stream);
             if (first)
                 hipLaunchKernelGGL((k_seg_count<true, KI>), dim3(grid), dim3(kSegThreads), 0, stream,
                                    static_cast<const KI*>(src), sc.tiles, sc.ntiles, shift, mask,
@@ -1107,7 +1107,7 @@ int launch_segmented_sort(hipStream_t stream, const KI* keys_in, uint32_t* keys_
             hipLaunchKernelGGL(k_seg_scan_digits, dim3(chunk_grid), dim3(kSegThreads), 0, stream, chunk_start,
                                num_chunks, R, sc.totals, sc.starts);
         }
-        TimedLaunch t(kKSortScatterKeys, stream);
+        TimedLaunch t(kKSegScatter, stream);
         if (first)
             hipLaunchKernelGGL((k_seg_scatter<true, KI>), dim3(grid), dim3(kSegThreads), 0, stream,
                                static_cast<const KI*>(src), dst, sc.tiles, sc.ntiles, shift, mask, 0u, 0u, sc.counts,
@@ -1194,7 +1194,7 @@ int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t*
                                    n_ptr, stride, uint64_t(kSortThreads) * I);
             hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(kRadix), 0, stream, zero, R, n_ptr, bucket_start);
         }
-        TimedLaunch t(kKSortScatterKeys, stream);
+        TimedLaunch t(kKSortScatterTop, stream);
         hipLaunchKernelGGL((k_sort_scatter<uint64_t, false, kRankUnstable, I, kSortThreads, uint32_t>),
                            dim3(scatter_grid), dim3(kSortThreads), kUnstableExtraLds, stream, keys_in, local, nullptr,
                            nullptr, n_ptr, top_shift, dmask, counts, stride, ChunkPrefix{}, 0u, 0u, nullptr, 64u);
