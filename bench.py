@@ -76,8 +76,11 @@ def algorithmic_bytes(kernel, st, key_bytes, launches, top_first=False):
         "sort_scatter_top": launches * (key_bytes + 4) * n,
         # Segmented passes after a top-digit pass read and write 32-bit keys.
         "seg_scatter": launches * 8 * n if top_first else None,
-        # The unique-leaf sort moves (u64 mask, u32 slot) pairs.
+        # The unique-leaf sort moves (u64 mask, u32 slot) pairs; its first read
+        # and its later passes' counts read the masks.
         "sort_scatter_pairs": launches * 2 * 12 * lu,
+        "pair_histogram": 8 * lu,
+        "pair_count": launches * 8 * lu,
         # grouped keys read once; (code, mask) per leaf into the chunk regions
         "leaf_reduce": sorted_bytes * n + 16 * l0,
         "leaf_compact": 32 * l0,

@@ -41,7 +41,8 @@ const char* const kKernelNames[kNumKernelIds] = {
     "rank_of_slot",   "level_count", "level_build",   "level_check",       "extract_count",      "level_extract",
     "final_count",    "final_positions", "final_write", "merge_insert",      "sketch",             "level_tail",
     "slot_list",      "query_voxels",    "vox_setup",   "vox_count",         "vox_write",
-    "trace_paths",    "resident_insert", "sort_scatter_top", "seg_count",   "seg_scatter"};
+    "trace_paths",    "resident_insert", "sort_scatter_top", "seg_count",   "seg_scatter",
+    "pair_histogram", "pair_count"};
 
 // Event pairs around the launches of the kernels selected by `mask`; resolved
 // into per-kernel totals once the stream has drained.

@@ -50,6 +50,8 @@ enum KernelId : int {
     kKSortScatterTop,  // top-digit pass of a top-digit-first key sort (64-bit in, 32-bit out)
     kKSegCount,        // segmented sort: per-tile counts
     kKSegScatter,      // segmented sort: scatter passes
+    kKPairHistogram,   // key/value sorts (the unique-leaf order): first read
+    kKPairCount,       // key/value sorts: per-tile counts of the later passes
     kNumKernelIds
 };
 

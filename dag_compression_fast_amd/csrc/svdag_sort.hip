@@ -583,7 +583,7 @@ static void launch_pass(hipStream_t stream, const K* src, K* dst, const uint32_t
                    kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(device_cu_count()) * 64) /
                               kXcds * kXcds));
     if (!counted) {
-        TimedLaunch t(kKSortCount, stream);
+        TimedLaunch t(vsrc ? kKPairCount : kKSortCount, stream);
         hipLaunchKernelGGL((k_sort_count<K, IPT>), dim3(grid), dim3(kSortThreads), 0, stream, src, n_ptr, shift, dmask,
                            counts, stride, hist);
     }
@@ -656,7 +656,7 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         constexpr int I = decltype(ipt_tag)::value;
         {
             // Also writes the first pass's tile counts.
-            TimedLaunch t(kKHistogram, stream);
+            TimedLaunch t(vals_in ? kKPairHistogram : kKHistogram, stream);
             hipLaunchKernelGGL((k_histogram<K, I>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in, n_ptr, hp,
                                hist, key_or, counts, stride, prefix);
         }
