@@ -4,8 +4,9 @@ tests/emu builds the library's .hip sources for the CPU kernel emulator with
 -fsanitize=address,undefined (make sanitize).  A child process with the ASan
 runtime preloaded runs builds through the C ABI over the paths the reference
 has device bounds checks for (its ENABLE_CHECKS kernels, DAGCompression.cu:
-278-334): plain builds at several depths and sizes, 32-bit keys, batched
-builds (per-chunk and global sort), merges, the flat layout, copies to the
+278-334): plain builds at several depths and sizes on both key-sort routes,
+32-bit keys, batched builds (per-chunk and global sort, up to 4096 chunks),
+merges, the flat layout, copies to the
 host and back, the resident DAG, a captured batched build, the no-allocation
 entry point, the key sort, point queries and the voxelizer.  Any sanitizer
 report fails the test."""
@@ -34,7 +35,12 @@ T.check_case(m, oracle, T.clustered_keys(rng, 9, 20000), 9, "clustered")
 T.check_case(m, oracle, T.solid_keys(rng, 7, 500), 7, "solid")
 u32 = m.create_dag(np.arange(3000, dtype=np.uint32), 8)
 assert_dag_equal(u32.to_numpy(), oracle.create_dag(np.arange(3000, dtype=np.uint64), 8), "u32")
+import test_top_digit_first as TD
+for depth, n in ((9, 3000), (11, 4000), (12, 3000)):  # one and two segmented passes
+    keys = rng.integers(0, 1 << (3 * depth), size=n, dtype=np.uint64)
+    TD.built(m, oracle, keys, depth, TD.ALWAYS)
 T.test_merge_dags_matches_oracle_and_global(m, oracle)
+T.test_batched_many_sparse_chunks(m, oracle)
 T.test_batched_chunk_depths(m, oracle, 8)
 T.test_batched_chunk_depths(m, oracle, 10)
 d = m.create_dag(T.clustered_keys(rng, 8, 5000), 8)
