@@ -1168,26 +1168,60 @@ static_assert(kBuildTile == kBuildTileItems && kExtractTile == kExtractTileItems
                   kFinalTile == kFinalTileItems && kListTile == kListTileItems,
               "tile sizes shared with the scratch planner");
 
-__device__ __forceinline__ int level_of(const FinalArgs& a, uint64_t g)
+// Per-level starts and arrays of the layout, staged in LDS once per block:
+// a lane's level lookup then reads LDS rather than indexing the kernel
+// argument block by a per-lane level (a dependent memory load per step).
+struct FinalLds {
+    uint64_t first[SVDAG_MAX_LEVELS + 1];  // first node of each level; [num_levels] = total_nodes
+    const uint8_t* masks[SVDAG_MAX_LEVELS];
+    const uint32_t* children[SVDAG_MAX_LEVELS];
+};
+
+// Every thread of the block must call it (one barrier).
+__device__ __forceinline__ void stage_levels(const FinalArgs& a, FinalLds& s)
+{
+    const int t = int(threadIdx.x);
+    if (t < a.num_levels) {
+        s.first[t] = a.levels[t].first_node;
+        s.masks[t] = a.levels[t].masks;
+        s.children[t] = a.levels[t].children;
+    }
+    if (t == a.num_levels) s.first[t] = a.total_nodes;
+    __syncthreads();
+}
+
+// Level of node g (< total_nodes): the last level starting at or before g;
+// empty levels start where the next one does and are passed over.
+__device__ __forceinline__ int level_of(const FinalLds& s, int num_levels, uint64_t g)
 {
     int l = 0;
-    while (l + 1 < a.num_levels && g >= a.levels[l + 1].first_node) ++l;
+    while (l + 1 < num_levels && g >= s.first[l + 1]) ++l;
     return l;
 }
 
-// Word counts (1 + popc(mask)) of the thread's nodes.
-__device__ __forceinline__ uint32_t node_words(const FinalArgs& a, uint64_t first, uint32_t (&sizes)[kFinalIpt])
+// Word counts (1 + popc(mask)) of the thread's nodes.  The levels of the
+// nodes are found first (they depend only on the index), so that all mask
+// loads are in flight at once; nodes past the end read the last node's mask
+// and count 0.
+__device__ __forceinline__ uint32_t node_words(const FinalArgs& a, const FinalLds& s, uint64_t first,
+                                               uint32_t (&sizes)[kFinalIpt])
 {
-    uint32_t sum = 0;
-    int l = first < a.total_nodes ? level_of(a, first) : 0;
+    const uint64_t last = a.total_nodes - 1;  // total_nodes >= 1 (the root)
+    int l = level_of(s, a.num_levels, min(first, last));
+    const uint8_t* src[kFinalIpt];
 #pragma unroll
     for (int j = 0; j < kFinalIpt; ++j) {
-        const uint64_t gi = first + j;
-        sizes[j] = 0;
-        if (gi < a.total_nodes) {
-            while (gi >= a.levels[l].first_node + a.levels[l].num_nodes) ++l;
-            sizes[j] = 1 + __popc(uint32_t(a.levels[l].masks[gi - a.levels[l].first_node]));
-        }
+        const uint64_t gi = min(first + j, last);
+        while (gi >= s.first[l + 1]) ++l;
+        src[j] = s.masks[l] + (gi - s.first[l]);
+    }
+    uint8_t m[kFinalIpt];
+#pragma unroll
+    for (int j = 0; j < kFinalIpt; ++j) m[j] = *src[j];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kFinalIpt; ++j) {
+        sizes[j] = first + j < a.total_nodes ? 1 + __popc(uint32_t(m[j])) : 0u;
         sum += sizes[j];
     }
     return sum;
@@ -1196,10 +1230,12 @@ __device__ __forceinline__ uint32_t node_words(const FinalArgs& a, uint64_t firs
 __global__ __launch_bounds__(kThreads) void k_final_count(FinalArgs a)
 {
     __shared__ uint64_t s_red[kThreads / kWave];
+    __shared__ FinalLds s_lv;
+    stage_levels(a, s_lv);
     const uint64_t ntiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint32_t sizes[kFinalIpt];
-        const uint32_t sum = node_words(a, tile * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt, sizes);
+        const uint32_t sum = node_words(a, s_lv, tile * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt, sizes);
         const uint64_t tot = block_reduce_sum<kThreads>(sum, s_red);
         if (threadIdx.x == 0) a.counts[tile] = uint32_t(tot);
     }
@@ -1208,11 +1244,13 @@ __global__ __launch_bounds__(kThreads) void k_final_count(FinalArgs a)
 __global__ __launch_bounds__(kThreads) void k_final_positions(FinalArgs a)
 {
     __shared__ uint32_t s_scan[kThreads / kWave];
+    __shared__ FinalLds s_lv;
+    stage_levels(a, s_lv);
     const uint64_t ntiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t first = tile * kFinalTile + uint64_t(threadIdx.x) * kFinalIpt;
         uint32_t sizes[kFinalIpt];
-        const uint32_t sum = node_words(a, first, sizes);
+        const uint32_t sum = node_words(a, s_lv, first, sizes);
         uint32_t unused;
         const uint32_t local = block_exclusive_scan<kThreads>(sum, s_scan, unused);
         uint64_t pos = a.offsets[tile] + local;
@@ -1226,16 +1264,17 @@ __global__ __launch_bounds__(kThreads) void k_final_positions(FinalArgs a)
 
 __global__ __launch_bounds__(kThreads) void k_final_write(FinalArgs a)
 {
+    __shared__ FinalLds s_lv;
+    stage_levels(a, s_lv);
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t gi = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; gi < a.total_nodes; gi += stride) {
-        const int l = level_of(a, gi);
-        const FinalLevel& L = a.levels[l];
-        const uint64_t i = gi - L.first_node;
-        const uint32_t mask = L.masks[i];
+        const int l = level_of(s_lv, a.num_levels, gi);
+        const uint64_t i = gi - s_lv.first[l];
+        const uint32_t mask = s_lv.masks[l][i];
         const uint32_t pos = a.positions[gi];
         a.words[pos] = mask;
         const bool last = l + 1 == a.num_levels;
-        const uint4* src = reinterpret_cast<const uint4*>(L.children + 8 * i);
+        const uint4* src = reinterpret_cast<const uint4*>(s_lv.children[l] + 8 * i);
         const uint4 c0 = src[0], c1 = src[1];
         const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
         uint32_t off = 1;
@@ -1244,7 +1283,7 @@ __global__ __launch_bounds__(kThreads) void k_final_write(FinalArgs a)
             if ((mask >> s) & 1u) {
                 const uint32_t c = ch[s];
                 const uint32_t target = last ? uint32_t(a.leaves_start + 2ull * c)
-                                             : a.positions[a.levels[l + 1].first_node + c];
+                                             : a.positions[s_lv.first[l + 1] + c];
                 a.words[pos + off] = target;
                 ++off;
             }

@@ -1194,9 +1194,12 @@ int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t*
                                    n_ptr, stride, uint64_t(kSortThreads) * I);
             hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(kRadix), 0, stream, zero, R, n_ptr, bucket_start);
         }
+        // Without the LDS that holds the 64-bit LSD pass to two workgroups per
+        // CU: this pass writes half the bytes and measured faster at four
+        // (207 against 243 us on 70M keys).
         TimedLaunch t(kKSortScatterTop, stream);
         hipLaunchKernelGGL((k_sort_scatter<uint64_t, false, kRankUnstable, I, kSortThreads, uint32_t>),
-                           dim3(scatter_grid), dim3(kSortThreads), kUnstableExtraLds, stream, keys_in, local, nullptr,
+                           dim3(scatter_grid), dim3(kSortThreads), 0, stream, keys_in, local, nullptr,
                            nullptr, n_ptr, top_shift, dmask, counts, stride, ChunkPrefix{}, 0u, 0u, nullptr, 64u);
     };
     if (ipt == kSortIptLarge) run(std::integral_constant<int, kSortIptLarge>{});
