@@ -690,9 +690,11 @@ constexpr int kBuildTile = kThreads * kBuildIpt;
 __device__ __forceinline__ void parent_runs(const uint64_t* __restrict__ codes, uint64_t n, uint64_t first,
                                             uint64_t (&code)[kBuildIpt], uint32_t& starts, uint32_t& gcount)
 {
+    // Unconditional loads (positions past the end read the last code and are
+    // never starts), so that all of them are in flight at once.  n >= 1 here.
 #pragma unroll
-    for (int j = 0; j < kBuildIpt; ++j) code[j] = first + j < n ? codes[first + j] : 0;
-    const uint64_t before = (first > 0 && first <= n) ? codes[first - 1] : ~0ull;
+    for (int j = 0; j < kBuildIpt; ++j) code[j] = codes[min(first + j, n - 1)];
+    const uint64_t before = first > 0 ? codes[min(first - 1, n - 1)] : ~0ull;
     starts = 0;
     gcount = 0;
 #pragma unroll
@@ -1067,8 +1069,14 @@ __global__ __launch_bounds__(kThreads) void k_extract_count(const uint64_t* __re
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t first = tile * kExtractTile;
         uint32_t occ = 0;
-        for (uint32_t j = threadIdx.x; j < uint32_t(kExtractTile); j += kThreads)
-            occ += (first + j < total && table_keys[first + j] != 0) ? 1u : 0u;
+        // Unconditional loads (slots past the end read the last slot and are
+        // not counted), all in flight at once.
+        uint64_t v[kExtractIpt];
+#pragma unroll
+        for (int k = 0; k < kExtractIpt; ++k) v[k] = table_keys[min(first + uint64_t(k) * kThreads + threadIdx.x, total - 1)];
+#pragma unroll
+        for (int k = 0; k < kExtractIpt; ++k)
+            occ += (first + uint64_t(k) * kThreads + threadIdx.x < total && v[k] != 0) ? 1u : 0u;
         const uint64_t cnt = block_reduce_sum<kThreads>(occ, s_red);
         if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
     }
@@ -1087,9 +1095,20 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
         const int64_t tile_base = int64_t(tile) * kExtractTile;
         const int64_t win_lo = tile_base - kHalo;
         const int64_t win_hi = tile_base + kExtractTile + kHalo;
-        for (int i = threadIdx.x; i < kExtractTile + 2 * kHalo; i += kThreads) {
+        // Unconditional loads of the window (clamped into the table; slots
+        // outside it read as empty), all in flight at once.
+        constexpr int kWinPerThread = (kExtractTile + 2 * kHalo + kThreads - 1) / kThreads;
+        uint64_t wv[kWinPerThread];
+#pragma unroll
+        for (int q = 0; q < kWinPerThread; ++q) {
+            const int64_t s = win_lo + q * kThreads + int64_t(threadIdx.x);
+            wv[q] = g.table_keys[min(max(s, int64_t(0)), int64_t(total) - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < kWinPerThread; ++q) {
+            const int i = q * kThreads + int(threadIdx.x);
             const int64_t s = win_lo + i;
-            s_win[i] = (s >= 0 && s < int64_t(total)) ? g.table_keys[s] : 0;
+            if (i < kExtractTile + 2 * kHalo) s_win[i] = (s >= 0 && s < int64_t(total)) ? wv[q] : 0;
         }
         __syncthreads();
         auto key_at = [&](int64_t s) -> uint64_t {
@@ -1118,16 +1137,26 @@ __global__ __launch_bounds__(kThreads) void k_level_extract(LevelExtractArgs g)
             ch1[j] = src[1];
         }
         if (g.child_rank_of_slot) {
-            // Child slots of the level below -> its ranks (occupied slots only:
-            // the others hold parent 0's record, which may be unset).
+            // Child slots of the level below -> its ranks.  Only occupied
+            // slots' real children are translated (the others hold parent 0's
+            // record, which may be unset); every gather is issued, reading
+            // rank 0 for the rest, so that all are in flight at once.
             const uint32_t* __restrict__ rk = g.child_rank_of_slot;
-            auto tr = [&](uint32_t v) { return v == kEmptyChild ? v : rk[v]; };
+            uint32_t r[kExtractIpt][8];
 #pragma unroll
             for (int j = 0; j < kExtractIpt; ++j) {
-                if ((occ >> j) & 1u) {
-                    ch0[j] = make_uint4(tr(ch0[j].x), tr(ch0[j].y), tr(ch0[j].z), tr(ch0[j].w));
-                    ch1[j] = make_uint4(tr(ch1[j].x), tr(ch1[j].y), tr(ch1[j].z), tr(ch1[j].w));
-                }
+                const bool live = (occ >> j) & 1u;
+                const uint32_t c[8] = {ch0[j].x, ch0[j].y, ch0[j].z, ch0[j].w, ch1[j].x, ch1[j].y, ch1[j].z, ch1[j].w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) r[j][e] = rk[live && c[e] != kEmptyChild ? c[e] : 0u];
+            }
+            auto tr = [](uint32_t v, uint32_t rank) { return v == kEmptyChild ? v : rank; };
+#pragma unroll
+            for (int j = 0; j < kExtractIpt; ++j) {
+                ch0[j] = make_uint4(tr(ch0[j].x, r[j][0]), tr(ch0[j].y, r[j][1]), tr(ch0[j].z, r[j][2]),
+                                    tr(ch0[j].w, r[j][3]));
+                ch1[j] = make_uint4(tr(ch1[j].x, r[j][4]), tr(ch1[j].y, r[j][5]), tr(ch1[j].z, r[j][6]),
+                                    tr(ch1[j].w, r[j][7]));
             }
         }
 #pragma unroll
