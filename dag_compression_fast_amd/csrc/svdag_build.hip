@@ -26,6 +26,7 @@
 #include "svdag_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace svdag {
 
@@ -114,15 +115,24 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l)
 // G groups of 64 raw keys from `base`; indices past the end read
 // the last key (n >= 1), and the caller ignores those lanes.  No use of the
 // values here, so the loads stay in flight until the keys are needed.
-template <typename K, int G>
+template <typename K, int G, typename KC>
 __device__ __forceinline__ void load_groups(const K* __restrict__ sorted, uint64_t base, uint64_t n,
-                                            uint64_t (&k)[G])
+                                            KC (&k)[G])
 {
     const uint32_t lane = lane_id();
 #pragma unroll
     for (int g = 0; g < G; ++g)  // read once: non-temporal
-        k[g] = uint64_t(__builtin_nontemporal_load(sorted + min(base + g * kWave + lane, n - 1)));
+        k[g] = KC(__builtin_nontemporal_load(sorted + min(base + g * kWave + lane, n - 1)));
 }
+
+// Lane l's value of v (wave-uniform result), for 32- and 64-bit values.
+__device__ __forceinline__ uint32_t readlane_of(uint32_t v, uint32_t l)
+{
+    return uint32_t(__builtin_amdgcn_readlane(int(v), int(l)));
+}
+__device__ __forceinline__ uint64_t readlane_of(uint64_t v, uint32_t l) { return readlane64(v, l); }
+__device__ __forceinline__ uint32_t shfl_up_of(uint32_t v) { return uint32_t(__shfl_up(int(v), 1, kWave)); }
+__device__ __forceinline__ uint64_t shfl_up_of(uint64_t v) { return __shfl_up(v, 1, kWave); }
 
 // Bucket of position p among Morton-ordered buckets with starts start(0..num)
 // (start(num) > p): the last b >= lo with start(b) <= p (empty buckets share
@@ -162,6 +172,10 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
     __shared__ uint64_t s_parents[kReduceWaves];
     constexpr int G = kReduceGroups<K>;
     constexpr bool BUCKETED = BM != kBucketsNone;
+    // Keys and group indices in registers: 32-bit for 32-bit keys (a group
+    // index then has at most 32 - 15 local bits and the bucket's 12 above
+    // them), 64-bit otherwise.
+    using KC = std::conditional_t<sizeof(K) == 4, uint32_t, uint64_t>;
     // Bucket starts (positions < 2^30) staged in LDS when the table is small.
     __shared__ uint32_t s_bstart[BM == kBucketsLds ? kLdsBuckets + 1 : 1];
     if constexpr (BM == kBucketsLds) {
@@ -200,33 +214,39 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             cur = bucket_of(start_of, 0, bk.num_chunks, c0 > 0 ? c0 - 1 : 0);
             cur_end = start_of(cur + 1);
         }
-        // Key at position p of a batch starting at bb, with its bucket.
-        auto add_bucket = [&](uint64_t bb, uint64_t (&v)[G]) {
+        // Group indices (code >> sh) of the batch of keys v at bb: the key's own
+        // bits above sh, with its bucket above those (bucketed keys).
+        const uint32_t gsh = BUCKETED ? bk.chunk_bits - sh : 0u;  // chunk_bits > sh
+        auto group_of = [&](uint64_t bb, const KC (&v)[G], KC (&out)[G]) {
             if constexpr (BUCKETED) {
                 const uint64_t last = min(bb + uint64_t(G) * kWave - 1, n - 1);
                 if (last < cur_end) {
+                    const KC pre = KC(cur) << gsh;
 #pragma unroll
-                    for (int g = 0; g < G; ++g) v[g] |= cur << bk.chunk_bits;
+                    for (int g = 0; g < G; ++g) out[g] = pre | (v[g] >> sh);
                 } else {
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const uint64_t p = min(bb + uint64_t(g) * kWave + lane, n - 1);
-                        v[g] |= bucket_of(start_of, cur, bk.num_chunks, p) << bk.chunk_bits;
+                        out[g] = (KC(bucket_of(start_of, cur, bk.num_chunks, p)) << gsh) | (v[g] >> sh);
                     }
                     cur = bucket_of(start_of, cur, bk.num_chunks, last);
                     cur_end = start_of(cur + 1);
                 }
+            } else {
+#pragma unroll
+                for (int g = 0; g < G; ++g) out[g] = v[g] >> sh;
             }
         };
         // Group of the key before the chunk (~0 never equals a group index).
-        uint64_t prev_b = ~0ull;
+        KC prev_b = KC(~0ull);
         if (c0 > 0) {
-            uint64_t pk = uint64_t(sorted[c0 - 1]);
-            if constexpr (BUCKETED) pk |= cur << bk.chunk_bits;  // cur: the bucket of c0 - 1
-            prev_b = (pk & code_mask) >> sh;
+            const KC pk = KC(sorted[c0 - 1]) & KC(code_mask);
+            if constexpr (BUCKETED) prev_b = (KC(cur) << gsh) | (pk >> sh);  // cur: the bucket of c0 - 1
+            else prev_b = pk >> sh;
         }
         bool open = false;  // a group owned by this wave is being accumulated
-        uint64_t cur_b = 0;
+        KC cur_b = 0;
         bool done = false;
 
         // Writes the open group's leaves in order and clears the table.  Row e
@@ -260,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
                 for (int e = 0; e < 8; ++e) {
                     if (m[e]) {
                         const uint32_t p = pos + uint32_t(__popcll(B[e] & lanemask_lt()));
-                        oc[p] = (cur_b << r) | uint64_t(e * kWave + lane);
+                        oc[p] = (uint64_t(cur_b) << r) | uint64_t(e * kWave + lane);
                         om[p] = m[e];
                     }
                     pos += uint32_t(__popcll(B[e]));
@@ -270,13 +290,13 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             open = false;
         };
 
-        uint64_t k[G], nxt[G];
+        KC k[G], nxt[G], gb[G];
         load_groups(sorted, c0, n, k);
         // No load may be pending at the loop head (see the wait at its end).
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        add_bucket(c0, k);
 #pragma unroll
-        for (int g = 0; g < G; ++g) k[g] &= code_mask;
+        for (int g = 0; g < G; ++g) k[g] &= KC(code_mask);
+        group_of(c0, k, gb);
         for (uint64_t base = c0; !done; base += uint64_t(G) * kWave) {
             // The next batch of the chunk is in flight while this one is reduced.
             const uint64_t next = base + uint64_t(G) * kWave;
@@ -290,12 +310,12 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             uint64_t H[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const uint64_t b = k[g] >> sh;
-                const uint64_t b0 = readlane64(b, 0), b63 = readlane64(b, kWave - 1);
+                const KC b = gb[g];
+                const KC b0 = readlane_of(b, 0), b63 = readlane_of(b, kWave - 1);
                 if (b0 == b63) {
                     H[g] = b0 != prev_b ? 1ull : 0ull;
                 } else {
-                    uint64_t pb = __shfl_up(b, 1, kWave);
+                    KC pb = shfl_up_of(b);
                     if (lane == 0) pb = prev_b;
                     H[g] = __ballot(b != pb);
                 }
@@ -305,9 +325,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             for (int g = 0; g < G; ++g) {
                 if (done) break;
                 const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
-                const uint64_t key = k[g];
+                const KC key = k[g];
                 uint64_t* const word = &tab[uint32_t(key >> 6) & low_mask];
-                const uint64_t bit = 1ull << (key & 63);
+                const uint64_t bit = 1ull << (uint32_t(key) & 63u);
                 uint64_t heads = H[g];
                 const bool keys_end = gi + kWave >= n;  // the last key is in this batch group
                 if (heads == 0ull) {
@@ -340,7 +360,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
                         break;
                     }
                     open = true;
-                    cur_b = readlane64(key >> sh, hi);
+                    cur_b = readlane_of(gb[g], hi);
                     heads &= heads - 1ull;
                     lo = hi;
                 }
@@ -351,9 +371,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             // next batch's keys are then complete at the loop head, so the
             // batch's first uses do not wait behind the prefetch issued there.
             __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-            add_bucket(next, nxt);
 #pragma unroll
-            for (int g = 0; g < G; ++g) k[g] = nxt[g] & code_mask;
+            for (int g = 0; g < G; ++g) k[g] = nxt[g] & KC(code_mask);
+            group_of(next, k, gb);
         }
         if (lane == 0) counts[chunk] = written;
     }
