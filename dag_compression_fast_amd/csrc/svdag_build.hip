@@ -738,10 +738,27 @@ __global__ __launch_bounds__(kThreads) void k_level_count(const uint64_t* __rest
     const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
     uint64_t gtot = 0;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t code[kBuildIpt];
-        uint32_t starts, gcount;
-        parent_runs(codes, n, tile * kBuildTile + uint64_t(threadIdx.x) * kBuildIpt, code, starts, gcount);
-        const uint64_t cnt = block_reduce_sum<kThreads>(__popc(starts), s_red);
+        // The same run starts as parent_runs, counted with coalesced reads:
+        // lane i of a round reads code i and its predecessor (all loads
+        // issued first, indices clamped into [0, n)).
+        const uint64_t base = tile * kBuildTile + threadIdx.x;
+        uint64_t cv[kBuildIpt], pv[kBuildIpt];
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) {
+            const uint64_t i = base + uint64_t(j) * kThreads;
+            cv[j] = codes[min(i, n - 1)];
+            pv[j] = codes[i > 0 ? min(i - 1, n - 1) : 0];
+        }
+        uint32_t starts = 0, gcount = 0;
+#pragma unroll
+        for (int j = 0; j < kBuildIpt; ++j) {
+            const uint64_t i = base + uint64_t(j) * kThreads;
+            if (i < n && (i == 0 || (cv[j] >> 3) != (pv[j] >> 3))) {
+                ++starts;
+                gcount += (i == 0 || (cv[j] >> 6) != (pv[j] >> 6)) ? 1u : 0u;
+            }
+        }
+        const uint64_t cnt = block_reduce_sum<kThreads>(starts, s_red);
         gtot += gcount;
         if (threadIdx.x == 0) counts[tile] = uint32_t(cnt);
     }
@@ -1326,14 +1343,19 @@ __global__ __launch_bounds__(kThreads) void k_final_write(FinalArgs a)
         const uint4* src = reinterpret_cast<const uint4*>(s_lv.children[l] + 8 * i);
         const uint4 c0 = src[0], c1 = src[1];
         const uint32_t ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        // Every child's target first (absent children read node 0's position
+        // and are dropped), so that all gathers are in flight at once.
+        uint32_t target[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t c = (mask >> s) & 1u ? ch[s] : 0u;
+            target[s] = last ? uint32_t(a.leaves_start + 2ull * c) : a.positions[s_lv.first[l + 1] + c];
+        }
         uint32_t off = 1;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             if ((mask >> s) & 1u) {
-                const uint32_t c = ch[s];
-                const uint32_t target = last ? uint32_t(a.leaves_start + 2ull * c)
-                                             : a.positions[s_lv.first[l + 1] + c];
-                a.words[pos + off] = target;
+                a.words[pos + off] = target[s];
                 ++off;
             }
         }
