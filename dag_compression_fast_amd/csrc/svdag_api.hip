@@ -860,7 +860,6 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
                                      lp_code, lp_mask, L.leaf_bound, tile_counts, tile_offsets, n, depth - 3,
                                      code_mask);
 
-    if (!capture) mark(ctx, "leaf reduce");
     // SortLevel(Leaves) (DAGCompression.cu:1070-1071): dedup table, then order.
     uint64_t* leaf_table = reinterpret_cast<uint64_t*>(A + L.leaf_table);
     uint32_t* leaf_rank = reinterpret_cast<uint32_t*>(A + L.leaf_rank);
@@ -962,7 +961,6 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         launch_table_clear(st, a.table_keys, &c->pre[l], &c->table_log2[l], ctx->load_shift, kTableOverflow,
                            L.level_slots_of[l]);
         launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_nodes[l + 1]);
-        if (!capture) mark(ctx, "level build");
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
                                &c->collisions, L.level_nodes[l]);
@@ -972,6 +970,10 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         // early enough on the side stream to end beside the builds.
         if (l == std::max(depth - 4, tail_top + 1)) TRY(order_leaves());
     }
+    // Stage markers (the completion deadline names the first stage not done)
+    // go between stage groups only: each event record leaves the GPU idle for
+    // a few microseconds.
+    if (!capture) mark(ctx, "level builds");
     TRY(join());
     const uint32_t* below = leaf_rank;  // slot -> rank map of the level below
     for (int l = depth - 3; l > tail_top; --l) {
@@ -993,9 +995,9 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         e.q_count = &c->uniq[l];
         e.w_count = &c->words[l];
         launch_level_extract(st, e, L.level_slots_of[l]);
-        if (!capture) mark(ctx, "level extract");
         below = e.rank_of_slot;
     }
+    if (!capture) mark(ctx, "level extracts");
     {
         TailArgs t{};
         t.top = tail_top;
