@@ -68,6 +68,21 @@ __device__ __forceinline__ TileWalk xcd_tiles(uint64_t ntiles)
     return TileWalk{start + local, end, groups};
 }
 
+// Partition of the tiles among the workgroups of a launch, each taking one
+// contiguous run: workgroup b owns partition
+// (b % kXcds) * (gridDim.x / kXcds) + b / kXcds when the grid is a multiple of
+// kXcds (neighbouring partitions then sit on one XCD), else partition b.
+// Partition q holds tiles [q * ntiles / P, (q + 1) * ntiles / P), P = gridDim.x.
+struct PartRange {
+    uint64_t begin, end;
+};
+__device__ __forceinline__ PartRange part_tiles(uint64_t ntiles)
+{
+    const uint64_t P = gridDim.x;
+    const uint64_t q = P % kXcds ? blockIdx.x : uint64_t(blockIdx.x % kXcds) * (P / kXcds) + blockIdx.x / kXcds;
+    return PartRange{q * ntiles / P, (q + 1) * ntiles / P};
+}
+
 // Chunk of input position i: the c with offsets[c] <= i < offsets[c + 1],
 // searched in [lo, hi).
 __device__ __forceinline__ uint64_t chunk_search(const ChunkPrefix& p, uint64_t i, uint64_t lo, uint64_t hi)
@@ -103,9 +118,15 @@ struct HistPlan {
     uint32_t mask[kMaxPasses];
 };
 
+// Tiles whose counts the histogram writes together.
+constexpr uint32_t kHistGroup = 8;
+
 // One read of the keys gives the first pass's per-tile digit counts
-// (counts[d * stride + tile], walked XCD by XCD like the count kernel) and
-// digit totals, and the OR of the keys (range check).
+// (counts[d * stride + tile]) and digit totals, and the OR of the keys (range
+// check).  Each workgroup counts one contiguous run of tiles (part_tiles),
+// kHistGroup tiles at a time, and writes the group's counts digit by digit as
+// runs of consecutive words: one 4-byte store per tile and digit, scattered
+// over the digit rows, took a third of the kernel's time.
 template <typename K, int IPT>
 __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict__ keys,
                                                             const uint64_t* __restrict__ n_ptr, HistPlan plan,
@@ -115,73 +136,72 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
                                                             ChunkPrefix prefix)
 {
     constexpr uint64_t kTile = uint64_t(kSortThreads) * IPT;
-    __shared__ uint32_t s_hist[kRadix];
-    __shared__ uint32_t s_cnt[kRadix];
-    for (int i = threadIdx.x; i < kRadix; i += kSortThreads) s_hist[i] = 0;
+    constexpr int DPT = kRadix / kSortThreads;  // digits whose totals a thread keeps
+    // One row of counters per tile of the group, padded by a word so that the
+    // digit-major read-out (lanes on consecutive tiles of a digit) spreads
+    // over the LDS banks.
+    constexpr uint32_t kRow = kRadix + 1;
+    __shared__ uint32_t s_cnt[kHistGroup * kRow];
     const uint64_t n = *n_ptr;
-    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    const PartRange part = part_tiles((n + kTile - 1) / kTile);
     const uint32_t radix0 = plan.mask[0] + 1;
-    const TileWalk walk = xcd_tiles(ntiles);
+    uint32_t tot[DPT] = {};
     K acc = 0;
-    // Keys of tile t; indices past the end read the last key (ignored below),
-    // so the loads need no guard.  Each key is read once: non-temporal loads.
-    auto load = [&](uint64_t t, K (&v)[IPT]) {
-        const uint64_t b = t * kTile;
-#pragma unroll
-        for (int i = 0; i < IPT; ++i)
-            v[i] = __builtin_nontemporal_load(keys + min(b + uint64_t(i) * kSortThreads + threadIdx.x, n - 1));
-    };
-    K k[IPT], nxt[IPT];
-    if (walk.tile < walk.end) load(walk.tile, k);
-    // No load may be pending at the loop head (see the wait at its end).
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    for (uint64_t tile = walk.tile; tile < walk.end; tile += walk.step) {
-        for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) s_cnt[d] = 0;
+    for (uint64_t g0 = part.begin; g0 < part.end; g0 += kHistGroup) {
+        const uint32_t gn = uint32_t(min(uint64_t(kHistGroup), part.end - g0));
+        for (uint32_t i = threadIdx.x; i < gn * kRow; i += kSortThreads) s_cnt[i] = 0;
         __syncthreads();
-        const uint64_t base = tile * kTile;
-        const uint32_t cnt = uint32_t(min(kTile, n - base));
-        // The tile's chunks are looked up before the prefetch below, whose
-        // loads would otherwise be waited for with the lookup's.
-        const TileChunks tc = prefix.offsets ? tile_chunks(prefix, base, cnt) : TileChunks{0, 0};
-        // The next tile's keys are in flight while this one is counted.
-        const bool more = tile + walk.step < walk.end;
-        if (more) load(tile + walk.step, nxt);
-        if (prefix.offsets) {
-            // Chunk-local keys: range check, then the chunk index above them.
-            bool bad = false;
+        for (uint32_t g = 0; g < gn; ++g) {
+            const uint64_t base = (g0 + g) * kTile;
+            const uint32_t cnt = uint32_t(min(kTile, n - base));
+            // The tile's chunks are looked up before its keys are loaded,
+            // whose loads would otherwise be waited for with the lookup's.
+            const TileChunks tc = prefix.offsets ? tile_chunks(prefix, base, cnt) : TileChunks{0, 0};
+            // Indices past the end read the last key (ignored below), so the
+            // loads need no guard.  Each key is read once: non-temporal loads.
+            K k[IPT];
+#pragma unroll
+            for (int i = 0; i < IPT; ++i)
+                k[i] = __builtin_nontemporal_load(keys + min(base + uint64_t(i) * kSortThreads + threadIdx.x, n - 1));
+            if (prefix.offsets) {
+                // Chunk-local keys: range check, then the chunk index above them.
+                bool bad = false;
+#pragma unroll
+                for (int i = 0; i < IPT; ++i) {
+                    const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
+                    if (idx < cnt) {
+                        bad |= (uint64_t(k[i]) >> prefix.chunk_bits) != 0;
+                        k[i] = K(prefixed(prefix, tc, base + idx, uint64_t(k[i])));
+                    }
+                }
+                if (bad) atomicOr(prefix.err, (unsigned long long)kErrKeyRange);
+            }
+            uint32_t* const row = s_cnt + g * kRow;
 #pragma unroll
             for (int i = 0; i < IPT; ++i) {
-                const uint32_t idx = uint32_t(i) * kSortThreads + threadIdx.x;
-                if (idx < cnt) {
-                    bad |= (uint64_t(k[i]) >> prefix.chunk_bits) != 0;
-                    k[i] = K(prefixed(prefix, tc, base + idx, uint64_t(k[i])));
+                if (uint32_t(i) * kSortThreads + threadIdx.x < cnt) {
+                    acc |= k[i];
+                    atomicAdd(&row[uint32_t(k[i] >> plan.shift[0]) & plan.mask[0]], 1u);
                 }
             }
-            if (bad) atomicOr(prefix.err, (unsigned long long)kErrKeyRange);
-        }
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            if (uint32_t(i) * kSortThreads + threadIdx.x < cnt) {
-                acc |= k[i];
-                atomicAdd(&s_cnt[uint32_t(k[i] >> plan.shift[0]) & plan.mask[0]], 1u);
-            }
         }
         __syncthreads();
-        for (uint32_t d = threadIdx.x; d < radix0; d += kSortThreads) {
-            const uint32_t c = s_cnt[d];
-            counts[uint64_t(d) * stride + tile] = c;
-            s_hist[d] += c;
+        for (uint32_t i = threadIdx.x; i < gn * radix0; i += kSortThreads) {
+            const uint32_t d = i / gn, g = i - d * gn;
+            counts[uint64_t(d) * stride + g0 + g] = s_cnt[g * kRow + d];
+        }
+#pragma unroll
+        for (int j = 0; j < DPT; ++j) {
+            const uint32_t d = threadIdx.x * DPT + j;
+            if (d < radix0)
+                for (uint32_t g = 0; g < gn; ++g) tot[j] += s_cnt[g * kRow + d];
         }
         __syncthreads();
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        if (more) {
-#pragma unroll
-            for (int i = 0; i < IPT; ++i) k[i] = nxt[i];
-        }
     }
-    for (uint32_t d = threadIdx.x; d <= plan.mask[0]; d += kSortThreads) {
-        const uint32_t v = s_hist[d];
-        if (v) atomicAdd(&hist[d], v);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+        const uint32_t d = threadIdx.x * DPT + j;
+        if (d < radix0 && tot[j]) atomicAdd(&hist[d], tot[j]);
     }
     if (key_or) {
         uint64_t a = uint64_t(acc);
@@ -189,6 +209,22 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
         for (int d = 32; d >= 1; d >>= 1) a |= __shfl_xor(a, d, kWave);
         if (lane_id() == 0 && a) atomicOr(key_or, (unsigned long long)a);
     }
+}
+
+// Workgroups of k_histogram<K, IPT> over stride tiles: one per workgroup slot
+// of the device (every partition runs at once), a multiple of kXcds, at most
+// one per tile.
+template <typename K, int IPT>
+uint32_t histogram_grid(uint64_t stride)
+{
+    static const int occ = [] {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_histogram<K, IPT>, kSortThreads, 0) != hipSuccess)
+            return 1;
+        return std::max(blocks, 1);
+    }();
+    const uint64_t slots = uint64_t(device_cu_count()) * uint64_t(occ);
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(slots >= kXcds ? slots / kXcds * kXcds : slots, stride)));
 }
 
 // Per-tile digit counts, digit-major: counts[d * stride + tile], and the
@@ -657,8 +693,8 @@ int launch_sort(hipStream_t stream, const SortPlan& plan, const K* keys_in, cons
         {
             // Also writes the first pass's tile counts.
             TimedLaunch t(vals_in ? kKPairHistogram : kKHistogram, stream);
-            hipLaunchKernelGGL((k_histogram<K, I>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in, n_ptr, hp,
-                               hist, key_or, counts, stride, prefix);
+            hipLaunchKernelGGL((k_histogram<K, I>), dim3(histogram_grid<K, I>(stride)), dim3(kSortThreads), 0, stream,
+                               keys_in, n_ptr, hp, hist, key_or, counts, stride, prefix);
         }
         const K* src = keys_in;
         const uint32_t* vsrc = vals_in;
@@ -1184,14 +1220,13 @@ int launch_top_digit_sort(hipStream_t stream, const uint64_t* keys_in, uint32_t*
         constexpr int I = decltype(ipt_tag)::value;
         const uint64_t stride = sort_tiles(n_bound, I);
         const int cus = device_cu_count();
-        const uint32_t grid = uint32_t(std::max<uint64_t>(
-            kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 8) / kXcds * kXcds));
         const uint32_t scatter_grid = uint32_t(std::max<uint64_t>(
             kXcds, std::min<uint64_t>((stride + kXcds - 1) / kXcds * kXcds, uint64_t(cus) * 64) / kXcds * kXcds));
         {
             TimedLaunch t(kKHistogram, stream);
-            hipLaunchKernelGGL((k_histogram<uint64_t, I>), dim3(grid), dim3(kSortThreads), 0, stream, keys_in, n_ptr,
-                               hp, zero, key_or, counts, stride, ChunkPrefix{});
+            hipLaunchKernelGGL((k_histogram<uint64_t, I>), dim3(histogram_grid<uint64_t, I>(stride)),
+                               dim3(kSortThreads), 0, stream, keys_in, n_ptr, hp, zero, key_or, counts, stride,
+                               ChunkPrefix{});
         }
         {
             TimedLaunch t(kKSortScan, stream);

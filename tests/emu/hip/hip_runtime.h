@@ -210,4 +210,13 @@ hipError_t hipGraphDestroy(hipGraph_t g);
 }
 enum { hipErrorNotReady = 600, hipEventDisableTiming = 2, hipStreamNonBlocking = 1 };
 
+// Occupancy: eight resident workgroups per CU for every kernel (with two CUs,
+// a partitioned launch over enough tiles gets sixteen workgroups, two per XCD).
+template <typename F>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* blocks, F, int, size_t)
+{
+    *blocks = 8;
+    return hipSuccess;
+}
+
 #define hipLaunchKernelGGL(F, G, B, SH, ST, ...) emu::launch(G, B, [=]() { F(__VA_ARGS__); })

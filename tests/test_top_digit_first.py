@@ -112,6 +112,19 @@ def test_top_digit_first_on_the_cpu_emulator(emu, oracle):
     run_errors(emu)
 
 
+def test_top_digit_partitions_on_the_cpu_emulator(emu, oracle):
+    """The top-digit pass splits the tiles into one contiguous run per
+    workgroup (the emulator reports 8 workgroups per CU on 2 CUs: 16
+    partitions, placed two per XCD).  40k keys leave some partitions empty
+    (10 tiles of 4096), 150k keys give every partition two or three tiles."""
+    rng = np.random.default_rng(0x9A27)
+    for depth, n in ((11, 40_000), (11, 150_000), (10, 150_000)):
+        keys = rng.integers(0, 1 << (3 * depth), size=n, dtype=np.uint64)
+        keys = np.concatenate([keys, keys[: n // 4]])
+        print(f"[top-digit] partitions d{depth}: {len(keys)} keys", flush=True)
+        built(emu, oracle, keys, depth, ALWAYS)
+
+
 def test_top_digit_first_blocks_in_reverse_order(emu, oracle):
     import emu_ctypes
     L = ctypes.CDLL(emu_ctypes.EMU_LIB)
