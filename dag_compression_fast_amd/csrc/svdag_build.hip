@@ -1634,7 +1634,7 @@ void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32
                         uint64_t code_mask, const ChunkPrefix& buckets)
 {
     const uint64_t chunks = (n_bound + kReduceChunk - 1) / kReduceChunk;
-    const int grid = grid_for((chunks + kReduceWaves - 1) / kReduceWaves, 8);
+    const uint64_t units = (chunks + kReduceWaves - 1) / kReduceWaves;
     {
         TimedLaunch t(kKLeafReduce, s);
         bool bucketed = false;
@@ -1642,24 +1642,29 @@ void launch_leaf_reduce(hipStream_t s, const K* sorted, BuildCounters* c, uint32
             if (buckets.offsets) {
                 bucketed = true;
                 if (buckets.num_chunks <= kLdsBuckets)
-                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsLds>), dim3(grid), dim3(kThreads), 0, s, sorted,
-                                       &c->n, code_mask, group_bits, reg_code, reg_mask, tile_counts,
-                                       &c->pre[leaf_level], reinterpret_cast<unsigned long long*>(&c->error), buckets);
+                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsLds>),
+                                       dim3(resident_grid(k_leaf_reduce<K, kBucketsLds>, kThreads, units)),
+                                       dim3(kThreads), 0, s, sorted, &c->n, code_mask, group_bits, reg_code, reg_mask,
+                                       tile_counts, &c->pre[leaf_level],
+                                       reinterpret_cast<unsigned long long*>(&c->error), buckets);
                 else
-                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsGlobal>), dim3(grid), dim3(kThreads), 0, s, sorted,
-                                       &c->n, code_mask, group_bits, reg_code, reg_mask, tile_counts,
-                                       &c->pre[leaf_level], reinterpret_cast<unsigned long long*>(&c->error), buckets);
+                    hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsGlobal>),
+                                       dim3(resident_grid(k_leaf_reduce<K, kBucketsGlobal>, kThreads, units)),
+                                       dim3(kThreads), 0, s, sorted, &c->n, code_mask, group_bits, reg_code, reg_mask,
+                                       tile_counts, &c->pre[leaf_level],
+                                       reinterpret_cast<unsigned long long*>(&c->error), buckets);
             }
         }
         if (!bucketed)
-            hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsNone>), dim3(grid), dim3(kThreads), 0, s, sorted, &c->n,
-                               code_mask, group_bits, reg_code, reg_mask, tile_counts, &c->pre[leaf_level],
-                               reinterpret_cast<unsigned long long*>(&c->error), ChunkPrefix{});
+            hipLaunchKernelGGL((k_leaf_reduce<K, kBucketsNone>),
+                               dim3(resident_grid(k_leaf_reduce<K, kBucketsNone>, kThreads, units)), dim3(kThreads), 0,
+                               s, sorted, &c->n, code_mask, group_bits, reg_code, reg_mask, tile_counts,
+                               &c->pre[leaf_level], reinterpret_cast<unsigned long long*>(&c->error), ChunkPrefix{});
     }
     launch_scan_counts(s, tile_counts, tile_offsets, &c->n, kReduceChunk, &c->leaf_positions);
     {
         TimedLaunch t(kKLeafCompact, s);
-        hipLaunchKernelGGL(k_leaf_compact, dim3(grid), dim3(kThreads), 0, s, reg_code, reg_mask, tile_counts,
+        hipLaunchKernelGGL(k_leaf_compact, dim3(grid_for(units, 8)), dim3(kThreads), 0, s, reg_code, reg_mask, tile_counts,
                            tile_offsets, &c->n, lp_code, lp_mask, lp_capacity,
                            reinterpret_cast<unsigned long long*>(&c->error));
     }
@@ -1729,16 +1734,16 @@ void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint
 
 void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound)
 {
-    const int grid = grid_for((child_bound + kBuildTile - 1) / kBuildTile, 8);
+    const uint64_t tiles = (child_bound + kBuildTile - 1) / kBuildTile;
     {
         TimedLaunch t(kKLevelCount, s);
-        hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(kThreads), 0, s, a.child_codes, a.child_count, a.tile_counts,
-                           a.next_count);
+        hipLaunchKernelGGL(k_level_count, dim3(resident_grid(k_level_count, kThreads, tiles)), dim3(kThreads), 0, s,
+                           a.child_codes, a.child_count, a.tile_counts, a.next_count);
     }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.child_count, kBuildTile, nullptr);
     {
         TimedLaunch t(kKLevelBuild, s);
-        hipLaunchKernelGGL(k_level_build, dim3(grid), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL(k_level_build, dim3(resident_grid(k_level_build, kThreads, tiles)), dim3(kThreads), 0, s, a);
     }
 }
 
@@ -1750,16 +1755,17 @@ void launch_level_tail(hipStream_t s, const TailArgs& a)
 
 void launch_level_extract(hipStream_t s, const LevelExtractArgs& a, uint64_t slot_bound)
 {
-    const int grid = grid_for((slot_bound + kExtractTile - 1) / kExtractTile, 8);
+    const uint64_t tiles = (slot_bound + kExtractTile - 1) / kExtractTile;
     {
         TimedLaunch t(kKExtractCount, s);
-        hipLaunchKernelGGL(k_extract_count, dim3(grid), dim3(kThreads), 0, s, a.table_keys, a.table_log2, a.tile_counts,
-                           a.slots_scratch);
+        hipLaunchKernelGGL(k_extract_count, dim3(resident_grid(k_extract_count, kThreads, tiles)), dim3(kThreads), 0,
+                           s, a.table_keys, a.table_log2, a.tile_counts, a.slots_scratch);
     }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.slots_scratch, kExtractTile, a.q_count);
     {
         TimedLaunch t(kKLevelExtract, s);
-        hipLaunchKernelGGL(k_level_extract, dim3(grid), dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL(k_level_extract, dim3(resident_grid(k_level_extract, kThreads, tiles)), dim3(kThreads), 0,
+                           s, a);
     }
 }
 
@@ -1832,15 +1838,17 @@ void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes
 {
     if (a.total_nodes == 0 && a.num_leaves == 0) return;
     if (a.total_nodes) {
-        const int grid = grid_for((a.total_nodes + kFinalTile - 1) / kFinalTile, 8);
+        const uint64_t tiles = (a.total_nodes + kFinalTile - 1) / kFinalTile;
         {
             TimedLaunch t(kKFinalCount, s);
-            hipLaunchKernelGGL(k_final_count, dim3(grid), dim3(kThreads), 0, s, a);
+            hipLaunchKernelGGL(k_final_count, dim3(resident_grid(k_final_count, kThreads, tiles)), dim3(kThreads), 0, s,
+                               a);
         }
         launch_scan_counts(s, a.counts, a.offsets, total_nodes_dev, kFinalTile, nullptr);
         {
             TimedLaunch t(kKFinalPositions, s);
-            hipLaunchKernelGGL(k_final_positions, dim3(grid), dim3(kThreads), 0, s, a);
+            hipLaunchKernelGGL(k_final_positions, dim3(resident_grid(k_final_positions, kThreads, tiles)),
+                               dim3(kThreads), 0, s, a);
         }
     }
     const uint64_t work = std::max(a.total_nodes, a.num_leaves);

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "svdag.h"
 
 namespace svdag {
@@ -231,6 +233,23 @@ constexpr uint64_t kListTileItems = 4096;     // table slots per slot-list tile
 inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
 
 int device_cu_count();
+
+// Workgroups of `threads` threads of `kernel` that the device holds at once
+// (its occupancy times the CU count; cached per kernel).  A kernel whose
+// workgroups each walk several tiles launches at most this many: a larger grid
+// leaves a partial second round of workgroups, each with a full share of tiles.
+int kernel_slots(const void* kernel, int threads);
+template <typename F>
+inline int kernel_slots(F kernel, int threads)
+{
+    return kernel_slots(reinterpret_cast<const void*>(kernel), threads);
+}
+// Grid for `tiles` units of work: one workgroup per unit, at most kernel_slots.
+template <typename F>
+inline int resident_grid(F kernel, int threads, uint64_t tiles)
+{
+    return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, uint64_t(kernel_slots(kernel, threads)))));
+}
 
 void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value);
 

@@ -20,7 +20,10 @@
 #include "svdag_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace svdag {
 
@@ -217,13 +220,7 @@ __global__ __launch_bounds__(kSortThreads) void k_histogram(const K* __restrict_
 template <typename K, int IPT>
 uint32_t histogram_grid(uint64_t stride)
 {
-    static const int occ = [] {
-        int blocks = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_histogram<K, IPT>, kSortThreads, 0) != hipSuccess)
-            return 1;
-        return std::max(blocks, 1);
-    }();
-    const uint64_t slots = uint64_t(device_cu_count()) * uint64_t(occ);
+    const uint64_t slots = uint64_t(kernel_slots(k_histogram<K, IPT>, kSortThreads));
     return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(slots >= kXcds ? slots / kXcds * kXcds : slots, stride)));
 }
 
@@ -581,6 +578,20 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(const K* __restrict__ 
 }
 
 }  // namespace
+
+int kernel_slots(const void* kernel, int threads)
+{
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, int>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& e : cache)
+        if (e.first == kernel) return e.second;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0) != hipSuccess || blocks < 1) blocks = 1;
+    const int slots = blocks * device_cu_count();
+    cache.emplace_back(kernel, slots);
+    return slots;
+}
 
 int device_cu_count()
 {
