@@ -198,7 +198,8 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
     const uint64_t nchunks = (n + kReduceChunk - 1) / kReduceChunk;
     const uint32_t sh = 6 + r;
     const uint32_t low_mask = (1u << r) - 1u;
-    uint64_t parents = 0;
+    uint64_t parents = 0;   // parents counted by the exact-room flush
+    uint32_t parent_lanes = 0;  // per lane: parents counted by the fast flush
     bool overflow = false;
 
     for (uint64_t chunk = uint64_t(blockIdx.x) * kReduceWaves + w; chunk < nchunks;
@@ -255,6 +256,33 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
         // the 8 leaves of one parent are 8 consecutive lanes of one row.
         auto flush = [&]() {
             __builtin_amdgcn_wave_barrier();
+            if (written + kReduceTable <= kReduceRegion) {
+                // Room for a full table (always, for valid bounds: the groups
+                // before the last hold at most one leaf per chunk key).  Row
+                // by row, with the per-row bookkeeping in vector registers:
+                // this flush runs once per group and its scalar work had been
+                // about half of the kernel's scalar instructions.  Lane l of
+                // row e counts parent 8e + l/8 when l % 8 == 0 and one of its
+                // 8 leaves is present.
+                uint32_t pos = written;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint64_t m = tab[e * kWave + lane];
+                    tab[e * kWave + lane] = 0ull;
+                    const uint64_t B = __ballot(m != 0ull);
+                    parent_lanes += ((lane & 7u) == 0u && ((B >> (lane & 56u)) & 0xFFull) != 0ull) ? 1u : 0u;
+                    if (m) {
+                        const uint32_t p = pos + uint32_t(__popcll(B & lanemask_lt()));
+                        oc[p] = (uint64_t(cur_b) << r) | uint64_t(e * kWave + lane);
+                        om[p] = m;
+                    }
+                    pos += uint32_t(__popcll(B));
+                }
+                written = pos;
+                __builtin_amdgcn_wave_barrier();
+                open = false;
+                return;
+            }
             uint64_t m[8], B[8];
             uint32_t total = 0;
 #pragma unroll
@@ -376,6 +404,12 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             group_of(next, k, gb);
         }
         if (lane == 0) counts[chunk] = written;
+    }
+    {
+        uint32_t pl = parent_lanes;
+#pragma unroll
+        for (int d = kWave / 2; d >= 1; d >>= 1) pl += uint32_t(__shfl_xor(int(pl), d, kWave));
+        parents += pl;
     }
     if (lane == 0) s_parents[w] = parents;
     if (overflow && lane == 0) atomicOr(err, (unsigned long long)kErrInternal);
