@@ -198,8 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
     const uint64_t nchunks = (n + kReduceChunk - 1) / kReduceChunk;
     const uint32_t sh = 6 + r;
     const uint32_t low_mask = (1u << r) - 1u;
-    uint64_t parents = 0;   // parents counted by the exact-room flush
-    uint32_t parent_lanes = 0;  // per lane: parents counted by the fast flush
+    uint32_t parent_lanes = 0;  // per lane: parents counted by the flushes
     bool overflow = false;
 
     for (uint64_t chunk = uint64_t(blockIdx.x) * kReduceWaves + w; chunk < nchunks;
@@ -256,65 +255,32 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
         // the 8 leaves of one parent are 8 consecutive lanes of one row.
         auto flush = [&]() {
             __builtin_amdgcn_wave_barrier();
-            if (written + kReduceTable <= kReduceRegion) {
-                // Room for a full table (always, for valid bounds: the groups
-                // before the last hold at most one leaf per chunk key).  Row
-                // by row, with the per-row bookkeeping in vector registers:
-                // this flush runs once per group and its scalar work had been
-                // about half of the kernel's scalar instructions.  Lane l of
-                // row e counts parent 8e + l/8 when l % 8 == 0 and one of its
-                // 8 leaves is present.
-                uint32_t pos = written;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const uint64_t m = tab[e * kWave + lane];
-                    tab[e * kWave + lane] = 0ull;
-                    const uint64_t B = __ballot(m != 0ull);
-                    parent_lanes += ((lane & 7u) == 0u && ((B >> (lane & 56u)) & 0xFFull) != 0ull) ? 1u : 0u;
-                    if (m) {
-                        const uint32_t p = pos + uint32_t(__popcll(B & lanemask_lt()));
-                        oc[p] = (uint64_t(cur_b) << r) | uint64_t(e * kWave + lane);
-                        om[p] = m;
-                    }
-                    pos += uint32_t(__popcll(B));
-                }
-                written = pos;
-                __builtin_amdgcn_wave_barrier();
-                open = false;
-                return;
-            }
-            uint64_t m[8], B[8];
-            uint32_t total = 0;
+            // A full table fits the region for valid bounds: the groups before
+            // the last hold at most one leaf per chunk key, so written <= 4095.
+            // Past that, an internal bound failed: reported, nothing written.
+            const bool room = written + kReduceTable <= kReduceRegion;
+            if (!room) overflow = true;
+            // Row by row, with the per-row bookkeeping in vector registers:
+            // this runs once per group and its scalar work had been about half
+            // of the kernel's scalar instructions (the scalar unit is its
+            // limit).  Lane l of row e counts parent 8e + l/8 when l % 8 == 0
+            // and one of its 8 leaves is present.
+            uint32_t pos = written;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                m[e] = tab[e * kWave + lane];
-                B[e] = __ballot(m[e] != 0ull);
-                total += uint32_t(__popcll(B[e]));
-                uint64_t x = B[e];
-                x |= x >> 4;
-                x |= x >> 2;
-                x |= x >> 1;
-                parents += uint64_t(__popcll(x & 0x0101010101010101ull));
-            }
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                if (m[e]) tab[e * kWave + lane] = 0ull;
-            __builtin_amdgcn_wave_barrier();
-            if (written + total > kReduceRegion) {
-                overflow = true;
-            } else {
-                uint32_t pos = written;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    if (m[e]) {
-                        const uint32_t p = pos + uint32_t(__popcll(B[e] & lanemask_lt()));
-                        oc[p] = (uint64_t(cur_b) << r) | uint64_t(e * kWave + lane);
-                        om[p] = m[e];
-                    }
-                    pos += uint32_t(__popcll(B[e]));
+                const uint64_t m = tab[e * kWave + lane];
+                tab[e * kWave + lane] = 0ull;
+                const uint64_t B = __ballot(m != 0ull);
+                parent_lanes += ((lane & 7u) == 0u && ((B >> (lane & 56u)) & 0xFFull) != 0ull) ? 1u : 0u;
+                if (m && room) {
+                    const uint32_t p = pos + uint32_t(__popcll(B & lanemask_lt()));
+                    oc[p] = (uint64_t(cur_b) << r) | uint64_t(e * kWave + lane);
+                    om[p] = m;
                 }
-                written += total;
+                pos += uint32_t(__popcll(B));
             }
+            if (room) written = pos;
+            __builtin_amdgcn_wave_barrier();
             open = false;
         };
 
@@ -358,31 +324,21 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
                 const uint64_t bit = 1ull << (uint32_t(key) & 63u);
                 uint64_t heads = H[g];
                 const bool keys_end = gi + kWave >= n;  // the last key is in this batch group
-                if (heads == 0ull) {
-                    // Every lane continues the open group, or one owned by an earlier wave.
-                    if (open) atomicOr((unsigned long long*)word, bit);
-                    if (keys_end) {
-                        if (open) flush();
-                        done = true;
-                    } else if (!open && gi + kWave >= c1) {
-                        done = true;  // no group starts in the chunk
-                    }
-                    continue;
-                }
-                // Segments [lo, next start): lanes before the first start continue
-                // the open group, or one owned by an earlier wave when none is open.
+                // Segments [lo, next start), then [lo, 64): lanes before the first
+                // start continue the open group, or one owned by an earlier wave
+                // when none is open.  One flush site (the open group ends at a
+                // start, or at the keys' end): every call site is a copy of it.
                 uint32_t lo = 0;
                 while (true) {
                     const uint32_t hi = heads ? uint32_t(__ffsll((long long)heads) - 1) : uint32_t(kWave);
                     if (open && lane >= lo && lane < hi) atomicOr((unsigned long long*)word, bit);
-                    if (hi == uint32_t(kWave)) {
-                        if (keys_end) {
-                            if (open) flush();
-                            done = true;
-                        }
+                    const bool last = hi == uint32_t(kWave);
+                    if (open && (!last || keys_end)) flush();
+                    if (last) {
+                        // The keys end here, or no group starts in the chunk.
+                        if (keys_end || (!open && gi + kWave >= c1)) done = true;
                         break;
                     }
-                    if (open) flush();
                     if (gi + hi >= c1) {
                         done = true;
                         break;
@@ -405,12 +361,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
         }
         if (lane == 0) counts[chunk] = written;
     }
-    {
-        uint32_t pl = parent_lanes;
+    uint64_t parents = parent_lanes;
 #pragma unroll
-        for (int d = kWave / 2; d >= 1; d >>= 1) pl += uint32_t(__shfl_xor(int(pl), d, kWave));
-        parents += pl;
-    }
+    for (int d = kWave / 2; d >= 1; d >>= 1) parents += uint64_t(uint32_t(__shfl_xor(int(uint32_t(parents)), d, kWave)));
     if (lane == 0) s_parents[w] = parents;
     if (overflow && lane == 0) atomicOr(err, (unsigned long long)kErrInternal);
     __syncthreads();
