@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -761,14 +762,16 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         return code;
     };
 
-    launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + L.zero_begin), (L.zero_end - L.zero_begin) / 8, 0);
-    // The key count: given, or (device offsets) checked and read on the device.
+    // The zeroed region holds the counters.  The key count: given (set by the
+    // same fill), or (device offsets) checked and read on the device.
+    const uint64_t n_word = (L.counters + offsetof(BuildCounters, n) - L.zero_begin) / 8;
     const uint64_t* d_off = nullptr;
     if (batch && batch->d_offsets) {
+        launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + L.zero_begin), (L.zero_end - L.zero_begin) / 8, 0);
         d_off = batch->d_offsets;
         launch_batch_count(st, d_off, batch->num_chunks, n, &c->n, err);
     } else {
-        launch_fill_u64(st, &c->n, 1, n);
+        launch_fill_u64(st, reinterpret_cast<uint64_t*>(A + L.zero_begin), (L.zero_end - L.zero_begin) / 8, 0, n_word, n);
     }
 
     // Sort keys on bits [0, 3*depth) (SortKeys, DAGCompression.cu:1024-1026).
@@ -1657,7 +1660,6 @@ int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag, uint32_t** 
     a.leaves_start = total_words - 2 * dag->num_leaves;
     const uint64_t ntiles = std::max<uint64_t>(1, tiles(nodes, kFinalTileItems));
     Carver cv;
-    const uint64_t o_total = cv.take(8);
     const uint64_t o_counts = cv.take(4 * ntiles);
     const uint64_t o_offsets = cv.take(8 * ntiles);
     const uint64_t o_pos = cv.take(4 * std::max<uint64_t>(nodes, 1));
@@ -1668,9 +1670,7 @@ int svdag_create_final_dag(svdag_context* ctx, const svdag_dag* dag, uint32_t** 
     a.positions = reinterpret_cast<uint32_t*>(ctx->arena + o_pos);
     a.counts = reinterpret_cast<uint32_t*>(ctx->arena + o_counts);
     a.offsets = reinterpret_cast<uint64_t*>(ctx->arena + o_offsets);
-    uint64_t* d_total = reinterpret_cast<uint64_t*>(ctx->arena + o_total);
-    launch_fill_u64(ctx->stream, d_total, 1, nodes);
-    launch_final(ctx->stream, a, d_total);
+    launch_final(ctx->stream, a);
     const hipError_t e = hipGetLastError();
     mark(ctx, "final layout");
     if (e != hipSuccess) {

@@ -42,10 +42,12 @@ __host__ int grid_for(uint64_t tiles, int per_cu)
     return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
-__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t count, uint64_t value)
+__global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t count, uint64_t value, uint64_t set_at,
+                           uint64_t set_value)
 {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = value;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        p[i] = i == set_at ? set_value : value;
 }
 
 // Exclusive scan of counts[0, ntiles) into offsets, ntiles = ceil(*items / per_tile);
@@ -53,10 +55,10 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t count, uint64_t va
 __global__ __launch_bounds__(kScanThreads) void k_scan_counts(const uint32_t* __restrict__ counts,
                                                               uint64_t* __restrict__ offsets,
                                                               const uint64_t* __restrict__ items, uint64_t per_tile,
-                                                              uint64_t* __restrict__ total_out)
+                                                              uint64_t* __restrict__ total_out, uint64_t items_host)
 {
     __shared__ uint64_t s_scan[kScanThreads / kWave];
-    const uint64_t ntiles = (*items + per_tile - 1) / per_tile;
+    const uint64_t ntiles = ((items ? *items : items_host) + per_tile - 1) / per_tile;
     uint64_t running = 0;
     // kScanItems consecutive counts per thread per block scan.
     for (uint64_t chunk = 0; chunk < ntiles; chunk += uint64_t(kScanThreads) * kScanItems) {
@@ -1599,19 +1601,20 @@ __global__ __launch_bounds__(kRootThreads) void k_resident_root_list(const uint3
 // ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value)
+void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value, uint64_t set_at, uint64_t set_value)
 {
     if (count == 0) return;
     TimedLaunch t(kKFill, s);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((count + kThreads - 1) / kThreads, 8)), dim3(kThreads), 0, s, p, count,
-                       value);
+                       value, set_at, set_value);
 }
 
-void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
-                        uint64_t per_tile, uint64_t* total_out)
+void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items_dev,
+                        uint64_t per_tile, uint64_t* total_out, uint64_t items_host)
 {
     TimedLaunch t(kKScanCounts, s);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, s, counts, offsets, items, per_tile, total_out);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanThreads), 0, s, counts, offsets, items_dev, per_tile, total_out,
+                       items_host);
 }
 
 template <typename K>
@@ -1821,7 +1824,7 @@ void launch_query_voxels(hipStream_t s, const uint32_t* words, uint64_t n_words,
                        n_words, depth, codes, n, out);
 }
 
-void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev)
+void launch_final(hipStream_t s, const FinalArgs& a)
 {
     if (a.total_nodes == 0 && a.num_leaves == 0) return;
     if (a.total_nodes) {
@@ -1831,7 +1834,7 @@ void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes
             hipLaunchKernelGGL(k_final_count, dim3(resident_grid(k_final_count, kThreads, tiles)), dim3(kThreads), 0, s,
                                a);
         }
-        launch_scan_counts(s, a.counts, a.offsets, total_nodes_dev, kFinalTile, nullptr);
+        launch_scan_counts(s, a.counts, a.offsets, nullptr, kFinalTile, nullptr, a.total_nodes);
         {
             TimedLaunch t(kKFinalPositions, s);
             hipLaunchKernelGGL(k_final_positions, dim3(resident_grid(k_final_positions, kThreads, tiles)),

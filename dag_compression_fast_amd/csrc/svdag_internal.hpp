@@ -251,7 +251,9 @@ inline int resident_grid(F kernel, int threads, uint64_t tiles)
     return int(std::max<uint64_t>(1, std::min<uint64_t>(tiles, uint64_t(kernel_slots(kernel, threads)))));
 }
 
-void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value);
+// p[0, count) = value, except p[set_at] = set_value when set_at < count.
+void launch_fill_u64(hipStream_t s, uint64_t* p, uint64_t count, uint64_t value, uint64_t set_at = ~0ull,
+                     uint64_t set_value = 0);
 
 // *n_out = offsets[num_chunks] when the device offsets start at 0, never
 // decrease and end at most at n_bound; else *n_out = 0 and kErrOffsets in *err.
@@ -259,9 +261,10 @@ void launch_batch_count(hipStream_t s, const uint64_t* offsets, uint64_t num_chu
                         unsigned long long* err);
 
 // One-workgroup exclusive scan of tile counts into offsets; the number of tiles
-// is ceil(*items / per_tile); the total goes to *total_out when non-null.
-void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items,
-                        uint64_t per_tile, uint64_t* total_out);
+// is ceil(items / per_tile), items = *items_dev, or items_host when items_dev is
+// null; the total goes to *total_out when non-null.
+void launch_scan_counts(hipStream_t s, const uint32_t* counts, uint64_t* offsets, const uint64_t* items_dev,
+                        uint64_t per_tile, uint64_t* total_out, uint64_t items_host = 0);
 
 
 // Leaf ReduceByKey over keys grouped by code >> (6 + group_bits): per-chunk
@@ -406,8 +409,7 @@ struct FinalArgs {
     uint32_t* counts;        // tiles(total_nodes, kFinalTileItems)
     uint64_t* offsets;
 };
-// total_nodes_dev: device copy of a.total_nodes (the scan reads it there).
-void launch_final(hipStream_t s, const FinalArgs& a, const uint64_t* total_nodes_dev);
+void launch_final(hipStream_t s, const FinalArgs& a);
 
 // ---- device voxeliser ------------------------------------------------------
 // Per-triangle constants of the thin surface rule: normal n, n . v0, the
