@@ -582,6 +582,8 @@ __global__ __launch_bounds__(kThreads) void k_slot_list(const uint64_t* __restri
 // than by leaf positions and its slot arrays stay cache-sized.  Only the table
 // size depends on it; the result does not.
 // ---------------------------------------------------------------------------
+constexpr int kSketchIpt = 8;
+
 __global__ __launch_bounds__(kThreads) void k_sketch(const uint64_t* __restrict__ values,
                                                      const uint64_t* __restrict__ count_ptr,
                                                      uint32_t* __restrict__ regs)
@@ -590,14 +592,25 @@ __global__ __launch_bounds__(kThreads) void k_sketch(const uint64_t* __restrict_
     for (uint32_t i = threadIdx.x; i < uint32_t(kSketchRegs); i += kThreads) s_reg[i] = 0;
     __syncthreads();
     const uint64_t n = *count_ptr;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
-        // A mixer unrelated to the table's home-slot hash.
-        const uint64_t h = fmix64(values[q] * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
-        const uint32_t idx = uint32_t(h >> (64 - kSketchBits));
-        const uint64_t rest = h << kSketchBits;
-        const uint32_t rho = rest ? uint32_t(__clzll((long long)rest)) + 1u : uint32_t(64 - kSketchBits + 1);
-        atomicMax(&s_reg[idx], rho);
+    // kSketchIpt values per thread and round, all loaded before any is used
+    // (one workgroup per CU holds too few waves to hide one load at a time).
+    // Positions past the end read the last value again: a register maximum
+    // does not change when a value is counted twice.
+    constexpr uint64_t kRound = uint64_t(kThreads) * kSketchIpt;
+    for (uint64_t b = uint64_t(blockIdx.x) * kRound; b < n; b += uint64_t(gridDim.x) * kRound) {
+        uint64_t v[kSketchIpt];
+#pragma unroll
+        for (int i = 0; i < kSketchIpt; ++i)
+            v[i] = __builtin_nontemporal_load(values + min(b + uint64_t(i) * kThreads + threadIdx.x, n - 1));
+#pragma unroll
+        for (int i = 0; i < kSketchIpt; ++i) {
+            // A mixer unrelated to the table's home-slot hash.
+            const uint64_t h = fmix64(v[i] * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+            const uint32_t idx = uint32_t(h >> (64 - kSketchBits));
+            const uint64_t rest = h << kSketchBits;
+            const uint32_t rho = rest ? uint32_t(__clzll((long long)rest)) + 1u : uint32_t(64 - kSketchBits + 1);
+            atomicMax(&s_reg[idx], rho);
+        }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < uint32_t(kSketchRegs); i += kThreads)
@@ -1704,8 +1717,8 @@ void launch_sketch(hipStream_t s, const uint64_t* values, const uint64_t* count_
 {
     TimedLaunch t(kKSketch, s);
     // Few workgroups: each merges its registers into the global ones once.
-    hipLaunchKernelGGL(k_sketch, dim3(grid_for((n_bound + kThreads - 1) / kThreads, 1)), dim3(kThreads), 0, s, values,
-                       count_ptr, regs);
+    const uint64_t rounds = (n_bound + uint64_t(kThreads) * kSketchIpt - 1) / (uint64_t(kThreads) * kSketchIpt);
+    hipLaunchKernelGGL(k_sketch, dim3(grid_for(rounds, 1)), dim3(kThreads), 0, s, values, count_ptr, regs);
 }
 
 void launch_sketch_estimate(hipStream_t s, const uint32_t* regs, const uint64_t* bound_ptr, uint64_t* estimate)
