@@ -960,10 +960,11 @@ int build(svdag_context* ctx, const K* d_keys, uint64_t n, int depth, svdag_dag*
         a.next_count = l > 0 ? &c->pre[l - 1] : nullptr;
         a.counters = c;
         // The table is sized from the level's node count (the build of the
-        // level below, or the leaf reduce, counted it).
-        launch_table_clear(st, a.table_keys, &c->pre[l], &c->table_log2[l], ctx->load_shift, kTableOverflow,
-                           L.level_slots_of[l]);
-        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_nodes[l + 1]);
+        // level below, or the leaf reduce, counted it) and cleared by the
+        // level's count kernel.
+        const TableClearArgs clear{a.table_keys, &c->pre[l], &c->table_log2[l], ctx->load_shift, kTableOverflow,
+                                   L.level_slots_of[l]};
+        launch_level_build(st, a, l == depth - 3 ? L.leaf_bound : L.level_nodes[l + 1], &clear);
         if (ctx->check_collisions)
             launch_level_check(st, a.parent_slots, a.table_rep, a.parent_masks, a.parent_children, &c->pre[l],
                                &c->collisions, L.level_nodes[l]);

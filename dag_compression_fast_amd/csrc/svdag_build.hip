@@ -730,11 +730,22 @@ __device__ __forceinline__ void parent_runs(const uint64_t* __restrict__ codes, 
     }
 }
 
+// With clear.keys set, the kernel also clears the level's table (as
+// k_table_clear: the table's node count is not the one this kernel adds to).
 __global__ __launch_bounds__(kThreads) void k_level_count(const uint64_t* __restrict__ codes,
                                                           const uint64_t* __restrict__ n_ptr,
                                                           uint32_t* __restrict__ counts,
-                                                          uint64_t* __restrict__ next_count)
+                                                          uint64_t* __restrict__ next_count, TableClearArgs clear)
 {
+    if (clear.keys) {
+        const uint64_t cnt = *clear.count;
+        const uint32_t need = ceil_log2_u64((cnt ? cnt : 1ull) << clear.load_shift);
+        const uint32_t lg = need > 8u ? need : 8u;
+        const uint64_t total = (1ull << lg) + clear.overflow;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *clear.log2_out = lg;
+        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+        for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) clear.keys[i] = 0;
+    }
     __shared__ uint64_t s_red[kThreads / kWave];
     const uint64_t n = *n_ptr;
     const uint64_t ntiles = (n + kBuildTile - 1) / kBuildTile;
@@ -1735,13 +1746,15 @@ void launch_rank_of_slot(hipStream_t s, const uint32_t* sorted_slots, const uint
                        sorted_slots, count_ptr, rank_of_slot);
 }
 
-void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound)
+void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound, const TableClearArgs* clear)
 {
     const uint64_t tiles = (child_bound + kBuildTile - 1) / kBuildTile;
     {
         TimedLaunch t(kKLevelCount, s);
-        hipLaunchKernelGGL(k_level_count, dim3(resident_grid(k_level_count, kThreads, tiles)), dim3(kThreads), 0, s,
-                           a.child_codes, a.child_count, a.tile_counts, a.next_count);
+        const uint64_t work = clear ? std::max(tiles, (clear->slot_bound + kThreads - 1) / kThreads) : tiles;
+        hipLaunchKernelGGL(k_level_count, dim3(resident_grid(k_level_count, kThreads, work)), dim3(kThreads), 0, s,
+                           a.child_codes, a.child_count, a.tile_counts, a.next_count,
+                           clear ? *clear : TableClearArgs{});
     }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.child_count, kBuildTile, nullptr);
     {

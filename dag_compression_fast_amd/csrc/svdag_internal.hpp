@@ -333,7 +333,18 @@ struct LevelBuildArgs {
     uint64_t* next_count;     // M of the level above (atomic adds), or null
     BuildCounters* counters;
 };
-void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound);
+// The level's table clear (launch_table_clear's arguments), done by the
+// level's count kernel instead of a launch of its own.
+struct TableClearArgs {
+    uint64_t* keys;
+    const uint64_t* count;  // the level's node count (sizes the table)
+    uint32_t* log2_out;
+    uint32_t load_shift;
+    uint32_t overflow;
+    uint64_t slot_bound;
+};
+void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_bound,
+                        const TableClearArgs* clear = nullptr);
 
 // ---- small top levels -------------------------------------------------------
 // Levels 0..top with top <= kTailTopLevel hold at most 8^top <= 512 nodes; one
