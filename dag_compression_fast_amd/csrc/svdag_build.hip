@@ -1752,9 +1752,9 @@ void launch_level_build(hipStream_t s, const LevelBuildArgs& a, uint64_t child_b
     {
         TimedLaunch t(kKLevelCount, s);
         const uint64_t work = clear ? std::max(tiles, (clear->slot_bound + kThreads - 1) / kThreads) : tiles;
+        const TableClearArgs ca = clear ? *clear : TableClearArgs{};  // by value: *clear may not outlive the call
         hipLaunchKernelGGL(k_level_count, dim3(resident_grid(k_level_count, kThreads, work)), dim3(kThreads), 0, s,
-                           a.child_codes, a.child_count, a.tile_counts, a.next_count,
-                           clear ? *clear : TableClearArgs{});
+                           a.child_codes, a.child_count, a.tile_counts, a.next_count, ca);
     }
     launch_scan_counts(s, a.tile_counts, const_cast<uint64_t*>(a.offsets), a.child_count, kBuildTile, nullptr);
     {
