@@ -706,7 +706,7 @@ __device__ __forceinline__ uint32_t ordered_insert(uint64_t* __restrict__ tk, ui
 // slots) -> parents.  A parent belongs to the thread holding its first child and
 // has at most 8 children, so a run extends at most 7 keys past the thread.
 // ---------------------------------------------------------------------------
-constexpr int kBuildIpt = 8;
+constexpr int kBuildIpt = 4;
 constexpr int kBuildTile = kThreads * kBuildIpt;
 
 __device__ __forceinline__ void parent_runs(const uint64_t* __restrict__ codes, uint64_t n, uint64_t first,
