@@ -473,7 +473,7 @@ __device__ __forceinline__ uint32_t elect_leader(uint64_t key, bool valid, uint3
 // per-wave LDS bucket chosen by its mask (one write wins), and a lane whose
 // bucket winner holds the same mask takes the winner's slot instead of probing.
 // Lanes that lose their bucket to another mask probe themselves.
-constexpr int kInsertIpt = 4;
+constexpr int kInsertIpt = 8;
 
 __global__ __launch_bounds__(kThreads) void k_leaf_insert(const uint64_t* __restrict__ lp_mask,
                                                           const uint64_t* __restrict__ count_ptr,
@@ -1103,7 +1103,7 @@ __global__ void k_level_check(const uint32_t* __restrict__ parent_slots, const u
 // Slots are read through an LDS window with a halo; clusters reaching past it
 // fall back to global loads.
 // ---------------------------------------------------------------------------
-constexpr int kExtractIpt = 2;
+constexpr int kExtractIpt = 1;
 constexpr int kExtractTile = kThreads * kExtractIpt;
 constexpr int kHalo = 128;
 

@@ -227,7 +227,7 @@ constexpr uint64_t kPackTileItems = 4096;     // grouped keys per leaf-reduce ch
 constexpr uint32_t kLeafGroupBits = 9;        // low leaf-code bits the key sort leaves unsorted
 constexpr uint64_t kLeafRegionItems = kPackTileItems + (1u << kLeafGroupBits);  // leaves one chunk may emit
 constexpr uint64_t kBuildTileItems = 1024;    // child codes per level-build tile
-constexpr uint64_t kExtractTileItems = 512;   // table slots per extract tile
+constexpr uint64_t kExtractTileItems = 256;   // table slots per extract tile
 constexpr uint64_t kFinalTileItems = 4096;    // nodes per final-layout tile
 constexpr uint64_t kListTileItems = 4096;     // table slots per slot-list tile
 inline uint64_t tiles(uint64_t n, uint64_t per) { return (n + per - 1) / per; }
