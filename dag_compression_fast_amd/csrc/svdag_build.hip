@@ -298,24 +298,22 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
             const uint64_t next = base + uint64_t(G) * kWave;
             const bool prefetch = next < c1;
             if (prefetch) load_groups(sorted, next, n, nxt);
-            // Group starts of the whole batch first.  Keys are sorted by group, so
-            // a batch group whose first and last lanes agree has no start inside
-            // it (lane 0 may still start one); only the others need the lane
-            // shuffle.  Lanes past the end repeat the last key: they never start
-            // a group, and OR-ing their bit again is harmless.
+            // Group starts of the whole batch first: a lane starts a group when
+            // its group index differs from the previous lane's (lane 0: from
+            // the previous batch group's last lane).  A shuffle and a ballot
+            // for every batch group: the shortcut of comparing the first and
+            // last lanes (no start inside when they agree) cost more scalar
+            // work and a branch than it saved.  Lanes past the end repeat the
+            // last key: they never start a group, and OR-ing their bit again is
+            // harmless.
             uint64_t H[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const KC b = gb[g];
-                const KC b0 = readlane_of(b, 0), b63 = readlane_of(b, kWave - 1);
-                if (b0 == b63) {
-                    H[g] = b0 != prev_b ? 1ull : 0ull;
-                } else {
-                    KC pb = shfl_up_of(b);
-                    if (lane == 0) pb = prev_b;
-                    H[g] = __ballot(b != pb);
-                }
-                prev_b = b63;
+                KC pb = shfl_up_of(b);
+                if (lane == 0) pb = prev_b;
+                H[g] = __ballot(b != pb);
+                prev_b = readlane_of(b, kWave - 1);
             }
 #pragma unroll
             for (int g = 0; g < G; ++g) {
