@@ -315,40 +315,49 @@ __global__ __launch_bounds__(kThreads) void k_leaf_reduce(const K* __restrict__ 
                 H[g] = __ballot(b != pb);
                 prev_b = readlane_of(b, kWave - 1);
             }
+            // A batch that ends before both the chunk's and the keys' end (all
+            // but the last of a chunk) cannot end the wave's work: FAST drops
+            // the per-group end checks (scalar work; the scalar unit is this
+            // kernel's limit).
+            auto reduce_batch = [&](auto fast_tag) {
+                constexpr bool FAST = decltype(fast_tag)::value;
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                if (done) break;
-                const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
-                const KC key = k[g];
-                uint64_t* const word = &tab[uint32_t(key >> 6) & low_mask];
-                const uint64_t bit = 1ull << (uint32_t(key) & 63u);
-                uint64_t heads = H[g];
-                const bool keys_end = gi + kWave >= n;  // the last key is in this batch group
-                // Segments [lo, next start), then [lo, 64): lanes before the first
-                // start continue the open group, or one owned by an earlier wave
-                // when none is open.  One flush site (the open group ends at a
-                // start, or at the keys' end): every call site is a copy of it.
-                uint32_t lo = 0;
-                while (true) {
-                    const uint32_t hi = heads ? uint32_t(__ffsll((long long)heads) - 1) : uint32_t(kWave);
-                    if (open && lane >= lo && lane < hi) atomicOr((unsigned long long*)word, bit);
-                    const bool last = hi == uint32_t(kWave);
-                    if (open && (!last || keys_end)) flush();
-                    if (last) {
-                        // The keys end here, or no group starts in the chunk.
-                        if (keys_end || (!open && gi + kWave >= c1)) done = true;
-                        break;
+                for (int g = 0; g < G; ++g) {
+                    if (!FAST && done) break;
+                    const uint64_t gi = base + uint64_t(g) * kWave;  // index of lane 0's key
+                    const KC key = k[g];
+                    uint64_t* const word = &tab[uint32_t(key >> 6) & low_mask];
+                    const uint64_t bit = 1ull << (uint32_t(key) & 63u);
+                    uint64_t heads = H[g];
+                    const bool keys_end = !FAST && gi + kWave >= n;  // the last key is in this batch group
+                    // Segments [lo, next start), then [lo, 64): lanes before the first
+                    // start continue the open group, or one owned by an earlier wave
+                    // when none is open.  One flush site (the open group ends at a
+                    // start, or at the keys' end): every call site is a copy of it.
+                    uint32_t lo = 0;
+                    while (true) {
+                        const uint32_t hi = heads ? uint32_t(__ffsll((long long)heads) - 1) : uint32_t(kWave);
+                        if (open && lane >= lo && lane < hi) atomicOr((unsigned long long*)word, bit);
+                        const bool last = hi == uint32_t(kWave);
+                        if (open && (!last || keys_end)) flush();
+                        if (last) {
+                            // The keys end here, or no group starts in the chunk.
+                            if (!FAST && (keys_end || (!open && gi + kWave >= c1))) done = true;
+                            break;
+                        }
+                        if (!FAST && gi + hi >= c1) {
+                            done = true;
+                            break;
+                        }
+                        open = true;
+                        cur_b = readlane_of(gb[g], hi);
+                        heads &= heads - 1ull;
+                        lo = hi;
                     }
-                    if (gi + hi >= c1) {
-                        done = true;
-                        break;
-                    }
-                    open = true;
-                    cur_b = readlane_of(gb[g], hi);
-                    heads &= heads - 1ull;
-                    lo = hi;
                 }
-            }
+            };
+            if (next < c1 && next < n) reduce_batch(std::true_type{});
+            else reduce_batch(std::false_type{});
             if (done) break;
             if (!prefetch) load_groups(sorted, next, n, nxt);
             // Settle every load here (vmcnt 0, other counters left alone): the
